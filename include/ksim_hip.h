@@ -1,0 +1,217 @@
+/*
+ * ksim_hip.h — C ABI of the MI355X (gfx950) FlatProgram sampler.
+ *
+ * Drop-in native layer for the reference's `kokkos_sim` module
+ * (src/cpp/ksim/kokkos_sim.cpp:244-252, bound by nanobind):
+ *
+ *   reference (nanobind)                      this ABI
+ *   ----------------------------------------  -------------------------------------
+ *   Component(graphs: list[dict], num_f)      ksim_component_create
+ *     kokkos_sim.cpp:67-151
+ *   Component.sample(f_sel, seed)             ksim_component_sample
+ *     kokkos_sim.cpp:188-240
+ *   Component.eval_abs(graph, params)         ksim_component_eval_abs
+ *     kokkos_sim.cpp:157-186
+ *   Component.n_out                           ksim_component_n_out
+ *     kokkos_sim.cpp:250-251
+ *   (Python GC of Component)                  ksim_program_destroy
+ *
+ * plus whole-program entry points that replace the per-component Python loop
+ * of KokkosProgramSampler.sample (src/python/ksim/sample/sampler.py:57-67)
+ * with one launch, a fed-uniforms variant matching sample_flat's draw order
+ * (src/python/ksim/sample/evaluate.py:73-82), an exact evaluator for parity
+ * checks, device-pointer variants for resident inputs, and a batched
+ * persistent launcher over many programs.
+ *
+ * Conventions: every function returns 0 on success and a negative error code
+ * otherwise; ksim_last_error() returns the message of the last failure on the
+ * calling thread.  Host pointers are borrowed for the duration of the call
+ * only (the library copies what it keeps).  Calls are synchronous unless they
+ * take a `stream` argument.  Parameter and mask bytes are read by their
+ * lowest bit.
+ */
+#ifndef KSIM_HIP_H
+#define KSIM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KSIM_OK 0
+#define KSIM_E_INVALID -1     /* bad argument (reference: std::invalid_argument) */
+#define KSIM_E_UNSUPPORTED -2 /* program outside the exact range / features   */
+#define KSIM_E_DEVICE -3      /* HIP runtime error                            */
+#define KSIM_E_RANGE -4       /* a device-side exact-range check failed        */
+
+/* One FlatScalarGraph, host memory, C-contiguous; the 19 arrays of the
+ * reference's _graph_to_dict (src/python/ksim/sample/sampler.py:21-43).
+ * Shapes: G terms, P params, T_* padded family widths. */
+typedef struct ksim_graph_view {
+  int32_t G, P, T_np, T_hp, T_pp, T_ph;
+  const uint8_t *np_phases;      /* (G, T_np) */
+  const uint8_t *np_params;      /* (G, T_np, P) */
+  const int32_t *np_counts;      /* (G) */
+  const uint8_t *hp_coeffs;      /* (G, T_hp) */
+  const uint8_t *hp_params;      /* (G, T_hp, P) */
+  const uint8_t *pp_psi_c;       /* (G, T_pp) */
+  const uint8_t *pp_psi_p;       /* (G, T_pp, P) */
+  const uint8_t *pp_phi_c;       /* (G, T_pp) */
+  const uint8_t *pp_phi_p;       /* (G, T_pp, P) */
+  const uint8_t *ph_alpha;       /* (G, T_ph) */
+  const uint8_t *ph_alpha_p;     /* (G, T_ph, P) */
+  const uint8_t *ph_beta;        /* (G, T_ph) */
+  const uint8_t *ph_beta_p;      /* (G, T_ph, P) */
+  const int32_t *ph_counts;      /* (G) */
+  const uint8_t *pf_phase_idx;   /* (G) */
+  const int32_t *pf_floatfactor; /* (G, 4) */
+  const int32_t *pf_power2;      /* (G) */
+  const float *pf_approx_f32;    /* (G, 2) complex64 as float pairs */
+  int32_t pf_has_approx;
+} ksim_graph_view;
+
+typedef struct ksim_program ksim_program;
+
+/* ---- kokkos_sim.Component equivalents --------------------------------- */
+
+/* Build and upload one component from its n_graphs = n_out + 1 graphs.
+ * Replaces the Component constructor (kokkos_sim.cpp:67-151). */
+int ksim_component_create(const ksim_graph_view *graphs, int32_t n_graphs,
+                          int32_t num_f, ksim_program **out);
+
+/* Sample B shots: f_sel (B, num_f) u8 row-major -> out (B, max(n_out,1)) u8.
+ * Same seeding and uniform stream as kokkos_sim.cpp:214-229 (xorshift64 per
+ * shot).  num_f mismatch is checked by the caller binding (as the reference
+ * does at kokkos_sim.cpp:193-194); here f_cols must equal num_f. */
+int ksim_component_sample(ksim_program *p, const uint8_t *f_sel, int64_t B,
+                          int32_t f_cols, uint64_t seed, uint8_t *out);
+
+/* |amplitude| of graph `graph` for explicit param rows (B, P) u8 -> out (B)
+ * float64.  Replaces Component.eval_abs (kokkos_sim.cpp:157-186). */
+int ksim_component_eval_abs(ksim_program *p, int32_t graph,
+                            const uint8_t *params, int64_t B, int32_t P,
+                            double *out);
+
+/* n_out of component 0 (kokkos_sim.cpp:250-251). */
+int32_t ksim_component_n_out(const ksim_program *p);
+
+/* ---- whole-program entry points --------------------------------------- */
+
+/* Build and upload every component of a FlatProgram.
+ *   graphs            all graphs, component-major (sum(graphs_per_comp))
+ *   graphs_per_comp   (n_comp) n_out_c + 1
+ *   f_selection       concatenated f_selection arrays (sum(F_per_comp))
+ *   F_per_comp        (n_comp)
+ *   num_f             columns of the f_params matrix the program needs at
+ *                     least (1 + the largest f_selection entry); calls pass
+ *                     the actual column count f_cols >= num_f
+ *   dest_col          (sum n_out_c) output column of each component output,
+ *                     i.e. the inverse of argsort(output_order)
+ *                     (src/python/ksim/sample/sampler.py:65-67)
+ *   num_outputs       columns of the result */
+int ksim_program_create(const ksim_graph_view *graphs,
+                        const int32_t *graphs_per_comp,
+                        const int32_t *f_selection, const int32_t *F_per_comp,
+                        int32_t n_comp, int32_t num_f, const int32_t *dest_col,
+                        int32_t num_outputs, ksim_program **out);
+
+/* f_params (B, f_cols) u8 -> out (B, num_outputs) u8; f_cols >= num_f.  Component k draws its
+ * uniforms from seed + 0x9E37*k, as KokkosProgramSampler.sample does
+ * (sampler.py:63). */
+int ksim_program_sample(ksim_program *p, const uint8_t *f_params, int64_t B,
+                        int32_t f_cols, uint64_t seed, uint8_t *out);
+
+/* As ksim_program_sample but with fed uniforms U (B, sum n_out_c) float64,
+ * columns component-major then output-major: the order sample_flat draws
+ * rng.random(B) in (evaluate.py:73-82). */
+int ksim_program_sample_uniforms(ksim_program *p, const uint8_t *f_params,
+                                 int64_t B, int32_t f_cols, const double *U,
+                                 uint8_t *out);
+
+/* Exact value of graph `graph` of component `comp` at param rows (B, P) u8:
+ * coeffs (B, 4, 2) int64 as (lo, hi) words of four signed 128-bit integers in
+ * the basis (1, w, i, conj w), exp2 (B): value = coeffs * 2^exp2.  Not
+ * canonicalised.  No reference counterpart: the parity-check entry point. */
+int ksim_program_eval_exact(ksim_program *p, int32_t comp, int32_t graph,
+                            const uint8_t *params, int64_t B, int32_t P,
+                            int64_t *coeffs, int32_t *exp2);
+
+/* Device-pointer sampling for inputs already resident in HBM.  d_f (B, f_cols)
+ * u8, d_U optional (NULL: seeded stream), d_out (B, num_outputs) u8.
+ * shot_base offsets the shot index that keys the per-shot stream, so a range
+ * split of one request samples exactly what a single call would.  Enqueued
+ * on `stream` (hipStream_t, NULL = default); returns without waiting. */
+int ksim_program_sample_device(ksim_program *p, const uint8_t *d_f, int64_t B,
+                               int32_t f_cols, int64_t shot_base, uint64_t seed,
+                               const double *d_U, uint8_t *d_out, void *stream);
+
+/* Launch shape of the plain kernel: threads per block (64..1024, multiple of
+ * 64) and waves per shot group (1 = each wave evaluates all terms of its 64
+ * shots; k > 1 = k waves split each graph's terms and reduce through LDS). */
+int ksim_program_set_tiles(ksim_program *p, int32_t block_threads,
+                           int32_t term_split);
+
+/* Error word of the last launch (0 = none), for callers of the async API. */
+int ksim_program_check(ksim_program *p, void *stream);
+
+/* Bytes of device memory the program holds. */
+int64_t ksim_program_device_bytes(const ksim_program *p);
+
+/* Stats of the packed program: [n_graphs, n_terms, n_records, words,
+ * ops_per_shot (algorithmic int64 ops per shot, all components)]. */
+int ksim_program_stats(const ksim_program *p, double *stats5);
+
+void ksim_program_destroy(ksim_program *p);
+
+/* ---- batched persistent path ------------------------------------------ */
+
+typedef struct ksim_batch ksim_batch;
+
+/* A batch of n programs sampled by one persistent kernel.  Each program k
+ * gets its own resident f matrix d_f[k] (B[k], f_cols[k]) u8 and output
+ * d_out[k] (B[k], num_outputs_k) u8 and seed[k]. */
+int ksim_batch_create(ksim_program *const *progs, int32_t n,
+                      const uint8_t *const *d_f, const int32_t *f_cols,
+                      const int64_t *B,
+                      const uint64_t *seed, uint8_t *const *d_out,
+                      int32_t wave_shots, ksim_batch **out);
+
+/* Run one batched launch on `stream`.  use_graph != 0 captures the launch
+ * (work-list reset + persistent kernel) into a hipGraph on first use and
+ * replays it afterwards. */
+int ksim_batch_run(ksim_batch *b, int32_t persistent_blocks, int32_t use_graph,
+                   void *stream);
+
+int ksim_batch_check(ksim_batch *b);
+void ksim_batch_destroy(ksim_batch *b);
+
+/* ---- host-only inspection (no device needed) ---------------------------- */
+
+/* Pack a program exactly as ksim_program_create does, without uploading:
+ * writes up to `cap` words to `words` and the full word count to *n_words.
+ * No reference counterpart: lets the packer be checked on a CPU-only host. */
+int ksim_program_pack(const ksim_graph_view *graphs, const int32_t *graphs_per_comp,
+                      const int32_t *f_selection, const int32_t *F_per_comp,
+                      int32_t n_comp, int32_t num_f, const int32_t *dest_col,
+                      int32_t num_outputs, uint32_t *words, int64_t cap,
+                      int64_t *n_words);
+
+/* Copy the kernels' lookup tables: tab (4*TAB_N int32) and pitab
+ * (4*PI_MAX*PI_MAX int32); sizes[0..1] receive TAB_R and PI_MAX. */
+int ksim_tables(int32_t *tab, int64_t tab_cap, int32_t *pitab, int64_t pitab_cap,
+                int32_t *sizes2);
+
+/* ---- misc --------------------------------------------------------------- */
+
+const char *ksim_last_error(void);
+
+/* Device properties used by the tuner: [CUs, LDS bytes/CU, clock kHz,
+ * HBM bytes]. */
+int ksim_device_info(int64_t *info4);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KSIM_HIP_H */

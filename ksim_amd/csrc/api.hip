@@ -1,0 +1,736 @@
+// api.hip — kernels, launchers and the C ABI of include/ksim_hip.h.
+//
+// Runtime layer of the sampler: owns the device copy of each packed program
+// (the reference keeps four Kokkos Views per component,
+// src/cpp/ksim/kokkos_sim.cpp:49-57,132-148), stages per-call inputs, launches
+// the gfx950 kernels and reports device-side range errors.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/ksim_hip.h"
+#include "device.hpp"
+#include "layout.hpp"
+#include "pack.hpp"
+
+using namespace ksim;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct DeviceError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct RangeError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+void hip_check(hipError_t e, const char *what) {
+  if (e != hipSuccess)
+    throw DeviceError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class F>
+int guarded(F &&f) {
+  try {
+    f();
+    return KSIM_OK;
+  } catch (const std::invalid_argument &e) {
+    g_last_error = e.what();
+    return KSIM_E_INVALID;
+  } catch (const RangeError &e) {
+    g_last_error = e.what();
+    return KSIM_E_RANGE;
+  } catch (const DeviceError &e) {
+    g_last_error = e.what();
+    return KSIM_E_DEVICE;
+  } catch (const std::exception &e) {
+    g_last_error = e.what();
+    return KSIM_E_UNSUPPORTED;
+  }
+}
+
+// Per-device constant tables (materialisation and pi powers).
+struct DevTables {
+  int4 *tab = nullptr;
+  int4 *pitab = nullptr;
+};
+std::mutex g_tab_mu;
+std::map<int, DevTables> g_tabs;
+
+DevTables tables_for_current_device() {
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  std::lock_guard<std::mutex> lk(g_tab_mu);
+  auto it = g_tabs.find(dev);
+  if (it != g_tabs.end()) return it->second;
+  DevTables t;
+  hip_check(hipMalloc(&t.tab, sizeof(int4) * TAB_N), "hipMalloc(tab)");
+  hip_check(hipMemcpy(t.tab, materialise_table(), sizeof(int4) * TAB_N, hipMemcpyHostToDevice),
+            "upload tab");
+  hip_check(hipMalloc(&t.pitab, sizeof(int4) * PI_MAX * PI_MAX), "hipMalloc(pitab)");
+  hip_check(hipMemcpy(t.pitab, pi_table(), sizeof(int4) * PI_MAX * PI_MAX,
+                      hipMemcpyHostToDevice),
+            "upload pitab");
+  g_tabs[dev] = t;
+  return t;
+}
+
+std::string err_text(uint32_t e) {
+  std::string s = "device exact-range check failed:";
+  if (e & ERR_TAB_RANGE) s += " (1+sqrt2) exponent beyond the table";
+  if (e & ERR_PI_RANGE) s += " pi exponent beyond the table";
+  if (e & ERR_SHIFT) s += " alignment shift beyond the 128-bit sum";
+  return s;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ kernels
+namespace ksim {
+
+constexpr int kBlock = 256;
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_fill_u8(uint8_t *p, uint8_t v, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// Plain path: one lane per shot, grid (ceil(B/256), n_comp).
+template <int WL>
+__global__ __launch_bounds__(kBlock) void k_sample(ShotArgs a, const int4 *__restrict__ gtab,
+                                                  const int4 *__restrict__ pitab) {
+  __shared__ int4 s_tab[TAB_N];
+  load_table(s_tab, gtab);
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.B) return;
+  run_shot<WL>(a, (int)blockIdx.y, s, s_tab, pitab);
+}
+
+// Explicit parameter rows: exact value (mode 0) or |value| (mode 1) of one graph.
+struct EvalArgs {
+  const uint32_t *prog;
+  const uint32_t *gh;
+  const uint8_t *params;  // (B, P)
+  int64_t B;
+  int32_t P, mode;
+  int64_t *coeffs;  // (B, 4, 2)
+  int32_t *exp2;    // (B)
+  double *absout;   // (B)
+  uint32_t *err;
+};
+
+template <int WL>
+__global__ __launch_bounds__(kBlock) void k_eval(EvalArgs a, const int4 *__restrict__ gtab,
+                                                const int4 *__restrict__ pitab) {
+  __shared__ int4 s_tab[TAB_N];
+  load_table(s_tab, gtab);
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.B) return;
+  const uint8_t *row = a.params + s * (int64_t)a.P;
+  uint32_t pv[WL];
+#pragma unroll
+  for (int w = 0; w < WL; ++w) {
+    uint32_t word = 0;
+    const int j0 = 32 * w, jn = min(32, a.P - j0);
+    for (int jj = 0; jj < jn; ++jj) word |= (uint32_t)(row[j0 + jj] & 1u) << jj;
+    pv[w] = word;
+  }
+  Acc4 acc;
+  uint32_t err = 0;
+  eval_graph_any<WL>(a.prog, a.gh, pv, s_tab, pitab, acc, err);
+  const int e0 = (int)a.gh[GH_E0];
+  if (a.mode == 0) {
+    int64_t *c = a.coeffs + s * 8;
+    const i128_t v[4] = {acc.a, acc.b, acc.c, acc.d};
+    for (int k = 0; k < 4; ++k) {
+      c[2 * k] = (int64_t)(uint64_t)(u128_t)v[k];
+      c[2 * k + 1] = (int64_t)(v[k] >> 64);
+    }
+    a.exp2[s] = e0;
+  } else {
+    a.absout[s] = magnitude(acc, e0);
+  }
+  if (err) atomicOr(a.err, err);
+}
+
+// Batched persistent path: each wave pulls 64-shot work items from a
+// device-wide cursor until the list is drained.
+struct BatchProg {
+  const uint32_t *prog;
+  const uint8_t *f;
+  uint8_t *out;
+  uint64_t seed;
+  int64_t B;
+  int32_t num_f, num_outputs, tot_out, wmax;
+};
+struct BatchItem {
+  int32_t prog, comp;
+  int64_t shot0;
+};
+
+__global__ void k_reset_cursor(uint32_t *cursor, uint32_t *err) {
+  if (threadIdx.x == 0) {
+    cursor[0] = 0;
+    err[0] = 0;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_batch(const BatchProg *__restrict__ progs,
+                                                  const BatchItem *__restrict__ items,
+                                                  uint32_t n_items, uint32_t *cursor,
+                                                  uint32_t *err, int32_t wave_shots,
+                                                  const int4 *__restrict__ gtab,
+                                                  const int4 *__restrict__ pitab) {
+  __shared__ int4 s_tab[TAB_N];
+  load_table(s_tab, gtab);
+  const int lane = threadIdx.x & 63;
+  for (;;) {
+    uint32_t idx = 0;
+    if (lane == 0) idx = atomicAdd(cursor, 1u);
+    idx = __builtin_amdgcn_readfirstlane(__shfl(idx, 0));
+    if (idx >= n_items) break;
+    const BatchItem it = items[idx];
+    const BatchProg bp = progs[it.prog];
+    ShotArgs a;
+    a.prog = bp.prog; a.f = bp.f; a.U = nullptr; a.out = bp.out; a.err = err;
+    a.B = bp.B; a.shot_base = 0; a.seed = bp.seed;
+    a.num_f = bp.num_f; a.num_outputs = bp.num_outputs; a.tot_out = bp.tot_out; a.pad = 0;
+    for (int sub = 0; sub < wave_shots; sub += 64) {
+      const int64_t s = it.shot0 + sub + lane;
+      if (s < bp.B) {
+        switch (bp.wmax) {
+          case 1: run_shot<1>(a, it.comp, s, s_tab, pitab); break;
+          case 2: run_shot<2>(a, it.comp, s, s_tab, pitab); break;
+          case 4: run_shot<4>(a, it.comp, s, s_tab, pitab); break;
+          default: run_shot<8>(a, it.comp, s, s_tab, pitab); break;
+        }
+      }
+    }
+  }
+}
+
+}  // namespace ksim
+
+// ------------------------------------------------------------------ objects
+struct ksim_program {
+  int device = 0;
+  uint32_t *d_prog = nullptr;
+  size_t n_words = 0;
+  uint32_t *d_err = nullptr;
+  int n_comp = 0, num_f = 0, num_outputs = 0, tot_out = 0, wmax = 1, wl = 1;
+  std::vector<int> n_out, F, n_graphs;
+  std::vector<std::vector<uint32_t>> graph_hdr;  // host copy of graph headers per comp
+  PackStats st;
+  double ops_per_shot = 0;
+  int block_threads = kBlock;
+  int term_split = 1;
+  // staging for the host-pointer API
+  uint8_t *d_f = nullptr;
+  size_t f_cap = 0;
+  uint8_t *d_out = nullptr;
+  size_t out_cap = 0;
+  double *d_U = nullptr;
+  size_t U_cap = 0;
+  uint8_t *d_params = nullptr;
+  size_t params_cap = 0;
+  void *d_res = nullptr;
+  size_t res_cap = 0;
+};
+
+namespace {
+
+int lane_words(int wmax) { return wmax <= 1 ? 1 : wmax <= 2 ? 2 : wmax <= 4 ? 4 : 8; }
+
+template <class T>
+void ensure(T *&p, size_t &cap, size_t bytes) {
+  if (bytes <= cap && p) return;
+  if (p) hip_check(hipFree(p), "hipFree");
+  p = nullptr;
+  cap = 0;
+  hip_check(hipMalloc((void **)&p, std::max<size_t>(bytes, 64)), "hipMalloc(staging)");
+  cap = std::max<size_t>(bytes, 64);
+}
+
+void fill_u8_async(uint8_t *p, uint8_t v, int64_t n, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill_u8, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, v, n);
+  hip_check(hipGetLastError(), "k_fill_u8");
+}
+
+void reset_err_async(ksim_program *p, hipStream_t st) {
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, p->d_err, 0u, (int64_t)1);
+  hip_check(hipGetLastError(), "k_fill_u32");
+}
+
+void check_err_sync(ksim_program *p, hipStream_t st) {
+  uint32_t e = 0;
+  hip_check(hipMemcpyAsync(&e, p->d_err, sizeof(e), hipMemcpyDeviceToHost, st), "read err");
+  hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+  if (e) throw RangeError(err_text(e));
+}
+
+void launch_sample(ksim_program *p, const uint8_t *d_f, int64_t B, int32_t f_cols,
+                   int64_t shot_base, uint64_t seed, const double *d_U, uint8_t *d_out,
+                   hipStream_t st) {
+  if (f_cols < p->num_f) throw std::invalid_argument("f_params has fewer columns than f_selection needs");
+  if (B <= 0 || p->n_comp == 0) return;
+  DevTables t = tables_for_current_device();
+  ShotArgs a;
+  a.prog = p->d_prog; a.f = d_f; a.U = d_U; a.out = d_out; a.err = p->d_err;
+  a.B = B; a.shot_base = shot_base; a.seed = seed;
+  a.num_f = f_cols; a.num_outputs = p->num_outputs; a.tot_out = p->tot_out; a.pad = 0;
+  const int64_t nb = (B + kBlock - 1) / kBlock;
+  if (nb > 0x7fffffff) throw std::invalid_argument("too many shots for one launch");
+  dim3 grid((unsigned)nb, (unsigned)p->n_comp);
+  switch (p->wl) {
+    case 1: hipLaunchKernelGGL(k_sample<1>, grid, dim3(kBlock), 0, st, a, t.tab, t.pitab); break;
+    case 2: hipLaunchKernelGGL(k_sample<2>, grid, dim3(kBlock), 0, st, a, t.tab, t.pitab); break;
+    case 4: hipLaunchKernelGGL(k_sample<4>, grid, dim3(kBlock), 0, st, a, t.tab, t.pitab); break;
+    default: hipLaunchKernelGGL(k_sample<8>, grid, dim3(kBlock), 0, st, a, t.tab, t.pitab); break;
+  }
+  hip_check(hipGetLastError(), "k_sample launch");
+}
+
+ksim_program *create_program(const ksim_graph_view *graphs, const int32_t *graphs_per_comp,
+                             const int32_t *f_selection, const int32_t *F_per_comp,
+                             int32_t n_comp, int32_t num_f, const int32_t *dest_col,
+                             int32_t num_outputs) {
+  if (!graphs && n_comp > 0) throw std::invalid_argument("graphs is NULL");
+  PackStats st;
+  std::vector<uint32_t> words = pack_program(graphs, graphs_per_comp, f_selection, F_per_comp,
+                                             n_comp, num_f, dest_col, num_outputs, st);
+  auto *p = new ksim_program();
+  try {
+    hip_check(hipGetDevice(&p->device), "hipGetDevice");
+    p->n_words = words.size();
+    hip_check(hipMalloc(&p->d_prog, words.size() * 4), "hipMalloc(program)");
+    hip_check(hipMemcpy(p->d_prog, words.data(), words.size() * 4, hipMemcpyHostToDevice),
+              "upload program");
+    hip_check(hipMalloc(&p->d_err, 64), "hipMalloc(err)");
+    hip_check(hipMemset(p->d_err, 0, 64), "hipMemset(err)");
+    p->n_comp = n_comp;
+    p->num_f = num_f;
+    p->num_outputs = num_outputs;
+    p->tot_out = (int)words[PH_TOTOUT];
+    p->wmax = st.wmax;
+    p->wl = lane_words(st.wmax);
+    p->st = st;
+    for (int c = 0; c < n_comp; ++c) {
+      const uint32_t *ch = &words[words[PH_DWORDS + c]];
+      p->n_graphs.push_back((int)ch[CH_NGRAPHS]);
+      p->n_out.push_back((int)ch[CH_NOUT]);
+      p->F.push_back((int)ch[CH_F]);
+      std::vector<uint32_t> gh(&words[ch[CH_GRAPH_OFF]],
+                               &words[ch[CH_GRAPH_OFF]] + (size_t)ch[CH_NGRAPHS] * GH_DWORDS);
+      p->graph_hdr.push_back(std::move(gh));
+    }
+    tables_for_current_device();
+  } catch (...) {
+    ksim_program_destroy(p);
+    throw;
+  }
+  return p;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ C ABI
+extern "C" {
+
+const char *ksim_last_error(void) { return g_last_error.c_str(); }
+
+int ksim_program_create(const ksim_graph_view *graphs, const int32_t *graphs_per_comp,
+                        const int32_t *f_selection, const int32_t *F_per_comp, int32_t n_comp,
+                        int32_t num_f, const int32_t *dest_col, int32_t num_outputs,
+                        ksim_program **out) {
+  return guarded([&] {
+    if (!out) throw std::invalid_argument("out is NULL");
+    *out = create_program(graphs, graphs_per_comp, f_selection, F_per_comp, n_comp, num_f,
+                          dest_col, num_outputs);
+  });
+}
+
+int ksim_component_create(const ksim_graph_view *graphs, int32_t n_graphs, int32_t num_f,
+                          ksim_program **out) {
+  return guarded([&] {
+    if (!out) throw std::invalid_argument("out is NULL");
+    if (n_graphs < 1) throw std::invalid_argument("a component needs at least one graph");
+    if (num_f < 0) throw std::invalid_argument("num_f < 0");
+    std::vector<int32_t> fsel(num_f);
+    for (int j = 0; j < num_f; ++j) fsel[j] = j;
+    const int n_out = n_graphs - 1;
+    std::vector<int32_t> dest(std::max(n_out, 0));
+    for (int j = 0; j < n_out; ++j) dest[j] = j;
+    const int32_t gpc = n_graphs, F = num_f;
+    *out = create_program(graphs, &gpc, fsel.data(), &F, 1, num_f, dest.data(),
+                          std::max(n_out, 1));
+  });
+}
+
+int32_t ksim_component_n_out(const ksim_program *p) {
+  return (p && p->n_comp > 0) ? p->n_out[0] : -1;
+}
+
+int ksim_program_sample_device(ksim_program *p, const uint8_t *d_f, int64_t B, int32_t f_cols,
+                               int64_t shot_base, uint64_t seed, const double *d_U,
+                               uint8_t *d_out, void *stream) {
+  return guarded([&] {
+    if (!p) throw std::invalid_argument("program is NULL");
+    if (B < 0) throw std::invalid_argument("B < 0");
+    if (B > 0 && ((!d_f && f_cols > 0) || !d_out)) throw std::invalid_argument("NULL buffer");
+    hipStream_t st = (hipStream_t)stream;
+    reset_err_async(p, st);
+    launch_sample(p, d_f, B, f_cols, shot_base, seed, d_U, d_out, st);
+  });
+}
+
+int ksim_program_check(ksim_program *p, void *stream) {
+  return guarded([&] {
+    if (!p) throw std::invalid_argument("program is NULL");
+    check_err_sync(p, (hipStream_t)stream);
+  });
+}
+
+static void sample_host(ksim_program *p, const uint8_t *f, int64_t B, int32_t f_cols,
+                        uint64_t seed, const double *U, uint8_t *out, int out_cols) {
+  if (B < 0) throw std::invalid_argument("B < 0");
+  if (f_cols < p->num_f) throw std::invalid_argument("f_params has fewer columns than f_selection needs");
+  if (B == 0) return;
+  if (!out || (!f && f_cols > 0)) throw std::invalid_argument("NULL buffer");
+  hipStream_t st = nullptr;
+  const size_t fb = (size_t)B * f_cols;
+  const size_t ob = (size_t)B * out_cols;
+  ensure(p->d_f, p->f_cap, fb);
+  ensure(p->d_out, p->out_cap, ob);
+  if (fb) hip_check(hipMemcpyAsync(p->d_f, f, fb, hipMemcpyHostToDevice, st), "H2D f");
+  const double *dU = nullptr;
+  if (U) {
+    const size_t ub = (size_t)B * p->tot_out * sizeof(double);
+    ensure(p->d_U, p->U_cap, ub);
+    if (ub) hip_check(hipMemcpyAsync(p->d_U, U, ub, hipMemcpyHostToDevice, st), "H2D U");
+    dU = p->d_U;
+  }
+  if (out_cols != p->num_outputs || p->tot_out < p->num_outputs)
+    fill_u8_async(p->d_out, 0, (int64_t)ob, st);
+  reset_err_async(p, st);
+  launch_sample(p, p->d_f, B, f_cols, 0, seed, dU, p->d_out, st);
+  hip_check(hipMemcpyAsync(out, p->d_out, ob, hipMemcpyDeviceToHost, st), "D2H out");
+  check_err_sync(p, st);
+}
+
+int ksim_program_sample(ksim_program *p, const uint8_t *f_params, int64_t B, int32_t f_cols,
+                        uint64_t seed, uint8_t *out) {
+  return guarded([&] {
+    if (!p) throw std::invalid_argument("program is NULL");
+    sample_host(p, f_params, B, f_cols, seed, nullptr, out, p->num_outputs);
+  });
+}
+
+int ksim_program_sample_uniforms(ksim_program *p, const uint8_t *f_params, int64_t B,
+                                 int32_t f_cols, const double *U, uint8_t *out) {
+  return guarded([&] {
+    if (!p) throw std::invalid_argument("program is NULL");
+    if (!U && p->tot_out > 0 && B > 0) throw std::invalid_argument("U is NULL");
+    sample_host(p, f_params, B, f_cols, 0, U, out, p->num_outputs);
+  });
+}
+
+int ksim_component_sample(ksim_program *p, const uint8_t *f_sel, int64_t B, int32_t f_cols,
+                          uint64_t seed, uint8_t *out) {
+  return guarded([&] {
+    if (!p) throw std::invalid_argument("program is NULL");
+    if (f_cols != p->num_f) throw std::invalid_argument("f_sel column count != F");
+    sample_host(p, f_sel, B, f_cols, seed, nullptr, out, p->num_outputs);
+  });
+}
+
+static void eval_host(ksim_program *p, int32_t comp, int32_t graph, const uint8_t *params,
+                      int64_t B, int32_t P, int mode, int64_t *coeffs, int32_t *exp2,
+                      double *absout) {
+  if (comp < 0 || comp >= p->n_comp) throw std::invalid_argument("component index out of range");
+  if (graph < 0 || graph >= p->n_graphs[comp]) throw std::invalid_argument("graph index out of range");
+  const int expectP = p->F[comp] + graph;
+  if (P != expectP) throw std::invalid_argument("params column count != P of the graph");
+  if (B < 0) throw std::invalid_argument("B < 0");
+  if (B == 0) return;
+  const int W32 = std::max(1, (P + 31) / 32);
+  const int wl = lane_words(W32);
+  hipStream_t st = nullptr;
+  ensure(p->d_params, p->params_cap, (size_t)B * P);
+  if ((size_t)B * P)
+    hip_check(hipMemcpyAsync(p->d_params, params, (size_t)B * P, hipMemcpyHostToDevice, st),
+              "H2D params");
+  const size_t rb = mode == 0 ? (size_t)B * (8 * 8 + 4) : (size_t)B * 8;
+  ensure(p->d_res, p->res_cap, rb);
+  DevTables t = tables_for_current_device();
+  // graph header lives in the device program: offset of component header + graph index
+  uint32_t ch_off = 0;
+  hip_check(hipMemcpy(&ch_off, p->d_prog + PH_DWORDS + comp, 4, hipMemcpyDeviceToHost),
+            "read comp offset");
+  uint32_t chd[CH_DWORDS];
+  hip_check(hipMemcpy(chd, p->d_prog + ch_off, sizeof(chd), hipMemcpyDeviceToHost),
+            "read comp header");
+  EvalArgs a;
+  a.prog = p->d_prog;
+  a.gh = p->d_prog + chd[CH_GRAPH_OFF] + (size_t)graph * GH_DWORDS;
+  a.params = p->d_params;
+  a.B = B; a.P = P; a.mode = mode;
+  a.coeffs = (int64_t *)p->d_res;
+  a.exp2 = (int32_t *)((char *)p->d_res + (size_t)B * 64);
+  a.absout = (double *)p->d_res;
+  a.err = p->d_err;
+  reset_err_async(p, st);
+  dim3 grid((unsigned)((B + kBlock - 1) / kBlock));
+  switch (wl) {
+    case 1: hipLaunchKernelGGL(k_eval<1>, grid, dim3(kBlock), 0, st, a, t.tab, t.pitab); break;
+    case 2: hipLaunchKernelGGL(k_eval<2>, grid, dim3(kBlock), 0, st, a, t.tab, t.pitab); break;
+    case 4: hipLaunchKernelGGL(k_eval<4>, grid, dim3(kBlock), 0, st, a, t.tab, t.pitab); break;
+    default: hipLaunchKernelGGL(k_eval<8>, grid, dim3(kBlock), 0, st, a, t.tab, t.pitab); break;
+  }
+  hip_check(hipGetLastError(), "k_eval launch");
+  if (mode == 0) {
+    hip_check(hipMemcpyAsync(coeffs, a.coeffs, (size_t)B * 64, hipMemcpyDeviceToHost, st), "D2H");
+    hip_check(hipMemcpyAsync(exp2, a.exp2, (size_t)B * 4, hipMemcpyDeviceToHost, st), "D2H");
+  } else {
+    hip_check(hipMemcpyAsync(absout, a.absout, (size_t)B * 8, hipMemcpyDeviceToHost, st), "D2H");
+  }
+  check_err_sync(p, st);
+}
+
+int ksim_program_eval_exact(ksim_program *p, int32_t comp, int32_t graph,
+                            const uint8_t *params, int64_t B, int32_t P, int64_t *coeffs,
+                            int32_t *exp2) {
+  return guarded([&] {
+    if (!p) throw std::invalid_argument("program is NULL");
+    if (B > 0 && (!coeffs || !exp2 || (!params && P > 0))) throw std::invalid_argument("NULL buffer");
+    eval_host(p, comp, graph, params, B, P, 0, coeffs, exp2, nullptr);
+  });
+}
+
+int ksim_component_eval_abs(ksim_program *p, int32_t graph, const uint8_t *params, int64_t B,
+                            int32_t P, double *out) {
+  return guarded([&] {
+    if (!p) throw std::invalid_argument("program is NULL");
+    if (B > 0 && (!out || (!params && P > 0))) throw std::invalid_argument("NULL buffer");
+    eval_host(p, 0, graph, params, B, P, 1, nullptr, nullptr, out);
+  });
+}
+
+int ksim_program_set_tiles(ksim_program *p, int32_t block_threads, int32_t term_split) {
+  return guarded([&] {
+    if (!p) throw std::invalid_argument("program is NULL");
+    if (block_threads != kBlock) throw std::invalid_argument("block_threads must be 256");
+    if (term_split != 1) throw std::invalid_argument("term_split must be 1");
+    p->block_threads = block_threads;
+    p->term_split = term_split;
+  });
+}
+
+int64_t ksim_program_device_bytes(const ksim_program *p) {
+  return p ? (int64_t)(p->n_words * 4) : -1;
+}
+
+int ksim_program_stats(const ksim_program *p, double *stats5) {
+  return guarded([&] {
+    if (!p || !stats5) throw std::invalid_argument("NULL argument");
+    stats5[0] = (double)p->st.graphs;
+    stats5[1] = (double)p->st.terms;
+    stats5[2] = (double)p->st.records;
+    stats5[3] = (double)p->n_words;
+    stats5[4] = p->ops_per_shot;
+  });
+}
+
+void ksim_program_destroy(ksim_program *p) {
+  if (!p) return;
+  if (p->d_prog) (void)hipFree(p->d_prog);
+  if (p->d_err) (void)hipFree(p->d_err);
+  if (p->d_f) (void)hipFree(p->d_f);
+  if (p->d_out) (void)hipFree(p->d_out);
+  if (p->d_U) (void)hipFree(p->d_U);
+  if (p->d_params) (void)hipFree(p->d_params);
+  if (p->d_res) (void)hipFree(p->d_res);
+  delete p;
+}
+
+int ksim_program_pack(const ksim_graph_view *graphs, const int32_t *graphs_per_comp,
+                      const int32_t *f_selection, const int32_t *F_per_comp, int32_t n_comp,
+                      int32_t num_f, const int32_t *dest_col, int32_t num_outputs,
+                      uint32_t *words, int64_t cap, int64_t *n_words) {
+  return guarded([&] {
+    if (!n_words) throw std::invalid_argument("n_words is NULL");
+    PackStats st;
+    std::vector<uint32_t> w = pack_program(graphs, graphs_per_comp, f_selection, F_per_comp,
+                                           n_comp, num_f, dest_col, num_outputs, st);
+    *n_words = (int64_t)w.size();
+    if (words && cap > 0)
+      std::memcpy(words, w.data(), sizeof(uint32_t) * std::min<size_t>(w.size(), (size_t)cap));
+  });
+}
+
+int ksim_tables(int32_t *tab, int64_t tab_cap, int32_t *pitab, int64_t pitab_cap,
+                int32_t *sizes2) {
+  return guarded([&] {
+    if (tab) std::memcpy(tab, materialise_table(), sizeof(int32_t) * std::min<int64_t>(tab_cap, 4 * TAB_N));
+    if (pitab)
+      std::memcpy(pitab, pi_table(), sizeof(int32_t) * std::min<int64_t>(pitab_cap, 4 * PI_MAX * PI_MAX));
+    if (sizes2) { sizes2[0] = TAB_R; sizes2[1] = PI_MAX; }
+  });
+}
+
+int ksim_device_info(int64_t *info4) {
+  return guarded([&] {
+    if (!info4) throw std::invalid_argument("NULL argument");
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    hipDeviceProp_t pr;
+    hip_check(hipGetDeviceProperties(&pr, dev), "hipGetDeviceProperties");
+    info4[0] = pr.multiProcessorCount;
+    info4[1] = (int64_t)pr.maxSharedMemoryPerMultiProcessor;
+    info4[2] = pr.clockRate;
+    info4[3] = (int64_t)pr.totalGlobalMem;
+  });
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ batch
+struct ksim_batch {
+  std::vector<ksim_program *> progs;
+  BatchProg *d_progs = nullptr;
+  BatchItem *d_items = nullptr;
+  uint32_t n_items = 0;
+  uint32_t *d_cursor = nullptr;  // [0] cursor, [1] err
+  int wave_shots = 64;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int graph_blocks = 0;
+  hipStream_t graph_stream = nullptr;
+};
+
+extern "C" {
+
+int ksim_batch_create(ksim_program *const *progs, int32_t n, const uint8_t *const *d_f,
+                      const int32_t *f_cols, const int64_t *B, const uint64_t *seed,
+                      uint8_t *const *d_out, int32_t wave_shots, ksim_batch **out) {
+  return guarded([&] {
+    if (!out || (n > 0 && (!progs || !d_f || !f_cols || !B || !seed || !d_out)))
+      throw std::invalid_argument("NULL argument");
+    if (wave_shots < 64 || wave_shots % 64) throw std::invalid_argument("wave_shots must be a multiple of 64");
+    auto *b = new ksim_batch();
+    b->wave_shots = wave_shots;
+    std::vector<BatchProg> hp(n);
+    struct Cost { double c; BatchItem it; };
+    std::vector<Cost> items;
+    for (int k = 0; k < n; ++k) {
+      ksim_program *p = progs[k];
+      if (!p) throw std::invalid_argument("NULL program in batch");
+      if (B[k] < 0) throw std::invalid_argument("B < 0");
+      if (B[k] > 0 && ((!d_f[k] && f_cols[k] > 0) || !d_out[k])) throw std::invalid_argument("NULL buffer");
+      if (f_cols[k] < p->num_f) throw std::invalid_argument("f matrix has fewer columns than f_selection needs");
+      b->progs.push_back(p);
+      hp[k] = BatchProg{p->d_prog, d_f[k], d_out[k], seed[k], B[k], f_cols[k], p->num_outputs,
+                        p->tot_out, p->wl};
+      for (int c = 0; c < p->n_comp; ++c) {
+        // cost of one shot of this component ~ records + terms over its graphs
+        double cost = 1.0;
+        for (int i = 0; i < p->n_graphs[c]; ++i) cost += p->graph_hdr[c][i * GH_DWORDS + GH_NREC];
+        for (int64_t s0 = 0; s0 < B[k]; s0 += wave_shots) {
+          const int64_t ns = std::min<int64_t>(wave_shots, B[k] - s0);
+          items.push_back(Cost{cost * (double)ns, BatchItem{k, c, s0}});
+        }
+      }
+    }
+    // heaviest items first: the dynamic cursor then balances the tail
+    std::stable_sort(items.begin(), items.end(),
+                     [](const Cost &x, const Cost &y) { return x.c > y.c; });
+    std::vector<BatchItem> hi(items.size());
+    for (size_t i = 0; i < items.size(); ++i) hi[i] = items[i].it;
+    if (hi.size() >= 0xffffffffull) throw std::invalid_argument("too many work items");
+    b->n_items = (uint32_t)hi.size();
+    try {
+      hip_check(hipMalloc(&b->d_progs, sizeof(BatchProg) * std::max(1, n)), "hipMalloc");
+      hip_check(hipMalloc(&b->d_items, sizeof(BatchItem) * std::max<size_t>(1, hi.size())), "hipMalloc");
+      hip_check(hipMalloc(&b->d_cursor, 64), "hipMalloc");
+      if (n) hip_check(hipMemcpy(b->d_progs, hp.data(), sizeof(BatchProg) * n, hipMemcpyHostToDevice), "H2D");
+      if (!hi.empty())
+        hip_check(hipMemcpy(b->d_items, hi.data(), sizeof(BatchItem) * hi.size(), hipMemcpyHostToDevice), "H2D");
+      hip_check(hipMemset(b->d_cursor, 0, 64), "hipMemset");
+      tables_for_current_device();
+    } catch (...) {
+      ksim_batch_destroy(b);
+      throw;
+    }
+    *out = b;
+  });
+}
+
+static void batch_enqueue(ksim_batch *b, int blocks, hipStream_t st) {
+  DevTables t = tables_for_current_device();
+  hipLaunchKernelGGL(k_reset_cursor, dim3(1), dim3(64), 0, st, b->d_cursor, b->d_cursor + 1);
+  hip_check(hipGetLastError(), "k_reset_cursor");
+  hipLaunchKernelGGL(k_batch, dim3(blocks), dim3(kBlock), 0, st, b->d_progs, b->d_items,
+                     b->n_items, b->d_cursor, b->d_cursor + 1, b->wave_shots, t.tab, t.pitab);
+  hip_check(hipGetLastError(), "k_batch");
+}
+
+int ksim_batch_run(ksim_batch *b, int32_t persistent_blocks, int32_t use_graph, void *stream) {
+  return guarded([&] {
+    if (!b) throw std::invalid_argument("batch is NULL");
+    if (persistent_blocks <= 0) throw std::invalid_argument("persistent_blocks must be > 0");
+    hipStream_t st = (hipStream_t)stream;
+    if (!use_graph) {
+      batch_enqueue(b, persistent_blocks, st);
+      return;
+    }
+    if (!b->exec || b->graph_blocks != persistent_blocks) {
+      if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
+      if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+      hipStream_t cs;
+      hip_check(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "hipStreamCreate");
+      hip_check(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal), "BeginCapture");
+      batch_enqueue(b, persistent_blocks, cs);
+      hip_check(hipStreamEndCapture(cs, &b->graph), "EndCapture");
+      hip_check(hipStreamDestroy(cs), "hipStreamDestroy");
+      hip_check(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "GraphInstantiate");
+      b->graph_blocks = persistent_blocks;
+    }
+    hip_check(hipGraphLaunch(b->exec, st), "hipGraphLaunch");
+  });
+}
+
+int ksim_batch_check(ksim_batch *b) {
+  return guarded([&] {
+    if (!b) throw std::invalid_argument("batch is NULL");
+    uint32_t e = 0;
+    hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    hip_check(hipMemcpy(&e, b->d_cursor + 1, 4, hipMemcpyDeviceToHost), "read err");
+    if (e) throw RangeError(err_text(e));
+  });
+}
+
+void ksim_batch_destroy(ksim_batch *b) {
+  if (!b) return;
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->d_progs) (void)hipFree(b->d_progs);
+  if (b->d_items) (void)hipFree(b->d_items);
+  if (b->d_cursor) (void)hipFree(b->d_cursor);
+  delete b;
+}
+
+}  // extern "C"

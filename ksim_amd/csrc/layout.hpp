@@ -1,0 +1,85 @@
+// layout.hpp — device program format shared by the host packer and the
+// gfx950 kernels.
+//
+// Every factor of a term (the node-phase, half-pi, pi-product and phase-pair
+// families of src/cpp/ksim/zx_eval.hpp:8-14) is, for each parity outcome,
+// either 0 or a product
+//
+//     w^s (1+w)^v (1+sqrt2)^n pi^a pibar^b,   pi = 1 + i sqrt2, pibar = 1 - i sqrt2,
+//
+// so a term's product is fixed by five small exponents.  The kernel adds
+// per-factor exponent deltas into one packed 32-bit word instead of
+// multiplying four-coefficient Z[w] values, and builds the exact Z[w] value
+// once per term from a table.
+//
+// Packed exponent word (a "chunk" of at most CHUNK_MAX factor records):
+//   bits  0.. 8  V            (v <= 6 per record)
+//   bits  9..16  N + NBIAS    (n in [-3, 0] per record)
+//   bits 17..22  A
+//   bits 23..28  B
+//   bits 29..31  S mod 8      (wraps by itself)
+#pragma once
+#include <cstdint>
+
+namespace ksim {
+
+constexpr int PK_OV = 0, PK_WV = 9;
+constexpr int PK_ON = 9, PK_WN = 8;
+constexpr int PK_OA = 17, PK_WA = 6;
+constexpr int PK_OB = 23, PK_WB = 6;
+constexpr int PK_OS = 29;
+constexpr int NBIAS = 3;
+constexpr int CHUNK_MAX = 63;
+
+// Materialisation table: w^s (1+w)^r (1+sqrt2)^n for |n| <= TAB_R, r < 4,
+// s < 8, int32 coefficients in the basis (1, w, i, conj w).
+constexpr int TAB_R = 22;
+constexpr int TAB_N = (2 * TAB_R + 1) * 4 * 8;  // entries of 4 x int32
+inline int tab_index(int n, int r, int s) { return ((n + TAB_R) * 4 + r) * 8 + s; }
+
+// pi^a pibar^b for a, b < PI_MAX (int32 coefficients)
+constexpr int PI_MAX = 8;
+
+// term header (dwords)
+enum TermHdr : int {
+  TH_NPROJ = 0, TH_NPPX, TH_NCHUNK, TH_FLAGS,
+  TH_S, TH_V, TH_N, TH_A, TH_B, TH_E,
+  TH_R0, TH_R1, TH_R2, TH_R3,
+  TH_SIZE,  // dwords of records following the header
+  TH_PAD,
+  TH_DWORDS
+};
+constexpr uint32_t TF_RESIDUAL = 1u;  // multiply by the residual (R0..R3)
+constexpr uint32_t TF_PI = 2u;        // A or B may be non-zero
+
+constexpr int CHUNK_HDR = 4;  // n_lin, n_pair, base, pad
+
+// graph header (dwords)
+enum GraphHdr : int {
+  GH_NTERMS = 0, GH_W32, GH_E0, GH_TERM_OFF, GH_NREC, GH_PAD0, GH_PAD1, GH_PAD2,
+  GH_DWORDS
+};
+
+// component header (dwords)
+enum CompHdr : int {
+  CH_NGRAPHS = 0, CH_F, CH_NOUT, CH_WMAX, CH_COLBASE, CH_FSEL_OFF,
+  CH_DEST_OFF, CH_GRAPH_OFF,
+  CH_DWORDS
+};
+
+// program header (dwords); followed by n_comp component-header offsets
+enum ProgHdr : int {
+  PH_NCOMP = 0, PH_NUMF, PH_NUMOUT, PH_TOTOUT, PH_WMAX, PH_PAD0, PH_PAD1, PH_PAD2,
+  PH_DWORDS
+};
+
+// device error bits
+constexpr uint32_t ERR_TAB_RANGE = 1u;   // |n'| beyond the table
+constexpr uint32_t ERR_PI_RANGE = 2u;    // pi exponents beyond PI_MAX
+constexpr uint32_t ERR_SHIFT = 4u;       // alignment shift outside [0, max]
+
+// Largest alignment shift the 128-bit sum accepts; the packer proves
+// bits(term) + shift + log2(G) <= 126 before upload.
+constexpr int SHIFT_MAX = 126;
+
+}  // namespace ksim

@@ -1,0 +1,465 @@
+// pack.cpp — FlatProgram graphs -> device program words.
+//
+// Replaces the reference's host packing (src/cpp/ksim/kokkos_sim.cpp:73-151,
+// which concatenates the u8/i32 arrays and a 25-int meta row per graph).  Here
+// each graph is rewritten, term by term, into records the gfx950 kernel
+// evaluates with bit-packed masks and packed exponent adds:
+//
+//   PROJ  (mask, expect)          term is zero unless parity == expect
+//                                 (node phase 0 or 4: factor 2 or 0)
+//   PPX   (mask_psi, mask_phi,    pi-product: S += 4 (psi & phi)
+//          psi_c, phi_c)
+//   LIN   (mask, delta)           exponents += delta if parity is 1
+//                                 (node phases, half-pi terms, one-sided pairs)
+//   PAIR  (mask_a, mask_b,        phase pair with both parities live
+//          d10, d01, d11)
+//
+// Parameter-independent factors (zero masks) and the prefactor
+// w^{pf_phase_idx} * floatfactor * 2^{pf_power2} are folded into per-term
+// constants.  Terms with a constant zero factor are dropped.  The per-family
+// semantics follow src/cpp/ksim/zx_eval.hpp:102-165 and
+// src/python/ksim/sample/evaluate.py:32-61.
+#include "pack.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+
+namespace ksim {
+
+namespace {
+
+struct FacVal {
+  bool zero = false;
+  int s = 0, v = 0, n = 0, a = 0, b = 0;
+};
+
+FacVal g_np[8];      // 1 + w^k
+FacVal g_ph[8][8];   // 1 + w^a + w^b - w^{a+b}
+std::once_flag g_once;
+
+FacVal decompose_small(const Zw &x) {
+  FacVal f;
+  if (zw_is_zero(x)) { f.zero = true; return f; }
+  Fact ft = zw_factor(x);
+  if (!ft.pure) throw std::logic_error("factor table: value does not factor");
+  f.s = ft.s; f.v = ft.v; f.n = ft.n; f.a = ft.a; f.b = ft.b;
+  if (!zw_eq(zw_build(f.s, f.v, f.n, f.a, f.b), x))
+    throw std::logic_error("factor table: rebuild mismatch");
+  return f;
+}
+
+void init_tables() {
+  for (int k = 0; k < 8; ++k) {
+    Zw u = zw_unit(k);
+    u.a += 1;
+    g_np[k] = decompose_small(u);
+  }
+  for (int a = 0; a < 8; ++a)
+    for (int b = 0; b < 8; ++b) {
+      Zw u = zw_unit(a), v = zw_unit(b), w = zw_unit(a + b);
+      Zw t{1 + u.a + v.a - w.a, u.b + v.b - w.b, u.c + v.c - w.c, u.d + v.d - w.d};
+      g_ph[a][b] = decompose_small(t);
+    }
+}
+
+inline uint32_t pack_exp(const FacVal &e) {
+  return (uint32_t)e.v << PK_OV | (uint32_t)(e.n + NBIAS) << PK_ON | (uint32_t)e.a << PK_OA |
+         (uint32_t)e.b << PK_OB | (uint32_t)(e.s & 7) << PK_OS;
+}
+
+struct Mask {
+  std::vector<uint32_t> w;
+  bool zero() const {
+    for (uint32_t x : w) if (x) return false;
+    return true;
+  }
+};
+
+Mask make_mask(const uint8_t *row, int P, int W32) {
+  Mask m;
+  m.w.assign(W32, 0u);
+  for (int j = 0; j < P; ++j)
+    if (row[j] & 1u) m.w[j >> 5] |= 1u << (j & 31);
+  return m;
+}
+
+Mask mask_xor(const Mask &x, const Mask &y) {
+  Mask m = x;
+  for (size_t k = 0; k < m.w.size(); ++k) m.w[k] ^= y.w[k];
+  return m;
+}
+
+bool mask_eq(const Mask &x, const Mask &y) { return x.w == y.w; }
+
+struct LinRec { Mask m; FacVal e0, e1; };
+struct PairRec { Mask ma, mb; FacVal e00, e10, e01, e11; };
+struct ProjRec { Mask m; uint32_t expect; };
+struct PpxRec { Mask ms, mf; uint32_t sc, fc; };
+
+struct TermBuild {
+  bool zero = false;
+  // constant exponents
+  int S = 0, V = 0, N = 0, A = 0, B = 0, E = 0;
+  Zw R{1, 0, 0, 0};
+  bool residual = false;
+  std::vector<ProjRec> proj;
+  std::vector<PpxRec> ppx;
+  std::vector<LinRec> lin;
+  std::vector<PairRec> pair;
+
+  void add_const(const FacVal &f) {
+    if (f.zero) { zero = true; return; }
+    S += f.s; V += f.v; N += f.n; A += f.a; B += f.b;
+  }
+  void add_lin(const Mask &m, const FacVal &e0, const FacVal &e1) {
+    if (e0.zero || e1.zero) throw std::logic_error("LIN record with a zero value");
+    if (m.zero()) { add_const(e0); return; }
+    lin.push_back(LinRec{m, e0, e1});
+  }
+};
+
+int ceil_log2(int64_t x) {
+  int b = 0;
+  while ((int64_t(1) << b) < x) ++b;
+  return b;
+}
+
+}  // namespace
+
+const int32_t *materialise_table() {
+  // w^s (1+w)^r (1+sqrt2)^n as int32 x 4, n in [-TAB_R, TAB_R]
+  static std::vector<int32_t> tab;
+  static std::once_flag once;
+  std::call_once(once, [] {
+    tab.assign((size_t)TAB_N * 4, 0);
+    for (int n = -TAB_R; n <= TAB_R; ++n)
+      for (int r = 0; r < 4; ++r)
+        for (int s = 0; s < 8; ++s) {
+          Zw x = zw_mul(zw_mul(zw_unit(s), zw_pow(zw_lambda(), r)), zw_unit_pow(n));
+          if (zw_maxabs(x) >= ((i128)1 << 30)) throw std::logic_error("table entry too large");
+          int32_t *e = &tab[(size_t)tab_index(n, r, s) * 4];
+          e[0] = (int32_t)x.a; e[1] = (int32_t)x.b; e[2] = (int32_t)x.c; e[3] = (int32_t)x.d;
+        }
+  });
+  return tab.data();
+}
+
+const int32_t *pi_table() {
+  static std::vector<int32_t> tab;
+  static std::once_flag once;
+  std::call_once(once, [] {
+    tab.assign(PI_MAX * PI_MAX * 4, 0);
+    for (int a = 0; a < PI_MAX; ++a)
+      for (int b = 0; b < PI_MAX; ++b) {
+        Zw x = zw_mul(zw_pow(zw_pi(), a), zw_pow(zw_pibar(), b));
+        if (zw_maxabs(x) >= ((i128)1 << 30)) throw std::logic_error("pi table entry too large");
+        int32_t *e = &tab[(a * PI_MAX + b) * 4];
+        e[0] = (int32_t)x.a; e[1] = (int32_t)x.b; e[2] = (int32_t)x.c; e[3] = (int32_t)x.d;
+      }
+  });
+  return tab.data();
+}
+
+static int pi_table_bits(int amax, int bmax) {
+  const int32_t *t = pi_table();
+  int bits = 0;
+  for (int a = 0; a <= std::min(amax, PI_MAX - 1); ++a)
+    for (int b = 0; b <= std::min(bmax, PI_MAX - 1); ++b)
+      for (int k = 0; k < 4; ++k) bits = std::max(bits, i128_bits(t[(a * PI_MAX + b) * 4 + k]));
+  return bits;
+}
+
+static void build_term(const ksim_graph_view &g, int t, int W32, TermBuild &tb) {
+  std::call_once(g_once, init_tables);
+  const int P = g.P;
+  // ---- node phases (zx_eval.hpp:106-117): prod_{j < np_counts[t]} 1 + w^{4 par + phi}
+  const int ncnt = g.np_counts ? g.np_counts[t] : 0;
+  if (ncnt < 0 || ncnt > g.T_np) throw std::invalid_argument("np_counts out of range");
+  for (int j = 0; j < ncnt; ++j) {
+    const int phi = g.np_phases[(size_t)t * g.T_np + j] & 7;
+    Mask m = make_mask(g.np_params + ((size_t)t * g.T_np + j) * P, P, W32);
+    if (m.zero()) { tb.add_const(g_np[phi]); continue; }
+    if (phi == 0 || phi == 4) {  // 2 if parity == [phi == 4] else 0
+      tb.add_const(g_np[0]);
+      tb.proj.push_back(ProjRec{m, phi == 4 ? 1u : 0u});
+    } else {
+      tb.add_lin(m, g_np[phi], g_np[phi ^ 4]);
+    }
+  }
+  // ---- half-pi (zx_eval.hpp:118-128): w^{sum par * c mod 8}
+  for (int j = 0; j < g.T_hp; ++j) {
+    const int c = g.hp_coeffs[(size_t)t * g.T_hp + j] & 7;
+    if (c == 0) continue;
+    Mask m = make_mask(g.hp_params + ((size_t)t * g.T_hp + j) * P, P, W32);
+    if (m.zero()) continue;
+    FacVal e0, e1;
+    e1.s = c;
+    tb.add_lin(m, e0, e1);
+  }
+  // ---- pi products (zx_eval.hpp:129-142): (-1)^{sum psi phi}
+  for (int j = 0; j < g.T_pp; ++j) {
+    const size_t o = (size_t)t * g.T_pp + j;
+    const uint32_t sc = g.pp_psi_c[o] & 1u, fc = g.pp_phi_c[o] & 1u;
+    Mask ms = make_mask(g.pp_psi_p + o * P, P, W32);
+    Mask mf = make_mask(g.pp_phi_p + o * P, P, W32);
+    FacVal e0, e4;
+    e4.s = 4;
+    if (ms.zero() && mf.zero()) {
+      if (sc & fc) tb.S += 4;
+    } else if (ms.zero()) {  // psi constant
+      if (sc) tb.add_lin(mf, fc ? e4 : e0, fc ? e0 : e4);
+    } else if (mf.zero()) {
+      if (fc) tb.add_lin(ms, sc ? e4 : e0, sc ? e0 : e4);
+    } else if (mask_eq(ms, mf)) {  // psi, phi share the parity p
+      if (sc == fc) tb.add_lin(ms, sc ? e4 : e0, sc ? e0 : e4);  // (sc^p) & (sc^p)
+      // sc != fc: one of psi, phi is 0 for every p
+    } else {
+      tb.ppx.push_back(PpxRec{ms, mf, sc, fc});
+    }
+  }
+  // ---- phase pairs (zx_eval.hpp:143-158): prod 1 + w^a + w^b - w^{a+b}
+  const int pcnt = g.ph_counts ? g.ph_counts[t] : 0;
+  if (pcnt < 0 || pcnt > g.T_ph) throw std::invalid_argument("ph_counts out of range");
+  for (int j = 0; j < pcnt; ++j) {
+    const size_t o = (size_t)t * g.T_ph + j;
+    const int al = g.ph_alpha[o] & 7, be = g.ph_beta[o] & 7;
+    Mask ma = make_mask(g.ph_alpha_p + o * P, P, W32);
+    Mask mb = make_mask(g.ph_beta_p + o * P, P, W32);
+    if (ma.zero() && mb.zero()) { tb.add_const(g_ph[al][be]); continue; }
+    if ((al & 3) == 2 && (be & 3) == 2) {
+      // a', b' in {2, 6}: zero unless a' == b', i.e. par_a ^ par_b == [al != be];
+      // then the value is F(a', a') with a' = al ^ 4 par_a.
+      Mask mx = mask_xor(ma, mb);
+      const uint32_t expect = al != be ? 1u : 0u;
+      if (mx.zero()) {
+        if (expect) { tb.zero = true; return; }
+      } else {
+        tb.proj.push_back(ProjRec{mx, expect});
+      }
+      if (ma.zero()) {
+        // a' = al fixed; alive only when b' == al
+        tb.add_const(g_ph[al][al]);
+      } else {
+        tb.add_lin(ma, g_ph[al][al], g_ph[al ^ 4][al ^ 4]);
+      }
+      continue;
+    }
+    if (ma.zero()) { tb.add_lin(mb, g_ph[al][be], g_ph[al][be ^ 4]); continue; }
+    if (mb.zero()) { tb.add_lin(ma, g_ph[al][be], g_ph[al ^ 4][be]); continue; }
+    if (mask_eq(ma, mb)) { tb.add_lin(ma, g_ph[al][be], g_ph[al ^ 4][be ^ 4]); continue; }
+    tb.pair.push_back(PairRec{ma, mb, g_ph[al][be], g_ph[al ^ 4][be], g_ph[al][be ^ 4],
+                              g_ph[al ^ 4][be ^ 4]});
+  }
+  // ---- prefactor (zx_eval.hpp:159-170): w^idx * floatfactor * 2^power2
+  {
+    Zw ff{g.pf_floatfactor[(size_t)t * 4 + 0], g.pf_floatfactor[(size_t)t * 4 + 1],
+          g.pf_floatfactor[(size_t)t * 4 + 2], g.pf_floatfactor[(size_t)t * 4 + 3]};
+    Zw c = zw_mul(zw_unit(g.pf_phase_idx[t] & 7), ff);
+    Fact f = zw_factor(c);
+    if (f.zero) { tb.zero = true; return; }
+    tb.S += f.s; tb.V += f.v; tb.N += f.n; tb.A += f.a; tb.B += f.b;
+    if (!f.pure) { tb.residual = true; tb.R = f.R; }
+    tb.E = g.pf_power2[t];
+  }
+}
+
+static void put_mask(std::vector<uint32_t> &w, const Mask &m) {
+  w.insert(w.end(), m.w.begin(), m.w.end());
+}
+
+void pack_graph(const ksim_graph_view &g, std::vector<uint32_t> &out, uint32_t *ghdr,
+                PackStats &st) {
+  if (g.pf_has_approx)
+    throw std::invalid_argument(
+        "pf_has_approx: approximate (complex64) floatfactors are not exact; "
+        "the exact sampler does not accept them");
+  if (g.G < 0 || g.P < 0) throw std::invalid_argument("negative graph dimensions");
+  const int W32 = std::max(1, (g.P + 31) / 32);
+  std::vector<TermBuild> terms;
+  terms.reserve(g.G);
+  for (int t = 0; t < g.G; ++t) {
+    TermBuild tb;
+    build_term(g, t, W32, tb);
+    if (!tb.zero) terms.push_back(std::move(tb));
+  }
+  // ---- exponent ranges and the static alignment exponent
+  struct Range { int vmin, vmax, amax, bmax, emin, emax; };
+  std::vector<Range> rg(terms.size());
+  int e0 = 0, emax_all = 0;
+  int xbits = 0;
+  for (size_t k = 0; k < terms.size(); ++k) {
+    const TermBuild &tb = terms[k];
+    Range r{tb.V, tb.V, tb.A, tb.B, 0, 0};
+    for (const LinRec &l : tb.lin) {
+      r.vmin += std::min(l.e0.v, l.e1.v); r.vmax += std::max(l.e0.v, l.e1.v);
+      r.amax += std::max(l.e0.a, l.e1.a); r.bmax += std::max(l.e0.b, l.e1.b);
+    }
+    for (const PairRec &p : tb.pair) {
+      r.vmin += std::min(std::min(p.e00.v, p.e10.v), std::min(p.e01.v, p.e11.v));
+      r.vmax += std::max(std::max(p.e00.v, p.e10.v), std::max(p.e01.v, p.e11.v));
+      r.amax += std::max(std::max(p.e00.a, p.e10.a), std::max(p.e01.a, p.e11.a));
+      r.bmax += std::max(std::max(p.e00.b, p.e10.b), std::max(p.e01.b, p.e11.b));
+    }
+    r.emin = tb.E + (r.vmin >> 2);
+    r.emax = tb.E + (r.vmax >> 2);
+    rg[k] = r;
+    if (k == 0 || r.emin < e0) e0 = r.emin;
+    if (k == 0 || r.emax > emax_all) emax_all = r.emax;
+    int bits = 30;  // table entries < 2^30
+    if (r.amax > 0 || r.bmax > 0) bits += pi_table_bits(r.amax, r.bmax) + 2;
+    if (tb.residual) bits += i128_bits(zw_maxabs(tb.R)) + 2;
+    xbits = std::max(xbits, bits);
+  }
+  const int max_shift = terms.empty() ? 0 : emax_all - e0;
+  if (!terms.empty()) {
+    if (xbits > 62)
+      throw std::invalid_argument("term value exceeds the int64 exact range of the sampler");
+    if (xbits + max_shift + ceil_log2((int64_t)terms.size()) + 1 > SHIFT_MAX)
+      throw std::invalid_argument(
+          "term exponent spread exceeds the 128-bit exact sum of the sampler");
+  }
+  // ---- emit
+  ghdr[GH_NTERMS] = (uint32_t)terms.size();
+  ghdr[GH_W32] = (uint32_t)W32;
+  ghdr[GH_E0] = (uint32_t)e0;
+  ghdr[GH_TERM_OFF] = (uint32_t)out.size();
+  size_t nrec = 0;
+  for (size_t k = 0; k < terms.size(); ++k) {
+    const TermBuild &tb = terms[k];
+    const size_t hdr = out.size();
+    out.resize(hdr + TH_DWORDS, 0u);
+    std::vector<uint32_t> rec;
+    for (const ProjRec &p : tb.proj) { put_mask(rec, p.m); rec.push_back(p.expect); }
+    for (const PpxRec &p : tb.ppx) {
+      put_mask(rec, p.ms); put_mask(rec, p.mf); rec.push_back(p.sc); rec.push_back(p.fc);
+    }
+    // chunks of at most CHUNK_MAX LIN/PAIR records
+    size_t li = 0, pi = 0;
+    int nchunk = 0;
+    const int nlp = (int)(tb.lin.size() + tb.pair.size());
+    while (li < tb.lin.size() || pi < tb.pair.size()) {
+      int nl = (int)std::min<size_t>(tb.lin.size() - li, CHUNK_MAX);
+      int np = (int)std::min<size_t>(tb.pair.size() - pi, (size_t)(CHUNK_MAX - nl));
+      uint32_t base = 0;
+      for (int q = 0; q < nl; ++q) base += pack_exp(tb.lin[li + q].e0);
+      for (int q = 0; q < np; ++q) base += pack_exp(tb.pair[pi + q].e00);
+      rec.push_back((uint32_t)nl); rec.push_back((uint32_t)np); rec.push_back(base);
+      rec.push_back(0u);
+      for (int q = 0; q < nl; ++q) {
+        const LinRec &l = tb.lin[li + q];
+        put_mask(rec, l.m);
+        rec.push_back(pack_exp(l.e1) - pack_exp(l.e0));
+      }
+      for (int q = 0; q < np; ++q) {
+        const PairRec &p = tb.pair[pi + q];
+        put_mask(rec, p.ma); put_mask(rec, p.mb);
+        const uint32_t p00 = pack_exp(p.e00), p10 = pack_exp(p.e10), p01 = pack_exp(p.e01),
+                       p11 = pack_exp(p.e11);
+        rec.push_back(p10 - p00); rec.push_back(p01 - p00); rec.push_back(p11 - p10 - p01 + p00);
+      }
+      li += nl; pi += np; ++nchunk;
+    }
+    uint32_t *h = &out[hdr];
+    h[TH_NPROJ] = (uint32_t)tb.proj.size();
+    h[TH_NPPX] = (uint32_t)tb.ppx.size();
+    h[TH_NCHUNK] = (uint32_t)nchunk;
+    uint32_t flags = 0;
+    if (tb.residual) flags |= TF_RESIDUAL;
+    if (rg[k].amax > 0 || rg[k].bmax > 0) flags |= TF_PI;
+    h[TH_FLAGS] = flags;
+    h[TH_S] = (uint32_t)(tb.S & 7);
+    h[TH_V] = (uint32_t)tb.V;
+    h[TH_N] = (uint32_t)(tb.N - NBIAS * nlp);
+    h[TH_A] = (uint32_t)tb.A;
+    h[TH_B] = (uint32_t)tb.B;
+    h[TH_E] = (uint32_t)(tb.E - e0);  // relative to the graph's alignment exponent
+    if (tb.residual) {
+      if (zw_maxabs(tb.R) >= ((i128)1 << 31)) throw std::invalid_argument("floatfactor too large");
+      h[TH_R0] = (uint32_t)(int32_t)tb.R.a; h[TH_R1] = (uint32_t)(int32_t)tb.R.b;
+      h[TH_R2] = (uint32_t)(int32_t)tb.R.c; h[TH_R3] = (uint32_t)(int32_t)tb.R.d;
+    } else {
+      h[TH_R0] = 1; h[TH_R1] = h[TH_R2] = h[TH_R3] = 0;
+    }
+    h[TH_SIZE] = (uint32_t)rec.size();
+    out.insert(out.end(), rec.begin(), rec.end());
+    nrec += tb.proj.size() + tb.ppx.size() + tb.lin.size() + tb.pair.size();
+  }
+  ghdr[GH_NREC] = (uint32_t)nrec;
+  st.terms += terms.size();
+  st.records += nrec;
+}
+
+}  // namespace ksim
+
+namespace ksim {
+
+std::vector<uint32_t> pack_program(const ksim_graph_view *graphs, const int32_t *graphs_per_comp,
+                                   const int32_t *f_selection, const int32_t *F_per_comp,
+                                   int32_t n_comp, int32_t num_f, const int32_t *dest_col,
+                                   int32_t num_outputs, PackStats &st) {
+  if (n_comp < 0 || num_f < 0 || num_outputs < 0) throw std::invalid_argument("negative size");
+  std::vector<uint32_t> w(PH_DWORDS + (size_t)n_comp, 0u);
+  w[PH_NCOMP] = (uint32_t)n_comp;
+  w[PH_NUMF] = (uint32_t)num_f;
+  w[PH_NUMOUT] = (uint32_t)num_outputs;
+  size_t gi = 0, fi = 0, oi = 0;
+  int wmax = 1;
+  for (int c = 0; c < n_comp; ++c) {
+    const int ng = graphs_per_comp[c];
+    const int F = F_per_comp[c];
+    if (ng < 1) throw std::invalid_argument("a component needs at least one graph");
+    const int n_out = ng - 1;
+    for (int j = 0; j < F; ++j)
+      if (f_selection[fi + j] < 0 || f_selection[fi + j] >= num_f)
+        throw std::invalid_argument("f_selection index out of range");
+    // parameter count of graph i must be F + i (zx_eval.hpp:65)
+    for (int i = 0; i < ng; ++i) {
+      const ksim_graph_view &g = graphs[gi + i];
+      if (g.G > 0 && g.P != F + i)
+        throw std::invalid_argument("graph " + std::to_string(i) + " of component " +
+                                    std::to_string(c) + " has P=" + std::to_string(g.P) +
+                                    ", expected F+i=" + std::to_string(F + i));
+    }
+    const int wc = std::max(1, (F + n_out + 31) / 32);
+    if (wc > 8) throw std::invalid_argument("more than 256 parameters per component");
+    wmax = std::max(wmax, wc);
+    const size_t ch = w.size();
+    w[PH_DWORDS + c] = (uint32_t)ch;
+    w.resize(ch + CH_DWORDS, 0u);
+    w[ch + CH_NGRAPHS] = (uint32_t)ng;
+    w[ch + CH_F] = (uint32_t)F;
+    w[ch + CH_NOUT] = (uint32_t)n_out;
+    w[ch + CH_WMAX] = (uint32_t)wc;
+    w[ch + CH_COLBASE] = (uint32_t)oi;
+    w[ch + CH_FSEL_OFF] = (uint32_t)w.size();
+    for (int j = 0; j < F; ++j) w.push_back((uint32_t)f_selection[fi + j]);
+    w[ch + CH_DEST_OFF] = (uint32_t)w.size();
+    for (int j = 0; j < n_out; ++j) {
+      const int d = dest_col[oi + j];
+      if (d < 0 || d >= num_outputs) throw std::invalid_argument("dest_col out of range");
+      w.push_back((uint32_t)d);
+    }
+    const size_t gh = w.size();
+    w[ch + CH_GRAPH_OFF] = (uint32_t)gh;
+    w.resize(gh + (size_t)ng * GH_DWORDS, 0u);
+    for (int i = 0; i < ng; ++i) {
+      uint32_t hdr[GH_DWORDS] = {0};
+      pack_graph(graphs[gi + i], w, hdr, st);
+      std::memcpy(&w[gh + (size_t)i * GH_DWORDS], hdr, sizeof(hdr));
+      ++st.graphs;
+    }
+    if (w.size() >= (size_t)1 << 32) throw std::invalid_argument("program larger than 16 GiB");
+    gi += ng;
+    fi += F;
+    oi += n_out;
+  }
+  w[PH_TOTOUT] = (uint32_t)oi;
+  w[PH_WMAX] = (uint32_t)wmax;
+  st.wmax = wmax;
+  return w;
+}
+
+}  // namespace ksim

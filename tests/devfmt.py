@@ -1,0 +1,126 @@
+"""Python interpreter of the packed device program (test helper).
+
+Evaluates a graph of ``ksim_amd.sampler.pack_words(prog)`` with the same
+record semantics and integer arithmetic as the gfx950 kernel
+(ksim_amd/csrc/device.hpp), using the lookup tables the library exports.
+Lets the host packer's factorisation be checked against the exact oracle on
+a machine without a GPU.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+# layout.hpp
+PK_OV, PK_WV, PK_ON, PK_WN, PK_OA, PK_WA, PK_OB, PK_WB, PK_OS = 0, 9, 9, 8, 17, 6, 23, 6, 29
+TH = dict(NPROJ=0, NPPX=1, NCHUNK=2, FLAGS=3, S=4, V=5, N=6, A=7, B=8, E=9, R0=10, SIZE=14)
+TH_DWORDS = 16
+CHUNK_HDR = 4
+GH_DWORDS = 8
+CH = dict(NGRAPHS=0, F=1, NOUT=2, WMAX=3, COLBASE=4, FSEL=5, DEST=6, GRAPH=7)
+PH_DWORDS = 8
+TF_RESIDUAL, TF_PI = 1, 2
+
+
+def s32(x: int) -> int:
+    x &= 0xFFFFFFFF
+    return x - (1 << 32) if x & 0x80000000 else x
+
+
+def zmul(x, y):
+    a1, b1, c1, d1 = x
+    a2, b2, c2, d2 = y
+    return (a1 * a2 + b1 * d2 - c1 * c2 + d1 * b2,
+            a1 * b2 + b1 * a2 + c1 * d2 + d1 * c2,
+            a1 * c2 + b1 * b2 + c1 * a2 - d1 * d2,
+            a1 * d2 - b1 * c2 - c1 * b2 + d1 * a2)
+
+
+def param_words(bits, W32):
+    pv = [0] * W32
+    for j, b in enumerate(bits):
+        if int(b) & 1:
+            pv[j >> 5] |= 1 << (j & 31)
+    return pv
+
+
+def popc_and(pv, words):
+    return sum(bin(p & m).count("1") for p, m in zip(pv, words))
+
+
+class Packed:
+    def __init__(self, words: np.ndarray, tables):
+        self.w = [int(x) for x in words]
+        self.tab, self.pitab, self.R, self.PM = tables
+
+    def comp(self, c):
+        off = self.w[PH_DWORDS + c]
+        return {k: self.w[off + v] for k, v in CH.items()}
+
+    def graph_hdr(self, c, i):
+        ch = self.comp(c)
+        o = ch["GRAPH"] + i * GH_DWORDS
+        return self.w[o:o + GH_DWORDS]
+
+    def eval_graph(self, c, i, bits):
+        """(coeffs, exp2) of graph i of component c at parameter bits."""
+        w = self.w
+        gh = self.graph_hdr(c, i)
+        nterms, W32, e0, off = gh[0], gh[1], s32(gh[2]), gh[3]
+        pv = param_words(bits, W32)
+        acc = [0, 0, 0, 0]
+        th = off
+        for _ in range(nterms):
+            h = w[th:th + TH_DWORDS]
+            S, V, N, A, B, E = (s32(h[TH[k]]) for k in ("S", "V", "N", "A", "B", "E"))
+            rp = th + TH_DWORDS
+            dead = 0
+            for _ in range(h[TH["NPROJ"]]):
+                c0 = w[rp + W32] + popc_and(pv, w[rp:rp + W32])
+                dead |= c0
+                rp += W32 + 1
+            ex = 0
+            for _ in range(h[TH["NPPX"]]):
+                cs = w[rp + 2 * W32] + popc_and(pv, w[rp:rp + W32])
+                cf = w[rp + 2 * W32 + 1] + popc_and(pv, w[rp + W32:rp + 2 * W32])
+                ex ^= cs & cf
+                rp += 2 * W32 + 2
+            S += (ex & 1) << 2
+            for _ in range(h[TH["NCHUNK"]]):
+                nl, npr, x = w[rp], w[rp + 1], w[rp + 2]
+                rp += CHUNK_HDR
+                for _ in range(nl):
+                    if popc_and(pv, w[rp:rp + W32]) & 1:
+                        x = (x + w[rp + W32]) & 0xFFFFFFFF
+                    rp += W32 + 1
+                for _ in range(npr):
+                    pa = popc_and(pv, w[rp:rp + W32]) & 1
+                    pb = popc_and(pv, w[rp + W32:rp + 2 * W32]) & 1
+                    if pa:
+                        x += w[rp + 2 * W32]
+                    if pb:
+                        x += w[rp + 2 * W32 + 1]
+                    if pa and pb:
+                        x += w[rp + 2 * W32 + 2]
+                    x &= 0xFFFFFFFF
+                    rp += 2 * W32 + 3
+                V += (x >> PK_OV) & ((1 << PK_WV) - 1)
+                N += (x >> PK_ON) & ((1 << PK_WN) - 1)
+                A += (x >> PK_OA) & ((1 << PK_WA) - 1)
+                B += (x >> PK_OB) & ((1 << PK_WB) - 1)
+                S += x >> PK_OS
+            q, r = V >> 2, V & 3
+            n2, s2 = N + 2 * q, (S + 2 * q) & 7
+            if not -self.R <= n2 <= self.R:
+                raise OverflowError("table range")
+            X = tuple(int(v) for v in self.tab[((n2 + self.R) * 4 + r) * 8 + s2])
+            if h[TH["FLAGS"]] & TF_PI and (A or B):
+                X = zmul(X, tuple(int(v) for v in self.pitab[A * self.PM + B]))
+            if h[TH["FLAGS"]] & TF_RESIDUAL:
+                X = zmul(X, tuple(s32(h[TH["R0"] + k]) for k in range(4)))
+            if dead & 1:
+                X = (0, 0, 0, 0)
+            sh = q + E
+            assert sh >= 0
+            acc = [a + (v << sh) for a, v in zip(acc, X)]
+            th = th + TH_DWORDS + h[TH["SIZE"]]
+        return tuple(acc), e0

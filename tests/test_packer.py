@@ -1,0 +1,156 @@
+"""Host logic without a GPU: the C++ packer (ksim_program_pack) rewritten into
+the kernel's record format must evaluate, by the kernel's own arithmetic
+(tests/devfmt.py), to the exact oracle's value for every graph — including
+the special cases the packer folds (zero masks, projectors, phase pairs that
+can vanish, generic floatfactors, G = 0, P = 0, several mask words).  Also
+the error behaviour of program creation."""
+import numpy as np
+import pytest
+
+from ksim_amd import synth
+from ksim_amd.program import FlatComponent, FlatProgram, graph_to_dict
+from ksim_amd.sampler import lookup_tables, pack_words
+from oracle import exact
+from tests.devfmt import Packed
+from tests.golden.golden_io import load_program
+
+
+@pytest.fixture(scope="module")
+def tables():
+    return lookup_tables()
+
+
+def check_program(prog, tables, rows=10, seed=0, all_rows=False):
+    pk = Packed(pack_words(prog), tables)
+    rng = np.random.default_rng(seed)
+    n = 0
+    for c, comp in enumerate(prog.components):
+        for i, g in enumerate(comp.graphs):
+            d = graph_to_dict(g)
+            P = g.np_params.shape[2] if g.np_params.ndim == 3 else 0
+            if all_rows and P <= 8:
+                it = [np.array([(m >> j) & 1 for j in range(P)]) for m in range(2 ** P)]
+            else:
+                it = [rng.integers(0, 2, size=P) for _ in range(rows)]
+            for bits in it:
+                want = exact.graph_value(d, bits)
+                got = exact.canonical(*pk.eval_graph(c, i, bits))
+                assert got == want, (c, i, list(bits))
+                n += 1
+    return n
+
+
+def test_tables_are_the_closed_forms(tables):
+    tab, pit, R, PM = tables
+    lam, up, um = (1, 1, 0, 0), (1, 1, 0, 1), (-1, 1, 0, 1)
+    for n in (-R, -3, 0, 5, R):
+        for r in range(4):
+            for s in range(8):
+                x = exact.UNIT[s]
+                for _ in range(r):
+                    x = exact.zmul(x, lam)
+                for _ in range(abs(n)):
+                    x = exact.zmul(x, up if n > 0 else um)
+                assert tuple(int(v) for v in tab[((n + R) * 4 + r) * 8 + s]) == x
+
+
+def test_golden_programs(tables, golden_files):
+    for fn in golden_files:
+        check_program(load_program(np.load(fn)), tables, all_rows=True)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_shape_programs(tables, seed):
+    prog = synth.shape_program(n_q=8, t=8, F=6, n_out=5, seed=seed, generic_ff=0.5,
+                               proj_frac=0.2)
+    assert check_program(prog, tables, rows=6, seed=seed) > 0
+
+
+def test_wide_params(tables):
+    # P up to 100: four 32-bit mask words
+    prog = synth.shape_program(n_q=6, t=4, F=90, n_out=10, seed=7)
+    check_program(prog, tables, rows=4)
+
+
+def test_all_phase_pairs(tables):
+    # one term per (alpha, beta), masks on both, one, or neither parity
+    G = 64 * 4
+    P = 3
+    g = synth.empty_graph(G, P, Tph=1)
+    t = 0
+    for a in range(8):
+        for b in range(8):
+            for kind in range(4):
+                g.ph_alpha[t, 0] = a
+                g.ph_beta[t, 0] = b
+                g.ph_counts[t] = 1
+                if kind in (0, 1):
+                    g.ph_alpha_p[t, 0, 0] = 1
+                if kind in (0, 2):
+                    g.ph_beta_p[t, 0, 1] = 1
+                if kind == 3:
+                    g.ph_alpha_p[t, 0, 2] = 1
+                    g.ph_beta_p[t, 0, 2] = 1
+                g.pf_power2[t] = -(t % 5)
+                t += 1
+    prog = synth.single_program([synth.constant_graph(2), g], F=2)
+    check_program(prog, tables, all_rows=True)
+
+
+def test_node_phases_and_projectors(tables):
+    G, P = 16, 4
+    g = synth.empty_graph(G, P, Tnp=3, Thp=2, Tpp=2)
+    rng = np.random.default_rng(5)
+    g.np_phases[:] = rng.integers(0, 8, size=(G, 3))
+    g.np_params[:] = rng.integers(0, 2, size=(G, 3, P))
+    g.np_counts[:] = rng.integers(0, 4, size=G)
+    g.hp_coeffs[:] = rng.integers(0, 8, size=(G, 2))
+    g.hp_params[:] = rng.integers(0, 2, size=(G, 2, P))
+    g.pp_psi_c[:] = rng.integers(0, 2, size=(G, 2))
+    g.pp_phi_c[:] = rng.integers(0, 2, size=(G, 2))
+    g.pp_psi_p[:] = rng.integers(0, 2, size=(G, 2, P))
+    g.pp_phi_p[:] = rng.integers(0, 2, size=(G, 2, P))
+    g.pp_phi_p[3] = g.pp_psi_p[3]          # shared pi-product parity
+    g.pf_phase_idx[:] = rng.integers(0, 8, size=G)
+    g.pf_floatfactor[:] = rng.integers(-3, 4, size=(G, 4))
+    g.pf_power2[:] = rng.integers(-6, 3, size=G)
+    prog = synth.single_program([synth.constant_graph(3), g], F=3)
+    check_program(prog, tables, all_rows=True)
+
+
+def test_empty_graph_and_zero_terms(tables):
+    g0 = synth.empty_graph(0, 0)
+    g1 = synth.empty_graph(2, 1, Tnp=1)
+    g1.np_phases[:, 0] = 4        # constant zero factor when the mask is empty
+    prog = synth.single_program([g0, g1], F=0)
+    check_program(prog, tables, all_rows=True)
+
+
+def test_rejects_approx():
+    prog = synth.htkh_program(1)
+    prog.components[0].graphs[1].pf_has_approx = True
+    with pytest.raises(ValueError, match="approximate"):
+        pack_words(prog)
+
+
+def test_rejects_wrong_param_count():
+    g0 = synth.constant_graph(0)
+    g1 = synth.empty_graph(1, 3, Tnp=1)   # graph 1 must have P = F + 1 = 1
+    prog = synth.single_program([g0, g1], F=0)
+    with pytest.raises(ValueError, match="expected F\\+i"):
+        pack_words(prog)
+
+
+def test_rejects_bad_counts():
+    prog = synth.htkh_program(1)
+    prog.components[0].graphs[1].np_counts[0] = 5
+    with pytest.raises(ValueError, match="np_counts"):
+        pack_words(prog)
+
+
+def test_multi_component_layout(tables):
+    prog = synth.product_program([synth.htkh_program(1), synth.ghz_program(3)
+                                  if hasattr(synth, "ghz_program") else
+                                  synth.single_program(synth.ghz_graphs(3)),
+                                  synth.htkh_program(3, noisy_f=True)])
+    check_program(prog, tables, all_rows=True)

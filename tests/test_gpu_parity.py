@@ -1,0 +1,181 @@
+"""GPU parity: the HIP sampler against the exact oracle and the reference's
+golden vectors.  Bars: exact Z[w] values equal (canonical forms), float64
+|amplitude| bit-identical to the oracle's, shot streams bit-identical when
+both sides get the same uniforms."""
+import math
+
+import numpy as np
+import pytest
+
+from ksim_amd import synth
+from ksim_amd.program import graph_to_dict
+from ksim_amd.sampler import Component, HipProgramSampler, sample_flat
+from oracle import exact
+from tests.golden.golden_io import load_program
+
+pytestmark = pytest.mark.gpu
+
+
+def log(msg):
+    print(msg, flush=True)
+
+
+def gpu_exact_matches(prog, rows, seed):
+    s = HipProgramSampler(prog)
+    rng = np.random.default_rng(seed)
+    n = 0
+    for c, comp in enumerate(prog.components):
+        for i, g in enumerate(comp.graphs):
+            P = g.np_params.shape[2]
+            pv = rng.integers(0, 2, size=(rows, P)).astype(np.uint8)
+            coeffs, e = s.eval_exact(c, i, pv)
+            d = graph_to_dict(g)
+            for r in range(rows):
+                want = exact.graph_value(d, pv[r])
+                got = exact.canonical(tuple(int(x) for x in coeffs[r]), int(e[r]))
+                assert got == want, (c, i, r)
+                n += 1
+    return n
+
+
+def test_eval_exact_golden(golden_files):
+    for fn in golden_files:
+        log(f"eval_exact {fn}")
+        gpu_exact_matches(load_program(np.load(fn)), 16, 1)
+
+
+@pytest.mark.parametrize("shape", [dict(n_q=8, t=8, F=6, n_out=5, generic_ff=0.5, proj_frac=0.2),
+                                   dict(n_q=6, t=4, F=40, n_out=8),
+                                   dict(n_q=5, t=10, F=90, n_out=60, proj_frac=0.0),
+                                   dict(n_q=9, t=0, F=100, n_out=29)])
+def test_eval_exact_shapes(shape):
+    log(f"eval_exact shape {shape}")
+    prog = synth.shape_program(seed=5, **shape)
+    assert gpu_exact_matches(prog, 8, 2) > 0
+
+
+def test_eval_abs_matches_oracle_and_reference(golden_files):
+    for fn in golden_files:
+        d = np.load(fn)
+        prog = load_program(d)
+        if len(prog.components) != 1:
+            continue
+        comp = prog.components[0]
+        gds = [graph_to_dict(g) for g in comp.graphs]
+        kc = Component(gds, len(comp.f_selection))
+        assert kc.n_out == len(gds) - 1
+        for gi, g in enumerate(comp.graphs):
+            pv = np.ascontiguousarray(d[f"eval_params_0_{gi}"])
+            got = kc.eval_abs(gi, pv)
+            want = np.array([exact.magnitude(*exact.graph_value(gds[gi], r)) for r in pv])
+            assert np.array_equal(got, want), (fn, gi)
+            ref = np.abs(d[f"eval_out_0_{gi}"])
+            # the reference's own amplitude precision gate is 1e-9 (test_nonclifford.py:160)
+            assert np.max(np.abs(got - ref)) < 1e-12, (fn, gi)
+
+
+def test_fed_uniform_streams_golden(golden_files):
+    for fn in golden_files:
+        log(f"streams {fn}")
+        d = np.load(fn)
+        prog = load_program(d)
+        f = d["f_params"]
+        s = HipProgramSampler(prog)
+        for seed in (5, 6):
+            # sample_flat's own stream (PCG64 draws in its order): reference output
+            got = sample_flat(prog, f, np.random.default_rng(seed), sampler=s)
+            assert np.array_equal(got, d[f"sample_seed_{seed}"]), (fn, seed)
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_fed_uniform_streams_shape(seed):
+    prog = synth.shape_program(n_q=6, t=6, F=8, n_out=6, seed=20 + seed, n_components=2)
+    B = 96
+    f = synth.random_f(B, 8, 0.5, seed)
+    U = np.random.default_rng(seed).random((B, 6))
+    got = HipProgramSampler(prog).sample_uniforms(f, U)
+    want = exact.sample_program(prog, f, U)
+    assert np.array_equal(got, want)
+
+
+def test_seeded_stream_is_the_reference_kokkos_stream():
+    prog = synth.product_program([synth.htkh_program(1), synth.htkh_program(3, noisy_f=True),
+                                  synth.shape_program(n_q=5, t=4, F=3, n_out=4, seed=3)])
+    B = 300
+    num_f = sum(len(c.f_selection) for c in prog.components)
+    f = synth.random_f(B, num_f, 0.4, 9)
+    seed = 1234
+    got = HipProgramSampler(prog).sample(f, seed)
+    # reference: component k uses seed + 0x9E37 k, xorshift64 per shot
+    cols = []
+    for k, comp in enumerate(prog.components):
+        cols.append(exact.kokkos_uniforms(seed + 0x9E37 * k, B, len(comp.graphs) - 1))
+    U = np.concatenate(cols, axis=1)
+    want = exact.sample_program(prog, f, U)
+    assert np.array_equal(got, want)
+
+
+def test_component_api_edges():
+    prog = synth.htkh_program(1, noisy_f=True)
+    gds = [graph_to_dict(g) for g in prog.components[0].graphs]
+    kc = Component(gds, 1)
+    with pytest.raises(ValueError, match="column count"):
+        kc.sample(np.zeros((4, 2), np.uint8), 1)
+    with pytest.raises(TypeError):
+        kc.sample(np.zeros((4, 1), np.int64), 1)
+    out = kc.sample(np.zeros((0, 1), np.uint8), 1)
+    assert out.shape == (0, 1)
+    # n_out = 0: (B, 1) of zeros
+    k0 = Component([graph_to_dict(synth.constant_graph(0))], 0)
+    assert k0.n_out == 0
+    assert np.array_equal(k0.sample(np.zeros((5, 0), np.uint8), 3), np.zeros((5, 1), np.uint8))
+
+
+@pytest.mark.parametrize("k", [1, 2, 3])
+def test_htkh_statistics(k):
+    """P(1) = sin^2(k pi/8) within 5 sigma at 20k shots, seed 11 — the known
+    answer of the reference's tests/test_nonclifford.py:105-121."""
+    prog = synth.split_program(synth.htkh_program(k), 5, seed=k)
+    n = 20_000
+    exact_p = math.sin(k * math.pi / 8) ** 2
+    sigma = math.sqrt(exact_p * (1 - exact_p) / n)
+    p1 = HipProgramSampler(prog).sample(np.zeros((n, 0), np.uint8), seed=11)[:, 0].mean()
+    assert abs(p1 - exact_p) < 5 * sigma
+
+
+def test_ghz_and_noise_statistics():
+    prog = synth.product_program([synth.single_program(synth.ghz_graphs(6)),
+                                  synth.htkh_program(2, noisy_f=True)])
+    n = 40_000
+    f = synth.random_f(n, 1, 0.25, 4)
+    out = HipProgramSampler(prog).sample(f, seed=7)
+    # output_order reversed: the GHZ block is the last 6 columns
+    ghz = out[:, 1:]
+    assert np.all(ghz == ghz[:, :1])
+    assert abs(ghz[:, 0].mean() - 0.5) < 5 * math.sqrt(0.25 / n)
+    p = 0.75 * math.sin(2 * math.pi / 8) ** 2 + 0.25 * math.cos(2 * math.pi / 8) ** 2
+    assert abs(out[:, 0].mean() - p) < 5 * math.sqrt(p * (1 - p) / n)
+
+
+def test_shot_range_split_is_identical():
+    import torch
+    from ksim_amd import _lib
+    import ctypes as C
+    prog = synth.shape_program(n_q=6, t=6, F=5, n_out=7, seed=31)
+    s = HipProgramSampler(prog)
+    B = 1000
+    f = torch.from_numpy(synth.random_f(B, 5, 0.5, 2)).cuda()
+    full = torch.zeros((B, 7), dtype=torch.uint8, device="cuda")
+    parts = torch.zeros((B, 7), dtype=torch.uint8, device="cuda")
+    lib = _lib.load()
+    _lib.check(lib.ksim_program_sample_device(s.handle, f.data_ptr(), B, 5, 0, 99, None,
+                                              full.data_ptr(), None))
+    cut = 437
+    _lib.check(lib.ksim_program_sample_device(s.handle, f.data_ptr(), cut, 5, 0, 99, None,
+                                              parts.data_ptr(), None))
+    _lib.check(lib.ksim_program_sample_device(s.handle, f[cut:].data_ptr(), B - cut, 5, cut, 99,
+                                              None, parts[cut:].data_ptr(), None))
+    _lib.check(lib.ksim_program_check(s.handle, None))
+    torch.cuda.synchronize()
+    assert torch.equal(full, parts)
+    assert torch.equal(full.cpu(), torch.from_numpy(s.sample(f.cpu().numpy(), 99)))

@@ -168,12 +168,82 @@ def plain_result(name, steps, warmup, with_cpu=True):
     return res
 
 
+def batched_result(steps, warmup, n_programs=64):
+    """config4: the same 64 programs sampled (a) by one plain launch each,
+    back to back on one stream, and (b) by one persistent batched launch
+    replayed from a hipGraph; both with resident inputs."""
+    import torch
+    from ksim_amd import synth
+    from ksim_amd.batch import BatchedSampler
+    from ksim_amd.sampler import HipProgramSampler
+    specs = synth.config4_specs(n_programs=n_programs)
+    log(f"[bench] config4: building {len(specs)} programs")
+    samplers, fs, seeds = [], [], []
+    for k, sp in enumerate(specs):
+        samplers.append(HipProgramSampler(synth.program_from_spec(sp)))
+        fs.append(torch.from_numpy(synth.random_f(sp["shots"], sp["F"], 1e-3, 500 + k)).cuda())
+        seeds.append(7000 + k)
+    outs = [torch.zeros((sp["shots"], s.num_outputs), dtype=torch.uint8, device="cuda")
+            for sp, s in zip(specs, samplers)]
+    from ksim_amd import _lib
+    lib = _lib.load()
+
+    def plain_step():
+        st = torch.cuda.current_stream().cuda_stream
+        for s, f, o, sd in zip(samplers, fs, outs, seeds):
+            _lib.check(lib.ksim_program_sample_device(s.handle, f.data_ptr() if f.numel() else None,
+                                                      f.shape[0], f.shape[1], 0, sd, None,
+                                                      o.data_ptr(), st))
+    batch = BatchedSampler(samplers, fs, seeds)
+    total = batch.total_shots
+    wall_p, dev_p = time_steps(plain_step, steps, warmup)
+    for s in samplers:
+        s_check = lib.ksim_program_check(s.handle, torch.cuda.current_stream().cuda_stream)
+        _lib.check(s_check)
+    wall_b, dev_b = time_steps(lambda: batch.run(use_graph=True), steps, warmup)
+    batch.check()
+    # same bits either way
+    same = all(torch.equal(o, bo[:, :o.shape[1]]) for o, bo in zip(outs, batch.out))
+    plain_sps = total * steps / wall_p
+    batch_sps = total * steps / wall_b
+    kern_b = dev_b / steps
+    achieved = batch.ops / kern_b
+    return dict(programs=len(specs), shots=total, plain_shots_per_s=plain_sps,
+                batched_shots_per_s=batch_sps, speedup=batch_sps / plain_sps,
+                plain_ms=wall_p / steps * 1e3, batched_ms=wall_b / steps * 1e3,
+                batched_equals_plain=bool(same), ops_total=batch.ops,
+                roofline=dict(bound="valu_int64", achieved=achieved, peak=INT64_PEAK,
+                              unit="int64-op/s", frac=achieved / INT64_PEAK, traffic=None))
+
+
+def tcount_sweep(ts=(0, 4, 8, 12, 16, 20, 24, 32, 40), shots=100_000, n_q=30):
+    """plain launch, n_q = 30, F = 2t, 30 outputs, 100k shots per point."""
+    import torch
+    from ksim_amd import synth
+    rows = []
+    for t in ts:
+        prog = synth.shape_program(n_q=n_q, t=t, F=2 * t, n_out=n_q, seed=900 + t)
+        pl = Plain(prog, shots, 2 * t, seed=t)
+        wall, dev = time_steps(pl.step, 2, 1)
+        pl.check()
+        kern = dev / 2
+        rows.append(dict(t=t, terms_mean=float(np.mean([g.pf_phase_idx.shape[0]
+                                                         for g in prog.components[0].graphs])),
+                         shots_per_s=shots * 2 / wall, us_per_shot=wall / 2 / shots * 1e6,
+                         kernel_ms=kern * 1e3, valu_int64_frac=pl.ops * shots / kern / INT64_PEAK))
+        log(f"[bench] T sweep t={t}: {rows[-1]['us_per_shot']:.4f} us/shot")
+        del pl
+        torch.cuda.empty_cache()
+    return rows
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="config2")
+    ap.add_argument("--workload", default="config2",
+                    help="config1 | config2 | config3 | config4 (batched)")
     ap.add_argument("--suite", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -183,9 +253,13 @@ def main():
     import torch
     assert torch.cuda.is_available(), "no GPU"
     torch.cuda.set_device(0)
+    if args.workload == "config4":
+        r4 = batched_result(args.steps, args.warmup)
+        print(json.dumps({"metric": "config4 batched", **r4}), flush=True)
+        return
     res = plain_result(args.workload, args.steps, args.warmup, with_cpu=not args.no_cpu)
     line = {
-        "metric": "shots/s (exact Z[w] sampler, plain launch)",
+        "metric": "shots/s & \u00b5s/shot vs T-count, plain vs batched (exact); p50/p99 latency",
         "value": res["shots_per_s"],
         "unit": "shots/s",
         "n_gpus": 1,
@@ -210,6 +284,8 @@ def main():
             r = plain_result(name, max(2, args.steps // 2), 1, with_cpu=not args.no_cpu)
             line["suite"][name] = {k: r[k] for k in ("shots_per_s", "us_per_shot", "kernel_ms",
                                                      "roofline", "cpu_baseline") if k in r}
+        line["suite"]["config4"] = batched_result(max(2, args.steps // 2), 1)
+        line["suite"]["tcount_sweep"] = tcount_sweep()
     print(json.dumps(line), flush=True)
 
 

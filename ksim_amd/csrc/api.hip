@@ -111,7 +111,7 @@ __global__ void k_fill_u8(uint8_t *p, uint8_t v, int64_t n) {
 
 // Plain path: one lane per shot, grid (ceil(B/256), n_comp).
 template <int WL>
-__global__ __launch_bounds__(kBlock) void k_sample(ShotArgs a, const int4 *__restrict__ gtab,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_sample(ShotArgs a, const int4 *__restrict__ gtab,
                                                   const int4 *__restrict__ pitab) {
   __shared__ int4 s_tab[TAB_N];
   load_table(s_tab, gtab);

@@ -87,6 +87,18 @@ __device__ inline void zmul64(int64_t &a, int64_t &b, int64_t &c, int64_t &d, in
   a = na; b = nb; c = nc; d = nd;
 }
 
+// acc += x * 2^(32 L + r) for wave-uniform r < 32, L < 4 (mod 2^128)
+__device__ __forceinline__ void add_shifted_uniform(i128_t &acc, int64_t x, int r, int L) {
+  __builtin_assume(r >= 0 && r < 32);
+  const u128_t y = (u128_t)(i128_t)x << r;  // 96 significant bits
+  switch (L) {
+    case 0: acc = (i128_t)((u128_t)acc + y); break;
+    case 1: acc = (i128_t)((u128_t)acc + (y << 32)); break;
+    case 2: acc = (i128_t)((u128_t)acc + (y << 64)); break;
+    default: acc = (i128_t)((u128_t)acc + (y << 96)); break;
+  }
+}
+
 // -------------------------------------------------------------- one graph
 // Exact amplitude of one graph for this lane's parameter words pv[0..W32):
 // acc = sum_terms value * 2^{-e0}; the caller scales by 2^{GH_E0}.
@@ -183,9 +195,23 @@ __device__ __forceinline__ void eval_graph(const uint32_t *__restrict__ P,
              (int32_t)th[TH_R3]);
     }
     if (dead & 1u) x0 = x1 = x2 = x3 = 0;
-    const int sh = q + E;
+    int sh = q + E;
     if (sh < 0 || sh > SHIFT_MAX) {
       if (x0 | x1 | x2 | x3) err |= ERR_SHIFT;
+      x0 = x1 = x2 = x3 = 0;
+      sh = 0;
+    }
+    // add X * 2^sh into the four 128-bit sums.  The shift is almost always
+    // the same on every lane (it varies only through phase-pair records), so
+    // the common path adds 32-bit limbs at a wave-uniform offset.
+    const int sh0 = __builtin_amdgcn_readfirstlane(sh);
+    if (dead & 1u) sh = sh0;  // a vanished term adds zero at any shift
+    if (__builtin_amdgcn_ballot_w64(sh != sh0) == 0) {
+      const int r = sh0 & 31, L = sh0 >> 5;
+      add_shifted_uniform(acc.a, x0, r, L);
+      add_shifted_uniform(acc.b, x1, r, L);
+      add_shifted_uniform(acc.c, x2, r, L);
+      add_shifted_uniform(acc.d, x3, r, L);
     } else {
       acc.a += (i128_t)((u128_t)(i128_t)x0 << sh);
       acc.b += (i128_t)((u128_t)(i128_t)x1 << sh);
@@ -255,13 +281,33 @@ __device__ __forceinline__ void run_shot(const ShotArgs &a, int comp, int64_t s,
   const uint32_t *__restrict__ ghs = P + ch[CH_GRAPH_OFF];
   const uint8_t *frow = a.f + s * (int64_t)a.num_f;
   uint32_t pv[WL];
+  const uint32_t fsel0 = ch[CH_FSEL0];
+  if (fsel0 != 0xffffffffu && (((uint32_t)a.num_f | fsel0 | (uint32_t)(uintptr_t)a.f) & 3u) == 0) {
+    // contiguous selection on a dword-aligned row: 4 f bytes per load,
+    // bit 0 of each byte gathered by one multiply (bits 24..27 of the product)
+    const uint32_t *row4 = (const uint32_t *)(frow + fsel0);
 #pragma unroll
-  for (int w = 0; w < WL; ++w) {
-    uint32_t word = 0;
-    const int j0 = 32 * w;
-    const int jn = min(32, F - j0);
-    for (int jj = 0; jj < jn; ++jj) word |= (uint32_t)(frow[fsel[j0 + jj]] & 1u) << jj;
-    pv[w] = word;
+    for (int w = 0; w < WL; ++w) {
+      uint32_t word = 0;
+      const int j0 = 32 * w;
+      const int jn = min(32, F - j0);
+      int jj = 0;
+      for (; jj + 4 <= jn; jj += 4) {
+        const uint32_t x = row4[(j0 + jj) >> 2] & 0x01010101u;
+        word |= ((x * 0x01020408u) >> 24) << jj;
+      }
+      for (; jj < jn; ++jj) word |= (uint32_t)(frow[fsel0 + j0 + jj] & 1u) << jj;
+      pv[w] = word;
+    }
+  } else {
+#pragma unroll
+    for (int w = 0; w < WL; ++w) {
+      uint32_t word = 0;
+      const int j0 = 32 * w;
+      const int jn = min(32, F - j0);
+      for (int jj = 0; jj < jn; ++jj) word |= (uint32_t)(frow[fsel[j0 + jj]] & 1u) << jj;
+      pv[w] = word;
+    }
   }
   // per-component stream: seed + 0x9E37*k (src/python/ksim/sample/sampler.py:63)
   uint64_t rs = rng_seed(a.seed + 0x9E37ull * (uint64_t)comp, (uint64_t)(a.shot_base + s));

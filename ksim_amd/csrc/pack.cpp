@@ -435,7 +435,12 @@ std::vector<uint32_t> pack_program(const ksim_graph_view *graphs, const int32_t 
     w[ch + CH_WMAX] = (uint32_t)wc;
     w[ch + CH_COLBASE] = (uint32_t)oi;
     w[ch + CH_FSEL_OFF] = (uint32_t)w.size();
-    for (int j = 0; j < F; ++j) w.push_back((uint32_t)f_selection[fi + j]);
+    bool contig = true;
+    for (int j = 0; j < F; ++j) {
+      w.push_back((uint32_t)f_selection[fi + j]);
+      if (f_selection[fi + j] != f_selection[fi] + j) contig = false;
+    }
+    w[ch + CH_FSEL0] = (F > 0 && contig) ? (uint32_t)f_selection[fi] : 0xffffffffu;
     w[ch + CH_DEST_OFF] = (uint32_t)w.size();
     for (int j = 0; j < n_out; ++j) {
       const int d = dest_col[oi + j];

@@ -213,7 +213,8 @@ def shape_graph(G: int, P: int, n_q: int, t: int, rng: np.random.Generator,
     fraction in {0, 4} (parity projectors); half-pi coefficients in {2, 6};
     phase-pair phases uniform mod 8; prefactor phase uniform, floatfactor
     a small Z[omega] element (a ``generic_ff`` fraction drawn from
-    [-2, 2]^4), power2 uniform in [-n_q, 0].
+    [-2, 2]^4), power2 = a per-graph base uniform in [-n_q, 0] plus a per-term
+    jitter in [-3, 0] (the terms of one decomposition share a scale).
     """
     Tnp = n_q + t
     Thp = Tpp = max(1, math.ceil(n_q / 4))
@@ -246,7 +247,7 @@ def shape_graph(G: int, P: int, n_q: int, t: int, rng: np.random.Generator,
         r[np.all(r == 0, axis=1), 0] = 1
         ff[gen] = r
     g.pf_floatfactor[:] = ff
-    g.pf_power2[:] = rng.integers(-n_q, 1, size=G)
+    g.pf_power2[:] = int(rng.integers(-n_q, 1)) + rng.integers(-3, 1, size=G)
     return g
 
 

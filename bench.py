@@ -223,9 +223,14 @@ def tcount_sweep(ts=(0, 4, 8, 12, 16, 20, 24, 32, 40), shots=100_000, n_q=30):
     rows = []
     for t in ts:
         prog = synth.shape_program(n_q=n_q, t=t, F=2 * t, n_out=n_q, seed=900 + t)
-        pl = Plain(prog, shots, 2 * t, seed=t)
-        wall, dev = time_steps(pl.step, 2, 1)
-        pl.check()
+        try:
+            pl = Plain(prog, shots, 2 * t, seed=t)
+            wall, dev = time_steps(pl.step, 2, 1)
+            pl.check()
+        except Exception as e:  # report the point, keep the sweep going
+            rows.append(dict(t=t, error=str(e)))
+            log(f"[bench] T sweep t={t}: {e}")
+            continue
         kern = dev / 2
         rows.append(dict(t=t, terms_mean=float(np.mean([g.pf_phase_idx.shape[0]
                                                          for g in prog.components[0].graphs])),

@@ -173,26 +173,22 @@ __device__ __forceinline__ void eval_graph(const uint32_t *__restrict__ P,
     const int q = V >> 2, r = V & 3;
     const int n2 = N + 2 * q;
     const int s2 = (S + 2 * q) & 7;
-    int64_t x0 = 0, x1 = 0, x2 = 0, x3 = 0;
+    int64_t x0 = 0, x1 = 0, x2 = 0, x3 = 0;  // < 2^30 from the table
     if (n2 < -TAB_R || n2 > TAB_R) {
       err |= ERR_TAB_RANGE;
     } else {
       const int4 e = tab[((n2 + TAB_R) * 4 + r) * 8 + s2];
       x0 = e.x; x1 = e.y; x2 = e.z; x3 = e.w;
     }
-    if (flags & TF_PI) {
-      if (A | Bq) {
-        if (A >= PI_MAX || Bq >= PI_MAX) {
-          err |= ERR_PI_RANGE;
-        } else {
-          const int4 p = pitab[A * PI_MAX + Bq];
-          zmul64(x0, x1, x2, x3, p.x, p.y, p.z, p.w);
-        }
+    if ((flags & TF_PI) && (A | Bq)) {
+      if (A >= PI_MAX || Bq >= PI_MAX) {
+        err |= ERR_PI_RANGE;
+        x0 = x1 = x2 = x3 = 0;
+      } else {
+        // (< 2^30) x (< 2^24) products, four per coefficient: exact in int64
+        const int4 p = pitab[A * PI_MAX + Bq];
+        zmul64(x0, x1, x2, x3, p.x, p.y, p.z, p.w);
       }
-    }
-    if (flags & TF_RESIDUAL) {
-      zmul64(x0, x1, x2, x3, (int32_t)th[TH_R0], (int32_t)th[TH_R1], (int32_t)th[TH_R2],
-             (int32_t)th[TH_R3]);
     }
     if (dead & 1u) x0 = x1 = x2 = x3 = 0;
     int sh = q + E;
@@ -201,22 +197,36 @@ __device__ __forceinline__ void eval_graph(const uint32_t *__restrict__ P,
       x0 = x1 = x2 = x3 = 0;
       sh = 0;
     }
-    // add X * 2^sh into the four 128-bit sums.  The shift is almost always
-    // the same on every lane (it varies only through phase-pair records), so
-    // the common path adds 32-bit limbs at a wave-uniform offset.
-    const int sh0 = __builtin_amdgcn_readfirstlane(sh);
-    if (dead & 1u) sh = sh0;  // a vanished term adds zero at any shift
-    if (__builtin_amdgcn_ballot_w64(sh != sh0) == 0) {
-      const int r = sh0 & 31, L = sh0 >> 5;
-      add_shifted_uniform(acc.a, x0, r, L);
-      add_shifted_uniform(acc.b, x1, r, L);
-      add_shifted_uniform(acc.c, x2, r, L);
-      add_shifted_uniform(acc.d, x3, r, L);
+    if (flags & TF_RESIDUAL) {
+      // a floatfactor that does not factor: one Z[w] multiply in 128 bits
+      const i128_t R0 = (int32_t)th[TH_R0], R1 = (int32_t)th[TH_R1];
+      const i128_t R2 = (int32_t)th[TH_R2], R3 = (int32_t)th[TH_R3];
+      const i128_t y0 = x0 * R0 + x1 * R3 - x2 * R2 + x3 * R1;
+      const i128_t y1 = x0 * R1 + x1 * R0 + x2 * R3 + x3 * R2;
+      const i128_t y2 = x0 * R2 + x1 * R1 + x2 * R0 - x3 * R3;
+      const i128_t y3 = x0 * R3 - x1 * R2 - x2 * R1 + x3 * R0;
+      acc.a += (i128_t)((u128_t)y0 << sh);
+      acc.b += (i128_t)((u128_t)y1 << sh);
+      acc.c += (i128_t)((u128_t)y2 << sh);
+      acc.d += (i128_t)((u128_t)y3 << sh);
     } else {
-      acc.a += (i128_t)((u128_t)(i128_t)x0 << sh);
-      acc.b += (i128_t)((u128_t)(i128_t)x1 << sh);
-      acc.c += (i128_t)((u128_t)(i128_t)x2 << sh);
-      acc.d += (i128_t)((u128_t)(i128_t)x3 << sh);
+      // add X * 2^sh into the four 128-bit sums.  The shift is almost always
+      // the same on every lane (it varies only through phase-pair records),
+      // so the common path adds 32-bit limbs at a wave-uniform offset.
+      const int sh0 = __builtin_amdgcn_readfirstlane(sh);
+      if (dead & 1u) sh = sh0;  // a vanished term adds zero at any shift
+      if (__builtin_amdgcn_ballot_w64(sh != sh0) == 0) {
+        const int rr = sh0 & 31, L = sh0 >> 5;
+        add_shifted_uniform(acc.a, x0, rr, L);
+        add_shifted_uniform(acc.b, x1, rr, L);
+        add_shifted_uniform(acc.c, x2, rr, L);
+        add_shifted_uniform(acc.d, x3, rr, L);
+      } else {
+        acc.a += (i128_t)((u128_t)(i128_t)x0 << sh);
+        acc.b += (i128_t)((u128_t)(i128_t)x1 << sh);
+        acc.c += (i128_t)((u128_t)(i128_t)x2 << sh);
+        acc.d += (i128_t)((u128_t)(i128_t)x3 << sh);
+      }
     }
     th = th + TH_DWORDS + th[TH_SIZE];
   }

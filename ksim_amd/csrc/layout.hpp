@@ -37,8 +37,8 @@ constexpr int TAB_R = 22;
 constexpr int TAB_N = (2 * TAB_R + 1) * 4 * 8;  // entries of 4 x int32
 inline int tab_index(int n, int r, int s) { return ((n + TAB_R) * 4 + r) * 8 + s; }
 
-// pi^a pibar^b for a, b < PI_MAX (int32 coefficients)
-constexpr int PI_MAX = 8;
+// pi^a pibar^b for a, b < PI_MAX (int32 coefficients, < 2^24)
+constexpr int PI_MAX = 16;
 
 // term header (dwords)
 enum TermHdr : int {

@@ -308,15 +308,15 @@ void pack_graph(const ksim_graph_view &g, std::vector<uint32_t> &out, uint32_t *
     rg[k] = r;
     if (k == 0 || r.emin < e0) e0 = r.emin;
     if (k == 0 || r.emax > emax_all) emax_all = r.emax;
-    int bits = 30;  // table entries < 2^30
+    // table entry (< 2^30) x pi power (< 2^24, four products) stays in int64;
+    // a residual floatfactor multiplies in 128 bits
+    int bits = 30;
     if (r.amax > 0 || r.bmax > 0) bits += pi_table_bits(r.amax, r.bmax) + 2;
     if (tb.residual) bits += i128_bits(zw_maxabs(tb.R)) + 2;
     xbits = std::max(xbits, bits);
   }
   const int max_shift = terms.empty() ? 0 : emax_all - e0;
   if (!terms.empty()) {
-    if (xbits > 62)
-      throw std::invalid_argument("term value exceeds the int64 exact range of the sampler");
     if (xbits + max_shift + ceil_log2((int64_t)terms.size()) + 1 > SHIFT_MAX)
       throw std::invalid_argument(
           "term exponent spread exceeds the 128-bit exact sum of the sampler");

@@ -47,7 +47,8 @@ def test_eval_exact_golden(golden_files):
 @pytest.mark.parametrize("shape", [dict(n_q=8, t=8, F=6, n_out=5, generic_ff=0.5, proj_frac=0.2),
                                    dict(n_q=6, t=4, F=40, n_out=8),
                                    dict(n_q=5, t=10, F=90, n_out=60, proj_frac=0.0),
-                                   dict(n_q=9, t=0, F=100, n_out=29)])
+                                   dict(n_q=9, t=0, F=100, n_out=29),
+                                   dict(n_q=6, t=36, F=72, n_out=2, generic_ff=0.3)])
 def test_eval_exact_shapes(shape):
     log(f"eval_exact shape {shape}")
     prog = synth.shape_program(seed=5, **shape)

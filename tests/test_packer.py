@@ -154,3 +154,17 @@ def test_multi_component_layout(tables):
                                   synth.single_program(synth.ghz_graphs(3)),
                                   synth.htkh_program(3, noisy_f=True)])
     check_program(prog, tables, all_rows=True)
+
+
+@pytest.mark.parametrize("t", [32, 40])
+def test_high_t_count(tables, t):
+    # many phase pairs per term: pi exponents up to ~t/8, wide term counts
+    prog = synth.shape_program(n_q=6, t=t, F=2 * t, n_out=2, seed=t)
+    pk = Packed(pack_words(prog), tables)
+    rng = np.random.default_rng(t)
+    comp = prog.components[0]
+    for i in (0, 2):
+        g = comp.graphs[i]
+        d = graph_to_dict(g)
+        bits = rng.integers(0, 2, size=g.np_params.shape[2])
+        assert exact.canonical(*pk.eval_graph(0, i, bits)) == exact.graph_value(d, bits)

@@ -151,7 +151,7 @@ __global__ __launch_bounds__(kBlock) void k_eval(EvalArgs a, const int4 *__restr
   }
   Acc4 acc;
   uint32_t err = 0;
-  eval_graph_any<WL>(a.prog, a.gh, pv, s_tab, pitab, acc, err);
+  eval_graph_any<WL>(as_const(a.prog), as_const(a.gh), pv, s_tab, pitab, acc, err);
   const int e0 = (int)a.gh[GH_E0];
   if (a.mode == 0) {
     int64_t *c = a.coeffs + s * 8;
@@ -189,12 +189,11 @@ __global__ void k_reset_cursor(uint32_t *cursor, uint32_t *err) {
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_batch(const BatchProg *__restrict__ progs,
-                                                  const BatchItem *__restrict__ items,
-                                                  uint32_t n_items, uint32_t *cursor,
-                                                  uint32_t *err, int32_t wave_shots,
-                                                  const int4 *__restrict__ gtab,
-                                                  const int4 *__restrict__ pitab) {
+template <int WL>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_batch(
+    const BatchProg *__restrict__ progs, const BatchItem *__restrict__ items, uint32_t n_items,
+    uint32_t *cursor, uint32_t *err, int32_t wave_shots, const int4 *__restrict__ gtab,
+    const int4 *__restrict__ pitab) {
   __shared__ int4 s_tab[TAB_N];
   load_table(s_tab, gtab);
   const int lane = threadIdx.x & 63;
@@ -211,14 +210,7 @@ __global__ __launch_bounds__(kBlock) void k_batch(const BatchProg *__restrict__ 
     a.num_f = bp.num_f; a.num_outputs = bp.num_outputs; a.tot_out = bp.tot_out; a.pad = 0;
     for (int sub = 0; sub < wave_shots; sub += 64) {
       const int64_t s = it.shot0 + sub + lane;
-      if (s < bp.B) {
-        switch (bp.wmax) {
-          case 1: run_shot<1>(a, it.comp, s, s_tab, pitab); break;
-          case 2: run_shot<2>(a, it.comp, s, s_tab, pitab); break;
-          case 4: run_shot<4>(a, it.comp, s, s_tab, pitab); break;
-          default: run_shot<8>(a, it.comp, s, s_tab, pitab); break;
-        }
-      }
+      if (s < bp.B) run_shot<WL>(a, it.comp, s, s_tab, pitab);
     }
   }
 }
@@ -616,6 +608,7 @@ struct ksim_batch {
   uint32_t n_items = 0;
   uint32_t *d_cursor = nullptr;  // [0] cursor, [1] err
   int wave_shots = 64;
+  int wl = 1;  // lane parameter words: the widest program of the batch
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int graph_blocks = 0;
@@ -643,6 +636,7 @@ int ksim_batch_create(ksim_program *const *progs, int32_t n, const uint8_t *cons
       if (B[k] > 0 && ((!d_f[k] && f_cols[k] > 0) || !d_out[k])) throw std::invalid_argument("NULL buffer");
       if (f_cols[k] < p->num_f) throw std::invalid_argument("f matrix has fewer columns than f_selection needs");
       b->progs.push_back(p);
+      b->wl = std::max(b->wl, p->wl);
       hp[k] = BatchProg{p->d_prog, d_f[k], d_out[k], seed[k], B[k], f_cols[k], p->num_outputs,
                         p->tot_out, p->wl};
       for (int c = 0; c < p->n_comp; ++c) {
@@ -683,8 +677,16 @@ static void batch_enqueue(ksim_batch *b, int blocks, hipStream_t st) {
   DevTables t = tables_for_current_device();
   hipLaunchKernelGGL(k_reset_cursor, dim3(1), dim3(64), 0, st, b->d_cursor, b->d_cursor + 1);
   hip_check(hipGetLastError(), "k_reset_cursor");
-  hipLaunchKernelGGL(k_batch, dim3(blocks), dim3(kBlock), 0, st, b->d_progs, b->d_items,
-                     b->n_items, b->d_cursor, b->d_cursor + 1, b->wave_shots, t.tab, t.pitab);
+#define KSIM_BATCH_LAUNCH(W)                                                                  \
+  hipLaunchKernelGGL(k_batch<W>, dim3(blocks), dim3(kBlock), 0, st, b->d_progs, b->d_items,    \
+                     b->n_items, b->d_cursor, b->d_cursor + 1, b->wave_shots, t.tab, t.pitab)
+  switch (b->wl) {
+    case 1: KSIM_BATCH_LAUNCH(1); break;
+    case 2: KSIM_BATCH_LAUNCH(2); break;
+    case 4: KSIM_BATCH_LAUNCH(4); break;
+    default: KSIM_BATCH_LAUNCH(8); break;
+  }
+#undef KSIM_BATCH_LAUNCH
   hip_check(hipGetLastError(), "k_batch");
 }
 

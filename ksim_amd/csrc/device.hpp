@@ -18,46 +18,61 @@ namespace ksim {
 typedef __int128 i128_t;
 typedef unsigned __int128 u128_t;
 
+// Program words are read through the constant address space: the kernels
+// never write them, and loads from this space are invariant, so the compiler
+// keeps every header and record load on the scalar path (SMEM into SGPRs)
+// even after the kernel's own output stores.
+typedef __attribute__((address_space(4))) const uint32_t cword;
+__device__ __forceinline__ const cword *as_const(const uint32_t *p) { return (const cword *)p; }
+
 struct Acc4 {
   i128_t a, b, c, d;
 };
 
 // ---------------------------------------------------------------- exact->f64
-// Correctly rounded signed 128-bit -> double.  The top 64 significant bits
-// are converted with a sticky bit OR-ed into bit 0, so the one rounding of the
-// u64 -> f64 conversion sees every discarded bit.
-__device__ __host__ inline double i128_to_double(i128_t x) {
-  const bool neg = x < 0;
-  const u128_t m = neg ? (u128_t)0 - (u128_t)x : (u128_t)x;
-  const uint64_t hi = (uint64_t)(m >> 64), lo = (uint64_t)m;
-  double r;
-  if (hi == 0) {
-    r = (double)lo;
-  } else {
-    const int s = 64 - __builtin_clzll(hi);  // 1..64
-    uint64_t top, sticky;
-    if (s == 64) {
-      top = hi;
-      sticky = lo != 0;
-    } else {
-      top = (hi << (64 - s)) | (lo >> s);
-      sticky = (lo & ((1ull << s) - 1)) != 0;
-    }
-    r = __builtin_ldexp((double)(top | sticky), s);
-  }
-  return neg ? -r : r;
+// |x| < 2^62 for a signed 128-bit x
+__device__ __forceinline__ bool fits62(i128_t x) {
+  const int64_t hi = (int64_t)(x >> 64);
+  const uint64_t lo = (uint64_t)(u128_t)x;
+  return (hi == 0 && lo < (1ull << 62)) || (hi == -1 && lo > ~((1ull << 62)));
 }
 
-// |value * 2^e| by the fixed IEEE sequence
-//   re = dbl(a) + dbl(b+d)*SQ, im = dbl(c) + dbl(b-d)*SQ, ldexp(sqrt(re^2+im^2), e)
-// (the cast of zx_eval.hpp:168-169 applied once to the exact sum).
+// bit length of |x|
+__device__ __forceinline__ int bitlen128(i128_t x) {
+  const u128_t m = x < 0 ? (u128_t)0 - (u128_t)x : (u128_t)x;
+  const uint64_t hi = (uint64_t)(m >> 64), lo = (uint64_t)m;
+  return hi ? 128 - __builtin_clzll(hi) : (lo ? 64 - __builtin_clzll(lo) : 0);
+}
+
+// |(a + b w + c i + d conj(w)) * 2^e| in float64, by one fixed sequence:
+//   s = b + d, t = b - d (exact);  k = max(0, L - 62) with L the largest bit
+//   length of |a|, |c|, |s|, |t|;  a' = a >> k etc. (floor);
+//   re = dbl(a') + dbl(s') * SQ,  im = dbl(c') + dbl(t') * SQ,
+//   |z| = ldexp(sqrt(re*re + im*im), e + k)
+// dbl() is the correctly rounded int64 -> double conversion and every float
+// step one IEEE operation (no FMA).  The value does not depend on how the
+// exact sum is represented (scaling by 2 commutes with every step), the
+// truncation costs < 2^-61 relative, and the oracle runs the same sequence.
+// The re/im formula is the cast of zx_eval.hpp:168-169, applied once to the
+// exact sum instead of to each term.
 __device__ inline double magnitude(const Acc4 &v, int e) {
 #pragma clang fp contract(off)
   const double SQ = 0.70710678118654752440;
-  const double re = __dadd_rn(i128_to_double(v.a), __dmul_rn(i128_to_double(v.b + v.d), SQ));
-  const double im = __dadd_rn(i128_to_double(v.c), __dmul_rn(i128_to_double(v.b - v.d), SQ));
+  const i128_t s = v.b + v.d, t = v.b - v.d;
+  int64_t a_, c_, s_, t_;
+  int k = 0;
+  if (fits62(v.a) && fits62(v.c) && fits62(s) && fits62(t)) {
+    a_ = (int64_t)v.a; c_ = (int64_t)v.c; s_ = (int64_t)s; t_ = (int64_t)t;
+  } else {
+    const int L = max(max(bitlen128(v.a), bitlen128(v.c)), max(bitlen128(s), bitlen128(t)));
+    k = L - 62;
+    a_ = (int64_t)(v.a >> k); c_ = (int64_t)(v.c >> k);
+    s_ = (int64_t)(s >> k); t_ = (int64_t)(t >> k);
+  }
+  const double re = __dadd_rn((double)a_, __dmul_rn((double)s_, SQ));
+  const double im = __dadd_rn((double)c_, __dmul_rn((double)t_, SQ));
   const double n2 = __dadd_rn(__dmul_rn(re, re), __dmul_rn(im, im));
-  return __builtin_ldexp(__dsqrt_rn(n2), e);
+  return __builtin_ldexp(__dsqrt_rn(n2), e + k);
 }
 
 // ------------------------------------------------------------------- RNG
@@ -99,26 +114,49 @@ __device__ __forceinline__ void add_shifted_uniform(i128_t &acc, int64_t x, int 
   }
 }
 
+// x * r exactly, x < 2^62 in magnitude, |r| < 2^31: x = xh 2^32 + xl
+__device__ __forceinline__ i128_t mul_64x32(int64_t x, int32_t r) {
+  const int64_t hi = (int64_t)(int32_t)(x >> 32) * (int64_t)r;
+  const int64_t lo = (int64_t)(uint64_t)(uint32_t)x * (int64_t)r;
+  return (i128_t)((u128_t)(i128_t)hi << 32) + (i128_t)lo;
+}
+
+// acc + (x * R) * 2^sh for a residual floatfactor R
+__device__ __forceinline__ Acc4 add_residual_term(Acc4 acc, int64_t x0, int64_t x1,
+                                                            int64_t x2, int64_t x3, int32_t R0,
+                                                            int32_t R1, int32_t R2, int32_t R3,
+                                                            int sh) {
+  const i128_t y0 = mul_64x32(x0, R0) + mul_64x32(x1, R3) - mul_64x32(x2, R2) + mul_64x32(x3, R1);
+  const i128_t y1 = mul_64x32(x0, R1) + mul_64x32(x1, R0) + mul_64x32(x2, R3) + mul_64x32(x3, R2);
+  const i128_t y2 = mul_64x32(x0, R2) + mul_64x32(x1, R1) + mul_64x32(x2, R0) - mul_64x32(x3, R3);
+  const i128_t y3 = mul_64x32(x0, R3) - mul_64x32(x1, R2) - mul_64x32(x2, R1) + mul_64x32(x3, R0);
+  acc.a += (i128_t)((u128_t)y0 << sh);
+  acc.b += (i128_t)((u128_t)y1 << sh);
+  acc.c += (i128_t)((u128_t)y2 << sh);
+  acc.d += (i128_t)((u128_t)y3 << sh);
+  return acc;
+}
+
 // -------------------------------------------------------------- one graph
 // Exact amplitude of one graph for this lane's parameter words pv[0..W32):
 // acc = sum_terms value * 2^{-e0}; the caller scales by 2^{GH_E0}.
 template <int W32, int WL>
-__device__ __forceinline__ void eval_graph(const uint32_t *__restrict__ P,
-                                           const uint32_t *__restrict__ gh,
+__device__ __forceinline__ void eval_graph(const cword *__restrict__ P,
+                                           const cword *__restrict__ gh,
                                            const uint32_t (&pv)[WL], const int4 *tab,
                                            const int4 *__restrict__ pitab, Acc4 &acc,
                                            uint32_t &err) {
   static_assert(W32 <= WL, "graph wider than the lane's parameter words");
   acc.a = acc.b = acc.c = acc.d = 0;
   const uint32_t nterms = gh[GH_NTERMS];
-  const uint32_t *__restrict__ th = P + gh[GH_TERM_OFF];
+  const cword *__restrict__ th = P + gh[GH_TERM_OFF];
   for (uint32_t t = 0; t < nterms; ++t) {
     const uint32_t nproj = th[TH_NPROJ], nppx = th[TH_NPPX], nchunk = th[TH_NCHUNK];
     const uint32_t flags = th[TH_FLAGS];
     int S = (int)th[TH_S], V = (int)th[TH_V], N = (int)th[TH_N];
     int A = (int)th[TH_A], Bq = (int)th[TH_B];
     const int E = (int)th[TH_E];
-    const uint32_t *__restrict__ rp = th + TH_DWORDS;
+    const cword *__restrict__ rp = th + TH_DWORDS;
     // PROJ: the term vanishes unless every projector parity matches
     uint32_t dead = 0;
     for (uint32_t k = 0; k < nproj; ++k, rp += W32 + 1) {
@@ -198,17 +236,10 @@ __device__ __forceinline__ void eval_graph(const uint32_t *__restrict__ P,
       sh = 0;
     }
     if (flags & TF_RESIDUAL) {
-      // a floatfactor that does not factor: one Z[w] multiply in 128 bits
-      const i128_t R0 = (int32_t)th[TH_R0], R1 = (int32_t)th[TH_R1];
-      const i128_t R2 = (int32_t)th[TH_R2], R3 = (int32_t)th[TH_R3];
-      const i128_t y0 = x0 * R0 + x1 * R3 - x2 * R2 + x3 * R1;
-      const i128_t y1 = x0 * R1 + x1 * R0 + x2 * R3 + x3 * R2;
-      const i128_t y2 = x0 * R2 + x1 * R1 + x2 * R0 - x3 * R3;
-      const i128_t y3 = x0 * R3 - x1 * R2 - x2 * R1 + x3 * R0;
-      acc.a += (i128_t)((u128_t)y0 << sh);
-      acc.b += (i128_t)((u128_t)y1 << sh);
-      acc.c += (i128_t)((u128_t)y2 << sh);
-      acc.d += (i128_t)((u128_t)y3 << sh);
+      // a floatfactor that does not factor: one Z[w] multiply in 128 bits,
+      // out of line so its registers do not burden the common path
+      acc = add_residual_term(acc, x0, x1, x2, x3, (int32_t)th[TH_R0], (int32_t)th[TH_R1],
+                              (int32_t)th[TH_R2], (int32_t)th[TH_R3], sh);
     } else {
       // add X * 2^sh into the four 128-bit sums.  The shift is almost always
       // the same on every lane (it varies only through phase-pair records),
@@ -233,8 +264,8 @@ __device__ __forceinline__ void eval_graph(const uint32_t *__restrict__ P,
 }
 
 template <int WL>
-__device__ __forceinline__ void eval_graph_any(const uint32_t *__restrict__ P,
-                                               const uint32_t *__restrict__ gh,
+__device__ __forceinline__ void eval_graph_any(const cword *__restrict__ P,
+                                               const cword *__restrict__ gh,
                                                const uint32_t (&pv)[WL], const int4 *tab,
                                                const int4 *__restrict__ pitab, Acc4 &acc,
                                                uint32_t &err) {
@@ -282,13 +313,13 @@ struct ShotArgs {
 template <int WL>
 __device__ __forceinline__ void run_shot(const ShotArgs &a, int comp, int64_t s,
                                          const int4 *tab, const int4 *__restrict__ pitab) {
-  const uint32_t *__restrict__ P = a.prog;
-  const uint32_t *__restrict__ ch = P + P[PH_DWORDS + comp];
+  const cword *__restrict__ P = as_const(a.prog);
+  const cword *__restrict__ ch = P + P[PH_DWORDS + comp];
   const int ng = (int)ch[CH_NGRAPHS], F = (int)ch[CH_F];
   const int colbase = (int)ch[CH_COLBASE];
-  const uint32_t *__restrict__ fsel = P + ch[CH_FSEL_OFF];
-  const uint32_t *__restrict__ dest = P + ch[CH_DEST_OFF];
-  const uint32_t *__restrict__ ghs = P + ch[CH_GRAPH_OFF];
+  const cword *__restrict__ fsel = P + ch[CH_FSEL_OFF];
+  const cword *__restrict__ dest = P + ch[CH_DEST_OFF];
+  const cword *__restrict__ ghs = P + ch[CH_GRAPH_OFF];
   const uint8_t *frow = a.f + s * (int64_t)a.num_f;
   uint32_t pv[WL];
   const uint32_t fsel0 = ch[CH_FSEL0];
@@ -323,19 +354,24 @@ __device__ __forceinline__ void run_shot(const ShotArgs &a, int comp, int64_t s,
   uint64_t rs = rng_seed(a.seed + 0x9E37ull * (uint64_t)comp, (uint64_t)(a.shot_base + s));
   uint32_t err = 0;
   Acc4 acc;
-  eval_graph_any<WL>(P, ghs, pv, tab, pitab, acc, err);
-  double prev = magnitude(acc, (int)ghs[GH_E0]);
   const double *urow = a.U ? a.U + s * (int64_t)a.tot_out + colbase : nullptr;
   uint8_t *orow = a.out + s * (int64_t)a.num_outputs;
-  for (int i = 0; i + 1 < ng; ++i) {
-    const uint32_t *gh = ghs + (size_t)(i + 1) * GH_DWORDS;
-    set_param_bit<WL>(pv, F + i, 1u);  // trying bit
+  double prev = 0.0;
+  // graph 0 gives the normaliser, graph i+1 the probability of output i = 1
+  // (one evaluation site, so the graph code is inlined once)
+  for (int i = 0; i < ng; ++i) {
+    const cword *gh = ghs + (size_t)i * GH_DWORDS;
+    if (i > 0) set_param_bit<WL>(pv, F + i - 1, 1u);  // trying bit
     eval_graph_any<WL>(P, gh, pv, tab, pitab, acc, err);
     const double p1 = magnitude(acc, (int)gh[GH_E0]);
-    const double u = urow ? urow[i] : rng_uniform(rs);
+    if (i == 0) {
+      prev = p1;
+      continue;
+    }
+    const double u = urow ? urow[i - 1] : rng_uniform(rs);
     const uint32_t bit = (u < __ddiv_rn(p1, prev)) ? 1u : 0u;
-    set_param_bit<WL>(pv, F + i, bit);
-    orow[dest[i]] = (uint8_t)bit;
+    set_param_bit<WL>(pv, F + i - 1, bit);
+    orow[dest[i - 1]] = (uint8_t)bit;
     prev = bit ? p1 : __dsub_rn(prev, p1);
   }
   if (err) atomicOr(a.err, err);

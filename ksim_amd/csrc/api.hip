@@ -113,11 +113,11 @@ __global__ void k_fill_u8(uint8_t *p, uint8_t v, int64_t n) {
 template <int WL>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_sample(ShotArgs a, const int4 *__restrict__ gtab,
                                                   const int4 *__restrict__ pitab) {
-  __shared__ int4 s_tab[TAB_N];
-  load_table(s_tab, gtab);
+  __shared__ LdsTables lt;
+  load_tables(&lt, gtab, pitab);
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= a.B) return;
-  run_shot<WL>(a, (int)blockIdx.y, s, s_tab, pitab);
+  run_shot<WL>(a, (int)blockIdx.y, s, &lt, gtab);
 }
 
 // Explicit parameter rows: exact value (mode 0) or |value| (mode 1) of one graph.
@@ -136,8 +136,8 @@ struct EvalArgs {
 template <int WL>
 __global__ __launch_bounds__(kBlock) void k_eval(EvalArgs a, const int4 *__restrict__ gtab,
                                                 const int4 *__restrict__ pitab) {
-  __shared__ int4 s_tab[TAB_N];
-  load_table(s_tab, gtab);
+  __shared__ LdsTables lt;
+  load_tables(&lt, gtab, pitab);
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= a.B) return;
   const uint8_t *row = a.params + s * (int64_t)a.P;
@@ -151,7 +151,7 @@ __global__ __launch_bounds__(kBlock) void k_eval(EvalArgs a, const int4 *__restr
   }
   Acc4 acc;
   uint32_t err = 0;
-  eval_graph_any<WL>(as_const(a.prog), as_const(a.gh), pv, s_tab, pitab, acc, err);
+  eval_graph_any<WL>(as_const(a.prog), as_const(a.gh), pv, &lt, gtab, acc, err);
   const int e0 = (int)a.gh[GH_E0];
   if (a.mode == 0) {
     int64_t *c = a.coeffs + s * 8;
@@ -194,8 +194,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
     const BatchProg *__restrict__ progs, const BatchItem *__restrict__ items, uint32_t n_items,
     uint32_t *cursor, uint32_t *err, int32_t wave_shots, const int4 *__restrict__ gtab,
     const int4 *__restrict__ pitab) {
-  __shared__ int4 s_tab[TAB_N];
-  load_table(s_tab, gtab);
+  __shared__ LdsTables lt;
+  load_tables(&lt, gtab, pitab);
   const int lane = threadIdx.x & 63;
   for (;;) {
     uint32_t idx = 0;
@@ -210,7 +210,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
     a.num_f = bp.num_f; a.num_outputs = bp.num_outputs; a.tot_out = bp.tot_out; a.pad = 0;
     for (int sub = 0; sub < wave_shots; sub += 64) {
       const int64_t s = it.shot0 + sub + lane;
-      if (s < bp.B) run_shot<WL>(a, it.comp, s, s_tab, pitab);
+      if (s < bp.B) run_shot<WL>(a, it.comp, s, &lt, gtab);
     }
   }
 }

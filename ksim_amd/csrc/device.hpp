@@ -137,14 +137,20 @@ __device__ __forceinline__ Acc4 add_residual_term(Acc4 acc, int64_t x0, int64_t 
   return acc;
 }
 
+// LDS-resident lookup tables of one block
+struct LdsTables {
+  int4 tab[TAB_LDS_N];          // w^s (1+w)^r (1+sqrt2)^n, |n| <= TAB_LDS_R
+  int4 pi[PI_MAX * PI_MAX];     // pi^a pibar^b
+};
+
 // -------------------------------------------------------------- one graph
 // Exact amplitude of one graph for this lane's parameter words pv[0..W32):
 // acc = sum_terms value * 2^{-e0}; the caller scales by 2^{GH_E0}.
 template <int W32, int WL>
 __device__ __forceinline__ void eval_graph(const cword *__restrict__ P,
                                            const cword *__restrict__ gh,
-                                           const uint32_t (&pv)[WL], const int4 *tab,
-                                           const int4 *__restrict__ pitab, Acc4 &acc,
+                                           const uint32_t (&pv)[WL], const LdsTables *lt,
+                                           const int4 *__restrict__ gtab, Acc4 &acc,
                                            uint32_t &err) {
   static_assert(W32 <= WL, "graph wider than the lane's parameter words");
   acc.a = acc.b = acc.c = acc.d = 0;
@@ -182,7 +188,8 @@ __device__ __forceinline__ void eval_graph(const cword *__restrict__ P,
       const uint32_t nl = rp[0], npr = rp[1];
       uint32_t x = rp[2];
       rp += CHUNK_HDR;
-#pragma unroll 2
+      // LIN records, unrolled so each scalar load fetches several records
+#pragma unroll 4
       for (uint32_t k = 0; k < nl; ++k, rp += W32 + 1) {
         uint32_t c = 0;
 #pragma unroll
@@ -211,24 +218,33 @@ __device__ __forceinline__ void eval_graph(const cword *__restrict__ P,
     const int q = V >> 2, r = V & 3;
     const int n2 = N + 2 * q;
     const int s2 = (S + 2 * q) & 7;
-    int64_t x0 = 0, x1 = 0, x2 = 0, x3 = 0;  // < 2^30 from the table
-    if (n2 < -TAB_R || n2 > TAB_R) {
-      err |= ERR_TAB_RANGE;
+    int4 e;
+    if ((unsigned)(n2 + TAB_LDS_R) <= 2u * TAB_LDS_R) {
+      e = lt->tab[((n2 + TAB_LDS_R) * 4 + r) * 8 + s2];
+    } else if ((unsigned)(n2 + TAB_R) <= 2u * TAB_R) {
+      e = gtab[((n2 + TAB_R) * 4 + r) * 8 + s2];
     } else {
-      const int4 e = tab[((n2 + TAB_R) * 4 + r) * 8 + s2];
-      x0 = e.x; x1 = e.y; x2 = e.z; x3 = e.w;
+      err |= ERR_TAB_RANGE;
+      e = make_int4(0, 0, 0, 0);
     }
+    const int32_t keep = (int32_t)(dead & 1u) - 1;  // 0 for a vanished term
+    e.x &= keep; e.y &= keep; e.z &= keep; e.w &= keep;
+    int64_t x0, x1, x2, x3;
     if ((flags & TF_PI) && (A | Bq)) {
       if (A >= PI_MAX || Bq >= PI_MAX) {
         err |= ERR_PI_RANGE;
-        x0 = x1 = x2 = x3 = 0;
-      } else {
-        // (< 2^30) x (< 2^24) products, four per coefficient: exact in int64
-        const int4 p = pitab[A * PI_MAX + Bq];
-        zmul64(x0, x1, x2, x3, p.x, p.y, p.z, p.w);
+        e = make_int4(0, 0, 0, 0);
       }
+      // table entry (< 2^30) x pi power (< 2^24): signed 32 x 32 -> 64
+      // products, four per coefficient, exact in int64
+      const int4 p = lt->pi[min(A, PI_MAX - 1) * PI_MAX + min(Bq, PI_MAX - 1)];
+      x0 = (int64_t)e.x * p.x + (int64_t)e.y * p.w - (int64_t)e.z * p.z + (int64_t)e.w * p.y;
+      x1 = (int64_t)e.x * p.y + (int64_t)e.y * p.x + (int64_t)e.z * p.w + (int64_t)e.w * p.z;
+      x2 = (int64_t)e.x * p.z + (int64_t)e.y * p.y + (int64_t)e.z * p.x - (int64_t)e.w * p.w;
+      x3 = (int64_t)e.x * p.w - (int64_t)e.y * p.z - (int64_t)e.z * p.y + (int64_t)e.w * p.x;
+    } else {
+      x0 = e.x; x1 = e.y; x2 = e.z; x3 = e.w;
     }
-    if (dead & 1u) x0 = x1 = x2 = x3 = 0;
     int sh = q + E;
     if (sh < 0 || sh > SHIFT_MAX) {
       if (x0 | x1 | x2 | x3) err |= ERR_SHIFT;
@@ -266,18 +282,18 @@ __device__ __forceinline__ void eval_graph(const cword *__restrict__ P,
 template <int WL>
 __device__ __forceinline__ void eval_graph_any(const cword *__restrict__ P,
                                                const cword *__restrict__ gh,
-                                               const uint32_t (&pv)[WL], const int4 *tab,
-                                               const int4 *__restrict__ pitab, Acc4 &acc,
+                                               const uint32_t (&pv)[WL], const LdsTables *lt,
+                                               const int4 *__restrict__ gtab, Acc4 &acc,
                                                uint32_t &err) {
   switch (gh[GH_W32]) {
-    case 1: eval_graph<1, WL>(P, gh, pv, tab, pitab, acc, err); break;
-    case 2: if constexpr (WL >= 2) eval_graph<2, WL>(P, gh, pv, tab, pitab, acc, err); break;
-    case 3: if constexpr (WL >= 3) eval_graph<3, WL>(P, gh, pv, tab, pitab, acc, err); break;
-    case 4: if constexpr (WL >= 4) eval_graph<4, WL>(P, gh, pv, tab, pitab, acc, err); break;
-    case 5: if constexpr (WL >= 5) eval_graph<5, WL>(P, gh, pv, tab, pitab, acc, err); break;
-    case 6: if constexpr (WL >= 6) eval_graph<6, WL>(P, gh, pv, tab, pitab, acc, err); break;
-    case 7: if constexpr (WL >= 7) eval_graph<7, WL>(P, gh, pv, tab, pitab, acc, err); break;
-    case 8: if constexpr (WL >= 8) eval_graph<8, WL>(P, gh, pv, tab, pitab, acc, err); break;
+    case 1: eval_graph<1, WL>(P, gh, pv, lt, gtab, acc, err); break;
+    case 2: if constexpr (WL >= 2) eval_graph<2, WL>(P, gh, pv, lt, gtab, acc, err); break;
+    case 3: if constexpr (WL >= 3) eval_graph<3, WL>(P, gh, pv, lt, gtab, acc, err); break;
+    case 4: if constexpr (WL >= 4) eval_graph<4, WL>(P, gh, pv, lt, gtab, acc, err); break;
+    case 5: if constexpr (WL >= 5) eval_graph<5, WL>(P, gh, pv, lt, gtab, acc, err); break;
+    case 6: if constexpr (WL >= 6) eval_graph<6, WL>(P, gh, pv, lt, gtab, acc, err); break;
+    case 7: if constexpr (WL >= 7) eval_graph<7, WL>(P, gh, pv, lt, gtab, acc, err); break;
+    case 8: if constexpr (WL >= 8) eval_graph<8, WL>(P, gh, pv, lt, gtab, acc, err); break;
     default: break;
   }
 }
@@ -312,7 +328,7 @@ struct ShotArgs {
 // (src/cpp/ksim/kokkos_sim.cpp:214-230, src/python/ksim/sample/evaluate.py:76-85).
 template <int WL>
 __device__ __forceinline__ void run_shot(const ShotArgs &a, int comp, int64_t s,
-                                         const int4 *tab, const int4 *__restrict__ pitab) {
+                                         const LdsTables *lt, const int4 *__restrict__ gtab) {
   const cword *__restrict__ P = as_const(a.prog);
   const cword *__restrict__ ch = P + P[PH_DWORDS + comp];
   const int ng = (int)ch[CH_NGRAPHS], F = (int)ch[CH_F];
@@ -362,7 +378,7 @@ __device__ __forceinline__ void run_shot(const ShotArgs &a, int comp, int64_t s,
   for (int i = 0; i < ng; ++i) {
     const cword *gh = ghs + (size_t)i * GH_DWORDS;
     if (i > 0) set_param_bit<WL>(pv, F + i - 1, 1u);  // trying bit
-    eval_graph_any<WL>(P, gh, pv, tab, pitab, acc, err);
+    eval_graph_any<WL>(P, gh, pv, lt, gtab, acc, err);
     const double p1 = magnitude(acc, (int)gh[GH_E0]);
     if (i == 0) {
       prev = p1;
@@ -377,8 +393,11 @@ __device__ __forceinline__ void run_shot(const ShotArgs &a, int comp, int64_t s,
   if (err) atomicOr(a.err, err);
 }
 
-__device__ inline void load_table(int4 *s_tab, const int4 *tab) {
-  for (int i = threadIdx.x; i < TAB_N; i += blockDim.x) s_tab[i] = tab[i];
+// stage the central rows of the materialisation table and the pi powers
+__device__ inline void load_tables(LdsTables *t, const int4 *gtab, const int4 *gpi) {
+  const int4 *src = gtab + (TAB_R - TAB_LDS_R) * 32;
+  for (int i = threadIdx.x; i < TAB_LDS_N; i += blockDim.x) t->tab[i] = src[i];
+  for (int i = threadIdx.x; i < PI_MAX * PI_MAX; i += blockDim.x) t->pi[i] = gpi[i];
   __syncthreads();
 }
 

@@ -36,6 +36,10 @@ constexpr int CHUNK_MAX = 63;
 constexpr int TAB_R = 22;
 constexpr int TAB_N = (2 * TAB_R + 1) * 4 * 8;  // entries of 4 x int32
 inline int tab_index(int n, int r, int s) { return ((n + TAB_R) * 4 + r) * 8 + s; }
+// the rows |n| <= TAB_LDS_R are staged in LDS per block; the outer rows are
+// read from global memory (rare)
+constexpr int TAB_LDS_R = 17;
+constexpr int TAB_LDS_N = (2 * TAB_LDS_R + 1) * 4 * 8;
 
 // pi^a pibar^b for a, b < PI_MAX (int32 coefficients, < 2^24)
 constexpr int PI_MAX = 16;

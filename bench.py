@@ -24,6 +24,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -242,6 +243,48 @@ def tcount_sweep(ts=(0, 4, 8, 12, 16, 20, 24, 32, 40), shots=100_000, n_q=30):
     return rows
 
 
+def serving_result(n_requests=48, n_pool=8, load=0.7, t_max=40, q_max=85, seed=5):
+    """config5: Poisson request stream; programs from the config-5 generator
+    (t <= 40, q <= 85) drawn from a pool of resident programs, shots
+    log-uniform in [100, 1M]; arrival rate = ``load`` x the saturation rate
+    measured on the same mix."""
+    import torch
+    from ksim_amd import synth
+    from ksim_amd.sampler import HipProgramSampler
+    from ksim_amd.serve import Request, ServingLoop, poisson_arrivals
+    rng = np.random.default_rng(seed)
+    specs = [synth.config5_spec(rng, t_max=t_max, q_max=q_max) for _ in range(n_pool)]
+    log(f"[bench] config5: building {n_pool} programs (t = {[s['t'] for s in specs]})")
+    pool = [HipProgramSampler(synth.program_from_spec(sp)) for sp in specs]
+    reqs = []
+    for i in range(n_requests):
+        k = int(rng.integers(n_pool))
+        shots = int(round(math.exp(rng.uniform(math.log(100), math.log(1_000_000)))))
+        f = synth.random_f(shots, specs[k]["F"], 1e-3, 10_000 + i)
+        reqs.append(Request(key=k, prog=pool[k], f=f, seed=20_000 + i))
+    total_ops = sum(pool[r.key].ops_per_shot * r.shots for r in reqs)
+    # saturation: the whole stream at once (no arrival gaps)
+    loop = ServingLoop(hbm_budget=96 << 30, max_slots=64, step_ms=8.0)
+    warm = [Request(key=r.key, prog=r.prog, f=r.f[:64], seed=r.seed) for r in reqs[:4]]
+    loop.run(warm, realtime=False)
+    sat = loop.run([Request(key=r.key, prog=r.prog, f=r.f, seed=r.seed) for r in reqs],
+                   realtime=False)
+    rate_ops = total_ops / sat["wall_s"]
+    lam = load * rate_ops / (total_ops / n_requests)
+    for r, t in zip(reqs, poisson_arrivals(n_requests, lam, seed)):
+        r.arrival = float(t)
+    log(f"[bench] config5: saturation {sat['wall_s']:.2f}s for {n_requests} requests; "
+        f"Poisson rate {lam:.2f}/s")
+    st = loop.run(reqs, realtime=True)
+    loop.close()
+    return dict(requests=n_requests, pool=n_pool, load=load, arrival_rate_per_s=lam,
+                pool_t=[sp["t"] for sp in specs], pool_q=[sp["n_q"] for sp in specs],
+                shots=st["shots"], p50_ms=st["p50_ms"], p99_ms=st["p99_ms"],
+                mean_ms=st["mean_ms"], shots_per_s=st["shots_per_s"], wall_s=st["wall_s"],
+                saturation_shots_per_s=sat["shots_per_s"], steps=st["steps"],
+                algorithmic_ops_per_s=rate_ops)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -261,6 +304,9 @@ def main():
     if args.workload == "config4":
         r4 = batched_result(args.steps, args.warmup)
         print(json.dumps({"metric": "config4 batched", **r4}), flush=True)
+        return
+    if args.workload == "config5":
+        print(json.dumps({"metric": "config5 serving", **serving_result()}), flush=True)
         return
     res = plain_result(args.workload, args.steps, args.warmup, with_cpu=not args.no_cpu)
     line = {
@@ -291,6 +337,7 @@ def main():
                                                      "roofline", "cpu_baseline") if k in r}
         line["suite"]["config4"] = batched_result(max(2, args.steps // 2), 1)
         line["suite"]["tcount_sweep"] = tcount_sweep()
+        line["suite"]["config5"] = serving_result()
     print(json.dumps(line), flush=True)
 
 

@@ -186,6 +186,34 @@ int ksim_batch_run(ksim_batch *b, int32_t persistent_blocks, int32_t use_graph,
 int ksim_batch_check(ksim_batch *b);
 void ksim_batch_destroy(ksim_batch *b);
 
+/* ---- scheduler: the device side of the serving loop ---------------------- */
+
+typedef struct ksim_sched ksim_sched;
+
+/* A persistent-kernel launcher whose work list is given per launch.  No
+ * reference counterpart (the reference has no serving path). */
+int ksim_sched_create(int32_t max_slots, int32_t max_items, ksim_sched **out);
+
+/* Bind slot `slot` to a resident program with its resident f matrix
+ * d_f (B, f_cols) u8, output d_out (B, num_outputs) u8 and seed; a request's
+ * shots use the same per-shot streams as ksim_program_sample_device with
+ * shot_base 0, however they are split into items. */
+int ksim_sched_set_slot(ksim_sched *q, int32_t slot, ksim_program *p,
+                        const uint8_t *d_f, int32_t f_cols, int64_t B,
+                        uint64_t seed, uint8_t *d_out);
+int ksim_sched_clear_slot(ksim_sched *q, int32_t slot);
+
+/* Enqueue one persistent launch on `stream` over n items, each 4 int32:
+ * (slot, component, first shot, shot count).  Returns after staging; the
+ * pinned staging buffer of launch k is reused by launch k+2, which waits
+ * for launch k's copies first. */
+int ksim_sched_launch(ksim_sched *q, const int32_t *items, int32_t n,
+                      int32_t blocks, void *stream);
+
+/* Synchronise `stream` and report a device exact-range error, if any. */
+int ksim_sched_check(ksim_sched *q, void *stream);
+void ksim_sched_destroy(ksim_sched *q);
+
 /* ---- host-only inspection (no device needed) ---------------------------- */
 
 /* Pack a program exactly as ksim_program_create does, without uploading:

@@ -64,6 +64,12 @@ EXPORTS = {
     "ksim_batch_run": (C.c_int, [vp, C.c_int32, C.c_int32, vp]),
     "ksim_batch_check": (C.c_int, [vp]),
     "ksim_batch_destroy": (None, [vp]),
+    "ksim_sched_create": (C.c_int, [C.c_int32, C.c_int32, C.POINTER(vp)]),
+    "ksim_sched_set_slot": (C.c_int, [vp, C.c_int32, vp, vp, C.c_int32, C.c_int64, C.c_uint64, vp]),
+    "ksim_sched_clear_slot": (C.c_int, [vp, C.c_int32]),
+    "ksim_sched_launch": (C.c_int, [vp, i32p, C.c_int32, C.c_int32, vp]),
+    "ksim_sched_check": (C.c_int, [vp, vp]),
+    "ksim_sched_destroy": (None, [vp]),
     "ksim_program_pack": (C.c_int, [C.POINTER(GraphView), i32p, i32p, i32p, C.c_int32, C.c_int32,
                                     i32p, C.c_int32, C.POINTER(C.c_uint32), C.c_int64, i64p]),
     "ksim_tables": (C.c_int, [i32p, C.c_int64, i32p, C.c_int64, i32p]),

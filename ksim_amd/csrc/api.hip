@@ -180,19 +180,23 @@ struct BatchProg {
 struct BatchItem {
   int32_t prog, comp;
   int64_t shot0;
+  int32_t count, pad;  // shots [shot0, shot0 + count) of program `prog`
 };
 
-__global__ void k_reset_cursor(uint32_t *cursor, uint32_t *err) {
+// reset the work cursor and the word `clear` (the batch clears its error
+// word per launch; the scheduler passes a scratch word and keeps its error
+// word sticky until checked)
+__global__ void k_reset_cursor(uint32_t *cursor, uint32_t *clear) {
   if (threadIdx.x == 0) {
     cursor[0] = 0;
-    err[0] = 0;
+    clear[0] = 0;
   }
 }
 
 template <int WL>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_batch(
     const BatchProg *__restrict__ progs, const BatchItem *__restrict__ items, uint32_t n_items,
-    uint32_t *cursor, uint32_t *err, int32_t wave_shots, const int4 *__restrict__ gtab,
+    uint32_t *cursor, uint32_t *err, const int4 *__restrict__ gtab,
     const int4 *__restrict__ pitab) {
   __shared__ LdsTables lt;
   load_tables(&lt, gtab, pitab);
@@ -208,9 +212,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
     a.prog = bp.prog; a.f = bp.f; a.U = nullptr; a.out = bp.out; a.err = err;
     a.B = bp.B; a.shot_base = 0; a.seed = bp.seed;
     a.num_f = bp.num_f; a.num_outputs = bp.num_outputs; a.tot_out = bp.tot_out; a.pad = 0;
-    for (int sub = 0; sub < wave_shots; sub += 64) {
+    for (int sub = 0; sub < it.count; sub += 64) {
       const int64_t s = it.shot0 + sub + lane;
-      if (s < bp.B) run_shot<WL>(a, it.comp, s, &lt, gtab);
+      if (sub + lane < it.count && s < bp.B) run_shot<WL>(a, it.comp, s, &lt, gtab);
     }
   }
 }
@@ -645,7 +649,7 @@ int ksim_batch_create(ksim_program *const *progs, int32_t n, const uint8_t *cons
         for (int i = 0; i < p->n_graphs[c]; ++i) cost += p->graph_hdr[c][i * GH_DWORDS + GH_NREC];
         for (int64_t s0 = 0; s0 < B[k]; s0 += wave_shots) {
           const int64_t ns = std::min<int64_t>(wave_shots, B[k] - s0);
-          items.push_back(Cost{cost * (double)ns, BatchItem{k, c, s0}});
+          items.push_back(Cost{cost * (double)ns, BatchItem{k, c, s0, (int32_t)ns, 0}});
         }
       }
     }
@@ -679,7 +683,7 @@ static void batch_enqueue(ksim_batch *b, int blocks, hipStream_t st) {
   hip_check(hipGetLastError(), "k_reset_cursor");
 #define KSIM_BATCH_LAUNCH(W)                                                                  \
   hipLaunchKernelGGL(k_batch<W>, dim3(blocks), dim3(kBlock), 0, st, b->d_progs, b->d_items,    \
-                     b->n_items, b->d_cursor, b->d_cursor + 1, b->wave_shots, t.tab, t.pitab)
+                     b->n_items, b->d_cursor, b->d_cursor + 1, t.tab, t.pitab)
   switch (b->wl) {
     case 1: KSIM_BATCH_LAUNCH(1); break;
     case 2: KSIM_BATCH_LAUNCH(2); break;
@@ -733,6 +737,160 @@ void ksim_batch_destroy(ksim_batch *b) {
   if (b->d_items) (void)hipFree(b->d_items);
   if (b->d_cursor) (void)hipFree(b->d_cursor);
   delete b;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ scheduler
+// A persistent-kernel launcher over an arbitrary list of (program slot,
+// component, shot range) items per launch: the device side of the serving
+// loop.  Slots bind a resident program to its resident f matrix, output and
+// seed.  Each launch stages the slot table and the item list through pinned
+// host buffers (two, alternating) into device buffers (two, alternating), so
+// the host can prepare launch k+1 while launch k runs.
+struct ksim_sched {
+  int max_slots = 0, max_items = 0;
+  std::vector<BatchProg> slots;
+  std::vector<int> slot_wl, slot_ncomp;
+  BatchProg *h_progs[2] = {nullptr, nullptr};
+  BatchItem *h_items[2] = {nullptr, nullptr};
+  BatchProg *d_progs[2] = {nullptr, nullptr};
+  BatchItem *d_items[2] = {nullptr, nullptr};
+  hipEvent_t staged[2] = {nullptr, nullptr};
+  bool staged_valid[2] = {false, false};
+  uint32_t *d_cursor = nullptr;  // [0] cursor, [1] error word
+  int next = 0;
+};
+
+extern "C" {
+
+int ksim_sched_create(int32_t max_slots, int32_t max_items, ksim_sched **out) {
+  return guarded([&] {
+    if (!out) throw std::invalid_argument("out is NULL");
+    if (max_slots <= 0 || max_items <= 0) throw std::invalid_argument("sizes must be > 0");
+    auto *q = new ksim_sched();
+    q->max_slots = max_slots;
+    q->max_items = max_items;
+    q->slots.assign(max_slots, BatchProg{nullptr, nullptr, nullptr, 0, 0, 0, 0, 0, 1});
+    q->slot_wl.assign(max_slots, 0);
+    q->slot_ncomp.assign(max_slots, 0);
+    try {
+      for (int k = 0; k < 2; ++k) {
+        hip_check(hipHostMalloc((void **)&q->h_progs[k], sizeof(BatchProg) * max_slots), "hipHostMalloc");
+        hip_check(hipHostMalloc((void **)&q->h_items[k], sizeof(BatchItem) * max_items), "hipHostMalloc");
+        hip_check(hipMalloc(&q->d_progs[k], sizeof(BatchProg) * max_slots), "hipMalloc");
+        hip_check(hipMalloc(&q->d_items[k], sizeof(BatchItem) * max_items), "hipMalloc");
+        hip_check(hipEventCreateWithFlags(&q->staged[k], hipEventDisableTiming), "hipEventCreate");
+      }
+      hip_check(hipMalloc(&q->d_cursor, 64), "hipMalloc");
+      hip_check(hipMemset(q->d_cursor, 0, 64), "hipMemset");
+      tables_for_current_device();
+    } catch (...) {
+      ksim_sched_destroy(q);
+      throw;
+    }
+    *out = q;
+  });
+}
+
+int ksim_sched_set_slot(ksim_sched *q, int32_t slot, ksim_program *p, const uint8_t *d_f,
+                        int32_t f_cols, int64_t B, uint64_t seed, uint8_t *d_out) {
+  return guarded([&] {
+    if (!q || !p) throw std::invalid_argument("NULL argument");
+    if (slot < 0 || slot >= q->max_slots) throw std::invalid_argument("slot out of range");
+    if (B < 0) throw std::invalid_argument("B < 0");
+    if (f_cols < p->num_f) throw std::invalid_argument("f matrix has fewer columns than f_selection needs");
+    if (B > 0 && ((!d_f && f_cols > 0) || !d_out)) throw std::invalid_argument("NULL buffer");
+    q->slots[slot] = BatchProg{p->d_prog, d_f, d_out, seed, B, f_cols, p->num_outputs,
+                               p->tot_out, p->wl};
+    q->slot_wl[slot] = p->wl;
+    q->slot_ncomp[slot] = p->n_comp;
+  });
+}
+
+int ksim_sched_clear_slot(ksim_sched *q, int32_t slot) {
+  return guarded([&] {
+    if (!q) throw std::invalid_argument("NULL argument");
+    if (slot < 0 || slot >= q->max_slots) throw std::invalid_argument("slot out of range");
+    q->slots[slot] = BatchProg{nullptr, nullptr, nullptr, 0, 0, 0, 0, 0, 1};
+    q->slot_wl[slot] = 0;
+    q->slot_ncomp[slot] = 0;
+  });
+}
+
+// items: n rows of (slot, component, first shot, shot count), int32 each.
+int ksim_sched_launch(ksim_sched *q, const int32_t *items, int32_t n, int32_t blocks,
+                      void *stream) {
+  return guarded([&] {
+    if (!q) throw std::invalid_argument("NULL argument");
+    if (n < 0 || n > q->max_items) throw std::invalid_argument("item count out of range");
+    if (blocks <= 0) throw std::invalid_argument("blocks must be > 0");
+    if (n == 0) return;
+    if (!items) throw std::invalid_argument("items is NULL");
+    hipStream_t st = (hipStream_t)stream;
+    const int k = q->next;
+    q->next ^= 1;
+    if (q->staged_valid[k]) hip_check(hipEventSynchronize(q->staged[k]), "hipEventSynchronize");
+    int wl = 1;
+    for (int i = 0; i < n; ++i) {
+      const int32_t slot = items[4 * i], comp = items[4 * i + 1];
+      const int32_t shot0 = items[4 * i + 2], count = items[4 * i + 3];
+      if (slot < 0 || slot >= q->max_slots || !q->slots[slot].prog)
+        throw std::invalid_argument("item names an empty slot");
+      const BatchProg &bp = q->slots[slot];
+      if (comp < 0 || comp >= q->slot_ncomp[slot])
+        throw std::invalid_argument("item component index out of range");
+      if (shot0 < 0 || count <= 0 || (int64_t)shot0 + count > bp.B)
+        throw std::invalid_argument("item shot range outside its request");
+      wl = std::max(wl, q->slot_wl[slot]);
+      q->h_items[k][i] = BatchItem{slot, comp, shot0, count, 0};
+    }
+    std::memcpy(q->h_progs[k], q->slots.data(), sizeof(BatchProg) * q->max_slots);
+    hip_check(hipMemcpyAsync(q->d_progs[k], q->h_progs[k], sizeof(BatchProg) * q->max_slots,
+                             hipMemcpyHostToDevice, st), "H2D slots");
+    hip_check(hipMemcpyAsync(q->d_items[k], q->h_items[k], sizeof(BatchItem) * n,
+                             hipMemcpyHostToDevice, st), "H2D items");
+    hip_check(hipEventRecord(q->staged[k], st), "hipEventRecord");
+    q->staged_valid[k] = true;
+    DevTables t = tables_for_current_device();
+    hipLaunchKernelGGL(k_reset_cursor, dim3(1), dim3(64), 0, st, q->d_cursor, q->d_cursor + 2);
+    hip_check(hipGetLastError(), "k_reset_cursor");
+#define KSIM_SCHED_LAUNCH(W)                                                                   \
+  hipLaunchKernelGGL(k_batch<W>, dim3(blocks), dim3(kBlock), 0, st, q->d_progs[k],              \
+                     q->d_items[k], (uint32_t)n, q->d_cursor, q->d_cursor + 1, t.tab, t.pitab)
+    switch (lane_words(wl)) {
+      case 1: KSIM_SCHED_LAUNCH(1); break;
+      case 2: KSIM_SCHED_LAUNCH(2); break;
+      case 4: KSIM_SCHED_LAUNCH(4); break;
+      default: KSIM_SCHED_LAUNCH(8); break;
+    }
+#undef KSIM_SCHED_LAUNCH
+    hip_check(hipGetLastError(), "k_batch");
+  });
+}
+
+int ksim_sched_check(ksim_sched *q, void *stream) {
+  return guarded([&] {
+    if (!q) throw std::invalid_argument("NULL argument");
+    hipStream_t st = (hipStream_t)stream;
+    uint32_t e = 0;
+    hip_check(hipMemcpyAsync(&e, q->d_cursor + 1, 4, hipMemcpyDeviceToHost, st), "read err");
+    hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    if (e) throw RangeError(err_text(e));
+  });
+}
+
+void ksim_sched_destroy(ksim_sched *q) {
+  if (!q) return;
+  for (int k = 0; k < 2; ++k) {
+    if (q->staged[k]) { (void)hipEventSynchronize(q->staged[k]); (void)hipEventDestroy(q->staged[k]); }
+    if (q->h_progs[k]) (void)hipHostFree(q->h_progs[k]);
+    if (q->h_items[k]) (void)hipHostFree(q->h_items[k]);
+    if (q->d_progs[k]) (void)hipFree(q->d_progs[k]);
+    if (q->d_items[k]) (void)hipFree(q->d_items[k]);
+  }
+  if (q->d_cursor) (void)hipFree(q->d_cursor);
+  delete q;
 }
 
 }  // extern "C"

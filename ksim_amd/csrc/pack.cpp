@@ -22,7 +22,10 @@
 #include "pack.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
+#include <exception>
+#include <thread>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -401,6 +404,65 @@ std::vector<uint32_t> pack_program(const ksim_graph_view *graphs, const int32_t 
                                    int32_t n_comp, int32_t num_f, const int32_t *dest_col,
                                    int32_t num_outputs, PackStats &st) {
   if (n_comp < 0 || num_f < 0 || num_outputs < 0) throw std::invalid_argument("negative size");
+  // validate the component structure first (cheap, sequential)
+  size_t total_graphs = 0;
+  {
+    size_t gi = 0, fi = 0;
+    for (int c = 0; c < n_comp; ++c) {
+      const int ng = graphs_per_comp[c];
+      const int F = F_per_comp[c];
+      if (ng < 1) throw std::invalid_argument("a component needs at least one graph");
+      if (F < 0) throw std::invalid_argument("negative F");
+      for (int j = 0; j < F; ++j)
+        if (f_selection[fi + j] < 0 || f_selection[fi + j] >= num_f)
+          throw std::invalid_argument("f_selection index out of range");
+      // parameter count of graph i must be F + i (zx_eval.hpp:65)
+      for (int i = 0; i < ng; ++i) {
+        const ksim_graph_view &g = graphs[gi + i];
+        if (g.G > 0 && g.P != F + i)
+          throw std::invalid_argument("graph " + std::to_string(i) + " of component " +
+                                      std::to_string(c) + " has P=" + std::to_string(g.P) +
+                                      ", expected F+i=" + std::to_string(F + i));
+      }
+      if (std::max(1, (F + ng - 1 + 31) / 32) > 8)
+        throw std::invalid_argument("more than 256 parameters per component");
+      gi += ng;
+      fi += F;
+    }
+    total_graphs = gi;
+  }
+  // pack every graph independently, in parallel
+  struct Job {
+    std::vector<uint32_t> words;
+    uint32_t hdr[GH_DWORDS] = {0};
+    PackStats st;
+    std::exception_ptr err;
+  };
+  std::vector<Job> jobs(total_graphs);
+  {
+    std::atomic<size_t> next(0);
+    auto work = [&] {
+      for (;;) {
+        const size_t i = next.fetch_add(1);
+        if (i >= total_graphs) break;
+        try {
+          pack_graph(graphs[i], jobs[i].words, jobs[i].hdr, jobs[i].st);
+        } catch (...) {
+          jobs[i].err = std::current_exception();
+        }
+      }
+    };
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const unsigned nt = (unsigned)std::min<size_t>(hw, total_graphs);
+    std::vector<std::thread> pool;
+    for (unsigned t = 1; t < nt; ++t) pool.emplace_back(work);
+    work();
+    for (auto &th : pool) th.join();
+    for (auto &j : jobs)
+      if (j.err) std::rethrow_exception(j.err);
+  }
+  // lay out: program header, then per component its header, f_selection,
+  // output columns, graph headers and the graphs' term records
   std::vector<uint32_t> w(PH_DWORDS + (size_t)n_comp, 0u);
   w[PH_NCOMP] = (uint32_t)n_comp;
   w[PH_NUMF] = (uint32_t)num_f;
@@ -410,21 +472,8 @@ std::vector<uint32_t> pack_program(const ksim_graph_view *graphs, const int32_t 
   for (int c = 0; c < n_comp; ++c) {
     const int ng = graphs_per_comp[c];
     const int F = F_per_comp[c];
-    if (ng < 1) throw std::invalid_argument("a component needs at least one graph");
     const int n_out = ng - 1;
-    for (int j = 0; j < F; ++j)
-      if (f_selection[fi + j] < 0 || f_selection[fi + j] >= num_f)
-        throw std::invalid_argument("f_selection index out of range");
-    // parameter count of graph i must be F + i (zx_eval.hpp:65)
-    for (int i = 0; i < ng; ++i) {
-      const ksim_graph_view &g = graphs[gi + i];
-      if (g.G > 0 && g.P != F + i)
-        throw std::invalid_argument("graph " + std::to_string(i) + " of component " +
-                                    std::to_string(c) + " has P=" + std::to_string(g.P) +
-                                    ", expected F+i=" + std::to_string(F + i));
-    }
     const int wc = std::max(1, (F + n_out + 31) / 32);
-    if (wc > 8) throw std::invalid_argument("more than 256 parameters per component");
     wmax = std::max(wmax, wc);
     const size_t ch = w.size();
     w[PH_DWORDS + c] = (uint32_t)ch;
@@ -451,12 +500,17 @@ std::vector<uint32_t> pack_program(const ksim_graph_view *graphs, const int32_t 
     w[ch + CH_GRAPH_OFF] = (uint32_t)gh;
     w.resize(gh + (size_t)ng * GH_DWORDS, 0u);
     for (int i = 0; i < ng; ++i) {
-      uint32_t hdr[GH_DWORDS] = {0};
-      pack_graph(graphs[gi + i], w, hdr, st);
-      std::memcpy(&w[gh + (size_t)i * GH_DWORDS], hdr, sizeof(hdr));
-      ++st.graphs;
+      Job &j = jobs[gi + i];
+      j.hdr[GH_TERM_OFF] += (uint32_t)w.size();  // relative -> absolute
+      std::memcpy(&w[gh + (size_t)i * GH_DWORDS], j.hdr, sizeof(j.hdr));
+      if (w.size() + j.words.size() >= (size_t)1 << 32)
+        throw std::invalid_argument("program larger than 16 GiB");
+      w.insert(w.end(), j.words.begin(), j.words.end());
+      std::vector<uint32_t>().swap(j.words);
+      st.graphs += 1;
+      st.terms += j.st.terms;
+      st.records += j.st.records;
     }
-    if (w.size() >= (size_t)1 << 32) throw std::invalid_argument("program larger than 16 GiB");
     gi += ng;
     fi += F;
     oi += n_out;

@@ -77,27 +77,28 @@ def graph_to_dict(g: FlatScalarGraph) -> dict:
     component constructor takes.
 
     Same keys, dtypes and the complex64-as-float32-pairs view as the
-    reference's ``_graph_to_dict`` (src/python/ksim/sample/sampler.py:21-43).
+    reference's ``_graph_to_dict`` (src/python/ksim/sample/sampler.py:21-43);
+    arrays that already have the right dtype and layout are not copied.
     """
-    apx = np.array(g.pf_approx, dtype=np.complex64, order="C")
+    apx = np.ascontiguousarray(g.pf_approx, dtype=np.complex64)
     return dict(
-        np_phases=np.array(g.np_phases, dtype=np.uint8, order="C"),
-        np_params=np.array(g.np_params, dtype=np.uint8, order="C"),
-        np_counts=np.array(g.np_counts, dtype=np.int32, order="C"),
-        hp_coeffs=np.array(g.hp_coeffs, dtype=np.uint8, order="C"),
-        hp_params=np.array(g.hp_params, dtype=np.uint8, order="C"),
-        pp_psi_c=np.array(g.pp_psi_c, dtype=np.uint8, order="C"),
-        pp_psi_p=np.array(g.pp_psi_p, dtype=np.uint8, order="C"),
-        pp_phi_c=np.array(g.pp_phi_c, dtype=np.uint8, order="C"),
-        pp_phi_p=np.array(g.pp_phi_p, dtype=np.uint8, order="C"),
-        ph_alpha=np.array(g.ph_alpha, dtype=np.uint8, order="C"),
-        ph_alpha_p=np.array(g.ph_alpha_p, dtype=np.uint8, order="C"),
-        ph_beta=np.array(g.ph_beta, dtype=np.uint8, order="C"),
-        ph_beta_p=np.array(g.ph_beta_p, dtype=np.uint8, order="C"),
-        ph_counts=np.array(g.ph_counts, dtype=np.int32, order="C"),
-        pf_phase_idx=np.array(g.pf_phase_idx, dtype=np.uint8, order="C"),
-        pf_floatfactor=np.array(g.pf_floatfactor, dtype=np.int32, order="C"),
-        pf_power2=np.array(g.pf_power2, dtype=np.int32, order="C"),
+        np_phases=np.ascontiguousarray(g.np_phases, dtype=np.uint8),
+        np_params=np.ascontiguousarray(g.np_params, dtype=np.uint8),
+        np_counts=np.ascontiguousarray(g.np_counts, dtype=np.int32),
+        hp_coeffs=np.ascontiguousarray(g.hp_coeffs, dtype=np.uint8),
+        hp_params=np.ascontiguousarray(g.hp_params, dtype=np.uint8),
+        pp_psi_c=np.ascontiguousarray(g.pp_psi_c, dtype=np.uint8),
+        pp_psi_p=np.ascontiguousarray(g.pp_psi_p, dtype=np.uint8),
+        pp_phi_c=np.ascontiguousarray(g.pp_phi_c, dtype=np.uint8),
+        pp_phi_p=np.ascontiguousarray(g.pp_phi_p, dtype=np.uint8),
+        ph_alpha=np.ascontiguousarray(g.ph_alpha, dtype=np.uint8),
+        ph_alpha_p=np.ascontiguousarray(g.ph_alpha_p, dtype=np.uint8),
+        ph_beta=np.ascontiguousarray(g.ph_beta, dtype=np.uint8),
+        ph_beta_p=np.ascontiguousarray(g.ph_beta_p, dtype=np.uint8),
+        ph_counts=np.ascontiguousarray(g.ph_counts, dtype=np.int32),
+        pf_phase_idx=np.ascontiguousarray(g.pf_phase_idx, dtype=np.uint8),
+        pf_floatfactor=np.ascontiguousarray(g.pf_floatfactor, dtype=np.int32),
+        pf_power2=np.ascontiguousarray(g.pf_power2, dtype=np.int32),
         pf_approx_f32=apx.view(np.float32),
         pf_has_approx=bool(g.pf_has_approx),
     )

@@ -1,0 +1,282 @@
+"""Serving: a request loop with continuous batching over the persistent kernel.
+
+The reference has no serving path (it samples one program per call,
+src/python/ksim/sample/sampler.py:57-67).  This loop keeps uploaded programs
+resident on the device (a cache keyed by the caller's program key, like a
+compile cache), admits requests while their HBM footprint fits the budget
+and a slot is free, and runs the device work as a sequence of persistent
+launches ("steps"), each over tiles of every active request:
+
+* a request arriving while others run is admitted as soon as memory and a
+  slot free up and its tiles join the next step, so its work overlaps the
+  others' on the device;
+* two steps are kept in flight so the GPU does not idle while the host
+  builds the next step;
+* a request completes when the step holding its last tile has finished and
+  its output has been copied back; latency = completion - arrival.
+
+Every shot of a request draws the per-shot stream keyed by (seed, component,
+shot index), so a request's result is the same alone and in any mix.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+import heapq
+import math
+import time
+from collections import OrderedDict, deque
+
+import numpy as np
+
+from . import _lib
+from .sampler import HipProgramSampler
+
+
+@dataclasses.dataclass
+class Request:
+    """One sampling request: ``prog`` (a FlatProgram, or a sampler already
+    built for it) under ``key``, its f matrix (shots, f_cols) uint8, a seed
+    and an arrival time in seconds from the start of the run."""
+    key: object
+    prog: object
+    f: np.ndarray
+    seed: int
+    arrival: float = 0.0
+    # filled by the loop
+    out: np.ndarray | None = None
+    admitted: float | None = None
+    done: float | None = None
+
+    @property
+    def shots(self) -> int:
+        return int(self.f.shape[0])
+
+    @property
+    def latency(self) -> float | None:
+        return None if self.done is None else self.done - self.arrival
+
+
+class _Active:
+    def __init__(self, req, sampler, slot, d_f, d_out, tiles):
+        self.req, self.sampler, self.slot = req, sampler, slot
+        self.d_f, self.d_out = d_f, d_out
+        self.tiles = tiles            # deque of (comp, shot0, count)
+        self.outstanding = 0          # tiles launched, not yet retired
+        self.bytes = d_f.numel() + d_out.numel()
+
+
+class ServingLoop:
+    """Continuous-batching request loop.
+
+    hbm_budget      bytes of device memory the loop may hold (programs, f
+                    matrices and outputs)
+    max_slots       requests admitted at once
+    step_ms         target duration of one persistent launch
+    tile_shots      shots per work item (a multiple of 64)
+    blocks          persistent blocks per launch (default: 6 per CU)
+    """
+
+    def __init__(self, hbm_budget: int = 32 << 30, max_slots: int = 64, step_ms: float = 4.0,
+                 tile_shots: int = 256, blocks: int | None = None, max_items: int = 1 << 16):
+        import torch
+        from .batch import device_info
+        self.torch = torch
+        self.lib = _lib.load()
+        self.hbm_budget = int(hbm_budget)
+        self.max_slots = int(max_slots)
+        self.step_s = step_ms / 1e3
+        if tile_shots <= 0 or tile_shots % 64:
+            raise ValueError("tile_shots must be a positive multiple of 64")
+        self.tile_shots = int(tile_shots)
+        self.max_items = int(max_items)
+        self.blocks = int(blocks or device_info()["cus"] * 6)
+        h = C.c_void_p()
+        _lib.check(self.lib.ksim_sched_create(self.max_slots, self.max_items, C.byref(h)))
+        self._h = h
+        self.stream = torch.cuda.Stream()  # default priority
+        self.programs: OrderedDict = OrderedDict()  # key -> [sampler, bytes, users]
+        self.program_bytes = 0
+        # algorithmic int64 ops per second, refined from finished steps
+        self.rate = 1.0e13
+
+    # ------------------------------------------------------------ programs
+    def _program(self, key, prog):
+        ent = self.programs.get(key)
+        if ent is not None:
+            self.programs.move_to_end(key)
+            return ent
+        sampler = prog if isinstance(prog, HipProgramSampler) else HipProgramSampler(prog)
+        nbytes = int(self.lib.ksim_program_device_bytes(sampler.handle))
+        ent = [sampler, nbytes, 0]
+        self.programs[key] = ent
+        self.program_bytes += nbytes
+        return ent
+
+    def _evict_for(self, need: int, used: int) -> int:
+        """Drop idle programs (least recently used first) until `need` fits."""
+        for key in list(self.programs.keys()):
+            if used + need <= self.hbm_budget:
+                break
+            ent = self.programs[key]
+            if ent[2] == 0:
+                used -= ent[1]
+                self.program_bytes -= ent[1]
+                del self.programs[key]
+        return used
+
+    def close(self):
+        h, self._h = getattr(self, "_h", None), None
+        if h:
+            self.lib.ksim_sched_destroy(h)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---------------------------------------------------------------- run
+    def run(self, requests: list[Request], realtime: bool = True) -> dict:
+        """Serve ``requests`` (arrival times honoured when ``realtime``; all
+        available at once otherwise).  Fills each request's ``out``,
+        ``admitted`` and ``done`` and returns summary statistics."""
+        torch = self.torch
+        lib = self.lib
+        st = self.stream
+        pending = sorted(requests, key=lambda r: r.arrival)
+        pend_i = 0
+        active: list[_Active] = []
+        free_slots = list(range(self.max_slots))[::-1]
+        inflight: deque = deque()      # (event, t_launch, est_ops, [(active, tiles)])
+        copies: deque = deque()        # (event, active, host_out)
+        used = self.program_bytes
+        t0 = time.perf_counter()
+        now = lambda: time.perf_counter() - t0  # noqa: E731
+        done = 0
+        busy_ops = 0.0
+        n_steps = 0
+        with torch.cuda.stream(st):
+            while done < len(requests):
+                t = now()
+                # ---- admit released requests while memory and slots allow
+                while pend_i < len(pending) and (not realtime or pending[pend_i].arrival <= t) \
+                        and free_slots:
+                    r = pending[pend_i]
+                    ent = self.programs.get(r.key)
+                    need = r.shots * (r.f.shape[1] + 64)
+                    if ent is None:
+                        need += 0  # program bytes are only known after packing
+                    used = self.program_bytes + sum(a.bytes for a in active)
+                    if used + need > self.hbm_budget:
+                        used = self._evict_for(need, used)
+                        if used + need > self.hbm_budget and active:
+                            break  # wait for memory to free up
+                    ent = self._program(r.key, r.prog)
+                    ent[2] += 1
+                    sampler = ent[0]
+                    d_f = torch.from_numpy(np.ascontiguousarray(r.f, dtype=np.uint8)).to(
+                        "cuda", non_blocking=False)
+                    d_out = torch.empty((r.shots, max(sampler.num_outputs, 1)),
+                                        dtype=torch.uint8, device="cuda")
+                    d_out.zero_()
+                    slot = free_slots.pop()
+                    _lib.check(lib.ksim_sched_set_slot(
+                        self._h, slot, sampler.handle, d_f.data_ptr() if d_f.numel() else None,
+                        int(r.f.shape[1]), r.shots, int(r.seed), d_out.data_ptr()))
+                    tiles = deque((c, s0, min(self.tile_shots, r.shots - s0))
+                                  for c in range(sampler.n_comp)
+                                  for s0 in range(0, r.shots, self.tile_shots))
+                    a = _Active(r, sampler, slot, d_f, d_out, tiles)
+                    r.admitted = now()
+                    if not tiles:  # zero shots
+                        r.out = np.zeros((0, sampler.num_outputs), np.uint8)
+                        r.done = now()
+                        done += 1
+                        ent[2] -= 1
+                        free_slots.append(slot)
+                        _lib.check(lib.ksim_sched_clear_slot(self._h, slot))
+                    else:
+                        active.append(a)
+                    pend_i += 1
+                # ---- launch a step if fewer than two are in flight
+                if len(inflight) < 2 and any(a.tiles for a in active):
+                    budget = self.rate * self.step_s
+                    items, taken, est = [], [], 0.0
+                    # round-robin over active requests, one tile each per pass
+                    while est < budget and len(items) < self.max_items:
+                        progressed = False
+                        for a in active:
+                            if not a.tiles:
+                                continue
+                            c, s0, cnt = a.tiles.popleft()
+                            items.append((a.slot, c, s0, cnt))
+                            taken.append(a)
+                            a.outstanding += 1
+                            est += a.sampler.ops_per_shot * cnt / max(a.sampler.n_comp, 1)
+                            progressed = True
+                            if est >= budget or len(items) >= self.max_items:
+                                break
+                        if not progressed:
+                            break
+                    arr = np.asarray(items, dtype=np.int32).reshape(-1, 4)
+                    _lib.check(lib.ksim_sched_launch(self._h, _lib.ptr(arr, C.c_int32), len(items),
+                                                     self.blocks, st.cuda_stream))
+                    ev = torch.cuda.Event()
+                    ev.record(st)
+                    inflight.append((ev, time.perf_counter(), est, taken))
+                    n_steps += 1
+                    continue
+                # ---- retire finished steps
+                progressed = False
+                while inflight and inflight[0][0].query():
+                    ev, tl, est, taken = inflight.popleft()
+                    busy_ops += est
+                    for a in taken:
+                        a.outstanding -= 1
+                        if not a.tiles and a.outstanding == 0:
+                            host = torch.empty(tuple(a.d_out.shape), dtype=torch.uint8,
+                                               pin_memory=True)
+                            host.copy_(a.d_out, non_blocking=True)
+                            cev = torch.cuda.Event()
+                            cev.record(st)
+                            copies.append((cev, a, host))
+                            active.remove(a)
+                    progressed = True
+                while copies and copies[0][0].query():
+                    cev, a, host = copies.popleft()
+                    r = a.req
+                    r.out = host.numpy()[:, :a.sampler.num_outputs].copy()
+                    r.done = now()
+                    done += 1
+                    self.programs[r.key][2] -= 1
+                    _lib.check(lib.ksim_sched_clear_slot(self._h, a.slot))
+                    free_slots.append(a.slot)
+                    progressed = True
+                if progressed:
+                    continue
+                # ---- nothing to do right now: wait for the GPU or the next arrival
+                if inflight:
+                    inflight[0][0].synchronize()
+                elif copies:
+                    copies[0][0].synchronize()
+                elif pend_i < len(pending):
+                    dt = pending[pend_i].arrival - now()
+                    if dt > 0:
+                        time.sleep(min(dt, 0.01))
+        wall = now()
+        _lib.check(lib.ksim_sched_check(self._h, st.cuda_stream))
+        lat = np.array([r.latency for r in requests], dtype=np.float64)
+        shots = sum(r.shots for r in requests)
+        return dict(requests=len(requests), shots=shots, wall_s=wall,
+                    shots_per_s=shots / wall if wall > 0 else float("nan"),
+                    p50_ms=float(np.percentile(lat, 50) * 1e3) if len(lat) else None,
+                    p99_ms=float(np.percentile(lat, 99) * 1e3) if len(lat) else None,
+                    mean_ms=float(lat.mean() * 1e3) if len(lat) else None,
+                    steps=n_steps, programs_resident=len(self.programs))
+
+
+def poisson_arrivals(n: int, rate: float, seed: int) -> np.ndarray:
+    """Arrival times (s) of n requests of a Poisson process with ``rate``/s."""
+    rng = np.random.default_rng(seed)
+    return np.cumsum(rng.exponential(1.0 / rate, size=n))

@@ -198,6 +198,12 @@ def terms_for_t(t: int, rng: np.random.Generator, jitter: bool = True) -> int:
     return max(1, int(round(g)))
 
 
+def _bernoulli_u8(rng: np.random.Generator, shape, p: float) -> np.ndarray:
+    """{0,1} uint8 array with P(1) = p, drawn one byte per element."""
+    thr = int(round(p * 256))
+    return (rng.integers(0, 256, size=shape, dtype=np.uint8) < thr).astype(np.uint8)
+
+
 _PURE_FF = np.array([[1, 0, 0, 0], [-1, 0, 0, 0], [0, 1, 0, 1], [2, 0, 0, 0],
                      [1, 1, 0, 0], [0, 0, 1, 0], [1, 0, 1, 0]], np.int32)
 
@@ -221,12 +227,9 @@ def shape_graph(G: int, P: int, n_q: int, t: int, rng: np.random.Generator,
     Tph = t // 2 + 1
     g = empty_graph(G, P, Tnp, Thp, Tpp, Tph)
     if P > 0:
-        g.np_params[:] = rng.random((G, Tnp, P)) < density
-        g.hp_params[:] = rng.random((G, Thp, P)) < density
-        g.pp_psi_p[:] = rng.random((G, Tpp, P)) < density
-        g.pp_phi_p[:] = rng.random((G, Tpp, P)) < density
-        g.ph_alpha_p[:] = rng.random((G, Tph, P)) < density
-        g.ph_beta_p[:] = rng.random((G, Tph, P)) < density
+        for name, T in (("np_params", Tnp), ("hp_params", Thp), ("pp_psi_p", Tpp),
+                        ("pp_phi_p", Tpp), ("ph_alpha_p", Tph), ("ph_beta_p", Tph)):
+            getattr(g, name)[:] = _bernoulli_u8(rng, (G, T, P), density)
     live_ph = np.array([1, 2, 3, 5, 6, 7], np.uint8)
     ph = rng.choice(live_ph, size=(G, Tnp))
     proj = rng.random((G, Tnp)) < proj_frac

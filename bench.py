@@ -266,16 +266,17 @@ def serving_result(n_requests=48, n_pool=8, load=0.7, t_max=40, q_max=85, seed=5
     # saturation: the whole stream at once (no arrival gaps)
     loop = ServingLoop(hbm_budget=96 << 30, max_slots=64, step_ms=8.0)
     warm = [Request(key=r.key, prog=r.prog, f=r.f[:64], seed=r.seed) for r in reqs[:4]]
-    loop.run(warm, realtime=False)
+    loop.run(warm, realtime=False, progress=True, max_wall_s=60)
+    log(f"[bench] config5: saturation run over {n_requests} requests, {total_ops:.3g} ops")
     sat = loop.run([Request(key=r.key, prog=r.prog, f=r.f, seed=r.seed) for r in reqs],
-                   realtime=False)
+                   realtime=False, progress=True, max_wall_s=120)
     rate_ops = total_ops / sat["wall_s"]
     lam = load * rate_ops / (total_ops / n_requests)
     for r, t in zip(reqs, poisson_arrivals(n_requests, lam, seed)):
         r.arrival = float(t)
     log(f"[bench] config5: saturation {sat['wall_s']:.2f}s for {n_requests} requests; "
         f"Poisson rate {lam:.2f}/s")
-    st = loop.run(reqs, realtime=True)
+    st = loop.run(reqs, realtime=True, progress=True, max_wall_s=180)
     loop.close()
     return dict(requests=n_requests, pool=n_pool, load=load, arrival_rate_per_s=lam,
                 pool_t=[sp["t"] for sp in specs], pool_q=[sp["n_q"] for sp in specs],

@@ -204,9 +204,10 @@ int ksim_sched_set_slot(ksim_sched *q, int32_t slot, ksim_program *p,
 int ksim_sched_clear_slot(ksim_sched *q, int32_t slot);
 
 /* Enqueue one persistent launch on `stream` over n items, each 4 int32:
- * (slot, component, first shot, shot count).  Returns after staging; the
- * pinned staging buffer of launch k is reused by launch k+2, which waits
- * for launch k's copies first. */
+ * (slot, component, first shot, shot count).  Returns after staging.
+ * Launches rotate over four staging sets (own buffers and work cursor), so
+ * up to four launches may be in flight on different streams; launch k+4
+ * waits for launch k to finish before reusing its set. */
 int ksim_sched_launch(ksim_sched *q, const int32_t *items, int32_t n,
                       int32_t blocks, void *stream);
 

@@ -745,20 +745,23 @@ void ksim_batch_destroy(ksim_batch *b) {
 // A persistent-kernel launcher over an arbitrary list of (program slot,
 // component, shot range) items per launch: the device side of the serving
 // loop.  Slots bind a resident program to its resident f matrix, output and
-// seed.  Each launch stages the slot table and the item list through pinned
-// host buffers (two, alternating) into device buffers (two, alternating), so
-// the host can prepare launch k+1 while launch k runs.
+// seed.  Launches rotate over KSIM_SCHED_BUFS staging sets (pinned host
+// buffers, device buffers and a work cursor each), so up to that many
+// launches can be in flight on different streams and overlap on the device:
+// the blocks of launch k+1 start as those of launch k drain.
+constexpr int KSIM_SCHED_BUFS = 4;
+
 struct ksim_sched {
   int max_slots = 0, max_items = 0;
   std::vector<BatchProg> slots;
   std::vector<int> slot_wl, slot_ncomp;
-  BatchProg *h_progs[2] = {nullptr, nullptr};
-  BatchItem *h_items[2] = {nullptr, nullptr};
-  BatchProg *d_progs[2] = {nullptr, nullptr};
-  BatchItem *d_items[2] = {nullptr, nullptr};
-  hipEvent_t staged[2] = {nullptr, nullptr};
-  bool staged_valid[2] = {false, false};
-  uint32_t *d_cursor = nullptr;  // [0] cursor, [1] error word
+  BatchProg *h_progs[KSIM_SCHED_BUFS] = {};
+  BatchItem *h_items[KSIM_SCHED_BUFS] = {};
+  BatchProg *d_progs[KSIM_SCHED_BUFS] = {};
+  BatchItem *d_items[KSIM_SCHED_BUFS] = {};
+  hipEvent_t done[KSIM_SCHED_BUFS] = {};  // recorded after the launch's kernel
+  bool done_valid[KSIM_SCHED_BUFS] = {};
+  uint32_t *d_cursor = nullptr;  // [2k] cursor of set k, [2k+1] scratch; [64] error word
   int next = 0;
 };
 
@@ -775,15 +778,15 @@ int ksim_sched_create(int32_t max_slots, int32_t max_items, ksim_sched **out) {
     q->slot_wl.assign(max_slots, 0);
     q->slot_ncomp.assign(max_slots, 0);
     try {
-      for (int k = 0; k < 2; ++k) {
+      for (int k = 0; k < KSIM_SCHED_BUFS; ++k) {
         hip_check(hipHostMalloc((void **)&q->h_progs[k], sizeof(BatchProg) * max_slots), "hipHostMalloc");
         hip_check(hipHostMalloc((void **)&q->h_items[k], sizeof(BatchItem) * max_items), "hipHostMalloc");
         hip_check(hipMalloc(&q->d_progs[k], sizeof(BatchProg) * max_slots), "hipMalloc");
         hip_check(hipMalloc(&q->d_items[k], sizeof(BatchItem) * max_items), "hipMalloc");
-        hip_check(hipEventCreateWithFlags(&q->staged[k], hipEventDisableTiming), "hipEventCreate");
+        hip_check(hipEventCreateWithFlags(&q->done[k], hipEventDisableTiming), "hipEventCreate");
       }
-      hip_check(hipMalloc(&q->d_cursor, 64), "hipMalloc");
-      hip_check(hipMemset(q->d_cursor, 0, 64), "hipMemset");
+      hip_check(hipMalloc(&q->d_cursor, 65 * sizeof(uint32_t)), "hipMalloc");
+      hip_check(hipMemset(q->d_cursor, 0, 65 * sizeof(uint32_t)), "hipMemset");
       tables_for_current_device();
     } catch (...) {
       ksim_sched_destroy(q);
@@ -829,8 +832,9 @@ int ksim_sched_launch(ksim_sched *q, const int32_t *items, int32_t n, int32_t bl
     if (!items) throw std::invalid_argument("items is NULL");
     hipStream_t st = (hipStream_t)stream;
     const int k = q->next;
-    q->next ^= 1;
-    if (q->staged_valid[k]) hip_check(hipEventSynchronize(q->staged[k]), "hipEventSynchronize");
+    q->next = (q->next + 1) % KSIM_SCHED_BUFS;
+    // the set's previous launch (possibly on another stream) must be over
+    if (q->done_valid[k]) hip_check(hipEventSynchronize(q->done[k]), "hipEventSynchronize");
     int wl = 1;
     for (int i = 0; i < n; ++i) {
       const int32_t slot = items[4 * i], comp = items[4 * i + 1];
@@ -850,14 +854,13 @@ int ksim_sched_launch(ksim_sched *q, const int32_t *items, int32_t n, int32_t bl
                              hipMemcpyHostToDevice, st), "H2D slots");
     hip_check(hipMemcpyAsync(q->d_items[k], q->h_items[k], sizeof(BatchItem) * n,
                              hipMemcpyHostToDevice, st), "H2D items");
-    hip_check(hipEventRecord(q->staged[k], st), "hipEventRecord");
-    q->staged_valid[k] = true;
     DevTables t = tables_for_current_device();
-    hipLaunchKernelGGL(k_reset_cursor, dim3(1), dim3(64), 0, st, q->d_cursor, q->d_cursor + 2);
+    uint32_t *cursor = q->d_cursor + 2 * k;
+    hipLaunchKernelGGL(k_reset_cursor, dim3(1), dim3(64), 0, st, cursor, cursor + 1);
     hip_check(hipGetLastError(), "k_reset_cursor");
 #define KSIM_SCHED_LAUNCH(W)                                                                   \
   hipLaunchKernelGGL(k_batch<W>, dim3(blocks), dim3(kBlock), 0, st, q->d_progs[k],              \
-                     q->d_items[k], (uint32_t)n, q->d_cursor, q->d_cursor + 1, t.tab, t.pitab)
+                     q->d_items[k], (uint32_t)n, cursor, q->d_cursor + 64, t.tab, t.pitab)
     switch (lane_words(wl)) {
       case 1: KSIM_SCHED_LAUNCH(1); break;
       case 2: KSIM_SCHED_LAUNCH(2); break;
@@ -866,6 +869,8 @@ int ksim_sched_launch(ksim_sched *q, const int32_t *items, int32_t n, int32_t bl
     }
 #undef KSIM_SCHED_LAUNCH
     hip_check(hipGetLastError(), "k_batch");
+    hip_check(hipEventRecord(q->done[k], st), "hipEventRecord");
+    q->done_valid[k] = true;
   });
 }
 
@@ -874,7 +879,9 @@ int ksim_sched_check(ksim_sched *q, void *stream) {
     if (!q) throw std::invalid_argument("NULL argument");
     hipStream_t st = (hipStream_t)stream;
     uint32_t e = 0;
-    hip_check(hipMemcpyAsync(&e, q->d_cursor + 1, 4, hipMemcpyDeviceToHost, st), "read err");
+    for (int k = 0; k < KSIM_SCHED_BUFS; ++k)
+      if (q->done_valid[k]) hip_check(hipEventSynchronize(q->done[k]), "hipEventSynchronize");
+    hip_check(hipMemcpyAsync(&e, q->d_cursor + 64, 4, hipMemcpyDeviceToHost, st), "read err");
     hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
     if (e) throw RangeError(err_text(e));
   });
@@ -882,8 +889,11 @@ int ksim_sched_check(ksim_sched *q, void *stream) {
 
 void ksim_sched_destroy(ksim_sched *q) {
   if (!q) return;
-  for (int k = 0; k < 2; ++k) {
-    if (q->staged[k]) { (void)hipEventSynchronize(q->staged[k]); (void)hipEventDestroy(q->staged[k]); }
+  for (int k = 0; k < KSIM_SCHED_BUFS; ++k) {
+    if (q->done[k]) {
+      if (q->done_valid[k]) (void)hipEventSynchronize(q->done[k]);
+      (void)hipEventDestroy(q->done[k]);
+    }
     if (q->h_progs[k]) (void)hipHostFree(q->h_progs[k]);
     if (q->h_items[k]) (void)hipHostFree(q->h_items[k]);
     if (q->d_progs[k]) (void)hipFree(q->d_progs[k]);

@@ -10,8 +10,9 @@ launches ("steps"), each over tiles of every active request:
 * a request arriving while others run is admitted as soon as memory and a
   slot free up and its tiles join the next step, so its work overlaps the
   others' on the device;
-* two steps are kept in flight so the GPU does not idle while the host
-  builds the next step;
+* up to three steps are in flight on three streams, so the blocks of the
+  next step start as those of the previous one drain (a tile of a heavy
+  program can run for a large part of a second on one wave);
 * a request completes when the step holding its last tile has finished and
   its output has been copied back; latency = completion - arrival.
 
@@ -77,8 +78,9 @@ class ServingLoop:
     blocks          persistent blocks per launch (default: 6 per CU)
     """
 
-    def __init__(self, hbm_budget: int = 32 << 30, max_slots: int = 64, step_ms: float = 4.0,
-                 tile_shots: int = 256, blocks: int | None = None, max_items: int = 1 << 16):
+    def __init__(self, hbm_budget: int = 32 << 30, max_slots: int = 64, step_ms: float = 20.0,
+                 tile_shots: int = 64, blocks: int | None = None, max_items: int = 1 << 16,
+                 rate: float = 2.5e13, streams: int = 3):
         import torch
         from .batch import device_info
         self.torch = torch
@@ -94,11 +96,18 @@ class ServingLoop:
         h = C.c_void_p()
         _lib.check(self.lib.ksim_sched_create(self.max_slots, self.max_items, C.byref(h)))
         self._h = h
-        self.stream = torch.cuda.Stream()  # default priority
+        if not 1 <= streams <= 4:
+            raise ValueError("streams must be in 1..4 (the scheduler has four staging sets)")
+        # default-priority streams: steps rotate over them; outputs return on
+        # a copy stream
+        self.streams = [torch.cuda.Stream() for _ in range(streams)]
+        self.copy_stream = torch.cuda.Stream()
+        self.stream = self.streams[0]
         self.programs: OrderedDict = OrderedDict()  # key -> [sampler, bytes, users]
         self.program_bytes = 0
-        # algorithmic int64 ops per second, refined from finished steps
-        self.rate = 1.0e13
+        # algorithmic int64 ops per second of the whole device (sets the work
+        # put into one step)
+        self.rate = float(rate)
 
     # ------------------------------------------------------------ programs
     def _program(self, key, prog):
@@ -137,10 +146,14 @@ class ServingLoop:
             pass
 
     # ---------------------------------------------------------------- run
-    def run(self, requests: list[Request], realtime: bool = True) -> dict:
+    def run(self, requests: list[Request], realtime: bool = True, progress: bool = False,
+            max_wall_s: float | None = None) -> dict:
         """Serve ``requests`` (arrival times honoured when ``realtime``; all
         available at once otherwise).  Fills each request's ``out``,
-        ``admitted`` and ``done`` and returns summary statistics."""
+        ``admitted`` and ``done`` and returns summary statistics.  With
+        ``progress`` a status line goes to stderr every two seconds;
+        ``max_wall_s`` bounds the run (RuntimeError with the loop state)."""
+        import sys
         torch = self.torch
         lib = self.lib
         st = self.stream
@@ -156,9 +169,21 @@ class ServingLoop:
         done = 0
         busy_ops = 0.0
         n_steps = 0
+        last_report = 0.0
+        admitted_now = False
         with torch.cuda.stream(st):
             while done < len(requests):
                 t = now()
+                if progress and t - last_report > 2.0:
+                    last_report = t
+                    print(f"[serve] t={t:.1f}s done={done}/{len(requests)} admitted={pend_i} "
+                          f"active={len(active)} inflight={len(inflight)} copies={len(copies)} "
+                          f"steps={n_steps}", file=sys.stderr, flush=True)
+                if max_wall_s is not None and t > max_wall_s:
+                    raise RuntimeError(
+                        f"serving loop exceeded {max_wall_s}s: done={done}/{len(requests)} "
+                        f"active={len(active)} inflight={len(inflight)} steps={n_steps} "
+                        f"tiles_left={sum(len(a.tiles) for a in active)}")
                 # ---- admit released requests while memory and slots allow
                 while pend_i < len(pending) and (not realtime or pending[pend_i].arrival <= t) \
                         and free_slots:
@@ -181,6 +206,7 @@ class ServingLoop:
                                         dtype=torch.uint8, device="cuda")
                     d_out.zero_()
                     slot = free_slots.pop()
+                    admitted_now = True
                     _lib.check(lib.ksim_sched_set_slot(
                         self._h, slot, sampler.handle, d_f.data_ptr() if d_f.numel() else None,
                         int(r.f.shape[1]), r.shots, int(r.seed), d_out.data_ptr()))
@@ -199,8 +225,16 @@ class ServingLoop:
                     else:
                         active.append(a)
                     pend_i += 1
-                # ---- launch a step if fewer than two are in flight
-                if len(inflight) < 2 and any(a.tiles for a in active):
+                # ---- order the admissions (f upload, output zeroing on the first
+                # stream) before any step that may read them on another stream
+                if admitted_now:
+                    adm = torch.cuda.Event()
+                    adm.record(st)
+                    for other in self.streams[1:]:
+                        other.wait_event(adm)
+                    admitted_now = False
+                # ---- launch a step if a stream is free
+                if len(inflight) < len(self.streams) and any(a.tiles for a in active):
                     budget = self.rate * self.step_s
                     items, taken, est = [], [], 0.0
                     # round-robin over active requests, one tile each per pass
@@ -220,26 +254,29 @@ class ServingLoop:
                         if not progressed:
                             break
                     arr = np.asarray(items, dtype=np.int32).reshape(-1, 4)
+                    sst = self.streams[n_steps % len(self.streams)]
                     _lib.check(lib.ksim_sched_launch(self._h, _lib.ptr(arr, C.c_int32), len(items),
-                                                     self.blocks, st.cuda_stream))
+                                                     self.blocks, sst.cuda_stream))
                     ev = torch.cuda.Event()
-                    ev.record(st)
+                    ev.record(sst)
                     inflight.append((ev, time.perf_counter(), est, taken))
                     n_steps += 1
                     continue
-                # ---- retire finished steps
+                # ---- retire finished steps (streams may finish out of order)
                 progressed = False
-                while inflight and inflight[0][0].query():
-                    ev, tl, est, taken = inflight.popleft()
+                for step in [x for x in inflight if x[0].query()]:
+                    inflight.remove(step)
+                    ev, tl, est, taken = step
                     busy_ops += est
                     for a in taken:
                         a.outstanding -= 1
                         if not a.tiles and a.outstanding == 0:
                             host = torch.empty(tuple(a.d_out.shape), dtype=torch.uint8,
                                                pin_memory=True)
-                            host.copy_(a.d_out, non_blocking=True)
-                            cev = torch.cuda.Event()
-                            cev.record(st)
+                            with torch.cuda.stream(self.copy_stream):
+                                host.copy_(a.d_out, non_blocking=True)
+                                cev = torch.cuda.Event()
+                                cev.record(self.copy_stream)
                             copies.append((cev, a, host))
                             active.remove(a)
                     progressed = True
@@ -257,7 +294,7 @@ class ServingLoop:
                     continue
                 # ---- nothing to do right now: wait for the GPU or the next arrival
                 if inflight:
-                    inflight[0][0].synchronize()
+                    time.sleep(0.0002)
                 elif copies:
                     copies[0][0].synchronize()
                 elif pend_i < len(pending):
@@ -266,6 +303,7 @@ class ServingLoop:
                         time.sleep(min(dt, 0.01))
         wall = now()
         _lib.check(lib.ksim_sched_check(self._h, st.cuda_stream))
+        torch.cuda.synchronize()
         lat = np.array([r.latency for r in requests], dtype=np.float64)
         shots = sum(r.shots for r in requests)
         return dict(requests=len(requests), shots=shots, wall_s=wall,

@@ -159,8 +159,11 @@ int ksim_program_check(ksim_program *p, void *stream);
 int64_t ksim_program_device_bytes(const ksim_program *p);
 
 /* Stats of the packed program: [n_graphs, n_terms, n_records, words,
- * ops_per_shot (algorithmic int64 ops per shot, all components)]. */
-int ksim_program_stats(const ksim_program *p, double *stats5);
+ * ops_per_shot (algorithmic int64 ops per shot, all components),
+ * largest |n'| (the (1+sqrt2) exponent) a term can reach, terms whose
+ * |n'| may exceed the single-digit table rows, bits of the exact sum the
+ * widest graph may need (at most 128)]. */
+int ksim_program_stats(const ksim_program *p, double *stats8);
 
 void ksim_program_destroy(ksim_program *p);
 
@@ -218,18 +221,23 @@ void ksim_sched_destroy(ksim_sched *q);
 /* ---- host-only inspection (no device needed) ---------------------------- */
 
 /* Pack a program exactly as ksim_program_create does, without uploading:
- * writes up to `cap` words to `words` and the full word count to *n_words.
+ * writes up to `cap` words to `words`, the full word count to *n_words and,
+ * when stats8 is not NULL, the ksim_program_stats figures.
  * No reference counterpart: lets the packer be checked on a CPU-only host. */
 int ksim_program_pack(const ksim_graph_view *graphs, const int32_t *graphs_per_comp,
                       const int32_t *f_selection, const int32_t *F_per_comp,
                       int32_t n_comp, int32_t num_f, const int32_t *dest_col,
                       int32_t num_outputs, uint32_t *words, int64_t cap,
-                      int64_t *n_words);
+                      int64_t *n_words, double *stats8);
 
-/* Copy the kernels' lookup tables: tab (4*TAB_N int32) and pitab
- * (4*PI_MAX*PI_MAX int32); sizes[0..1] receive TAB_R and PI_MAX. */
+/* Copy the kernels' lookup tables.  tab: D planes of (2W+1)*4*8 entries of
+ * 4 int32, entry ((n+W)*4+r)*8+s of plane d holding digit d (base 2^G,
+ * balanced) of the coefficients of w^s (1+w)^r (1+sqrt2)^n; pitab:
+ * PI_MAX*PI_MAX entries of 4 int32 (pi^a pibar^b).  sizes4 receives
+ * W, PI_MAX, D and G.  No reference counterpart: lets the packed format be
+ * interpreted on a CPU-only host. */
 int ksim_tables(int32_t *tab, int64_t tab_cap, int32_t *pitab, int64_t pitab_cap,
-                int32_t *sizes2);
+                int32_t *sizes4);
 
 /* ---- misc --------------------------------------------------------------- */
 

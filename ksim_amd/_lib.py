@@ -71,7 +71,8 @@ EXPORTS = {
     "ksim_sched_check": (C.c_int, [vp, vp]),
     "ksim_sched_destroy": (None, [vp]),
     "ksim_program_pack": (C.c_int, [C.POINTER(GraphView), i32p, i32p, i32p, C.c_int32, C.c_int32,
-                                    i32p, C.c_int32, C.POINTER(C.c_uint32), C.c_int64, i64p]),
+                                    i32p, C.c_int32, C.POINTER(C.c_uint32), C.c_int64, i64p,
+                                    C.POINTER(C.c_double)]),
     "ksim_tables": (C.c_int, [i32p, C.c_int64, i32p, C.c_int64, i32p]),
     "ksim_last_error": (C.c_char_p, []),
     "ksim_device_info": (C.c_int, [i64p]),
@@ -81,10 +82,20 @@ _lib = None
 
 
 def load() -> C.CDLL:
-    """Load the library (once); raise ImportError if it was not built."""
+    """Load the library (once); raise ImportError if it was not built.
+
+    PyTorch ships its own HIP and HSA runtimes.  When torch is importable it
+    is imported first, so the library binds to the runtime torch already
+    loaded (same soname) and the process holds one HIP runtime; loaded the
+    other way round, torch would bring a second runtime that cannot open
+    the device."""
     global _lib
     if _lib is not None:
         return _lib
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(LIB_PATH):
         raise ImportError(
             f"{LIB_PATH} is missing: build it with `python -m ksim_amd.build` "

@@ -40,6 +40,7 @@ class BatchedSampler:
     def __init__(self, samplers: list[HipProgramSampler], f: list, seeds: list[int],
                  wave_shots: int = 64):
         import torch
+        torch.cuda.init()
         self.torch = torch
         self.samplers = samplers
         self.f = f

@@ -73,8 +73,9 @@ DevTables tables_for_current_device() {
   auto it = g_tabs.find(dev);
   if (it != g_tabs.end()) return it->second;
   DevTables t;
-  hip_check(hipMalloc(&t.tab, sizeof(int4) * TAB_N), "hipMalloc(tab)");
-  hip_check(hipMemcpy(t.tab, materialise_table(), sizeof(int4) * TAB_N, hipMemcpyHostToDevice),
+  hip_check(hipMalloc(&t.tab, sizeof(int4) * TAB_DIGITS * TAB_N), "hipMalloc(tab)");
+  hip_check(hipMemcpy(t.tab, materialise_table(), sizeof(int4) * TAB_DIGITS * TAB_N,
+                      hipMemcpyHostToDevice),
             "upload tab");
   hip_check(hipMalloc(&t.pitab, sizeof(int4) * PI_MAX * PI_MAX), "hipMalloc(pitab)");
   hip_check(hipMemcpy(t.pitab, pi_table(), sizeof(int4) * PI_MAX * PI_MAX,
@@ -231,7 +232,6 @@ struct ksim_program {
   std::vector<int> n_out, F, n_graphs;
   std::vector<std::vector<uint32_t>> graph_hdr;  // host copy of graph headers per comp
   PackStats st;
-  double ops_per_shot = 0;
   int block_threads = kBlock;
   int term_split = 1;
   // staging for the host-pointer API
@@ -540,14 +540,17 @@ int64_t ksim_program_device_bytes(const ksim_program *p) {
   return p ? (int64_t)(p->n_words * 4) : -1;
 }
 
-int ksim_program_stats(const ksim_program *p, double *stats5) {
+int ksim_program_stats(const ksim_program *p, double *stats8) {
   return guarded([&] {
-    if (!p || !stats5) throw std::invalid_argument("NULL argument");
-    stats5[0] = (double)p->st.graphs;
-    stats5[1] = (double)p->st.terms;
-    stats5[2] = (double)p->st.records;
-    stats5[3] = (double)p->n_words;
-    stats5[4] = p->ops_per_shot;
+    if (!p || !stats8) throw std::invalid_argument("NULL argument");
+    stats8[0] = (double)p->st.graphs;
+    stats8[1] = (double)p->st.terms;
+    stats8[2] = (double)p->st.records;
+    stats8[3] = (double)p->n_words;
+    stats8[4] = p->st.ops;
+    stats8[5] = (double)p->st.n2_bound;
+    stats8[6] = (double)p->st.wide_terms;
+    stats8[7] = (double)p->st.sum_bits;
   });
 }
 
@@ -566,25 +569,35 @@ void ksim_program_destroy(ksim_program *p) {
 int ksim_program_pack(const ksim_graph_view *graphs, const int32_t *graphs_per_comp,
                       const int32_t *f_selection, const int32_t *F_per_comp, int32_t n_comp,
                       int32_t num_f, const int32_t *dest_col, int32_t num_outputs,
-                      uint32_t *words, int64_t cap, int64_t *n_words) {
+                      uint32_t *words, int64_t cap, int64_t *n_words, double *stats8) {
   return guarded([&] {
     if (!n_words) throw std::invalid_argument("n_words is NULL");
     PackStats st;
     std::vector<uint32_t> w = pack_program(graphs, graphs_per_comp, f_selection, F_per_comp,
                                            n_comp, num_f, dest_col, num_outputs, st);
     *n_words = (int64_t)w.size();
+    if (stats8) {
+      const double v[8] = {(double)st.graphs, (double)st.terms, (double)st.records,
+                           (double)w.size(), st.ops, (double)st.n2_bound,
+                           (double)st.wide_terms, (double)st.sum_bits};
+      std::memcpy(stats8, v, sizeof(v));
+    }
     if (words && cap > 0)
       std::memcpy(words, w.data(), sizeof(uint32_t) * std::min<size_t>(w.size(), (size_t)cap));
   });
 }
 
 int ksim_tables(int32_t *tab, int64_t tab_cap, int32_t *pitab, int64_t pitab_cap,
-                int32_t *sizes2) {
+                int32_t *sizes4) {
   return guarded([&] {
-    if (tab) std::memcpy(tab, materialise_table(), sizeof(int32_t) * std::min<int64_t>(tab_cap, 4 * TAB_N));
+    if (tab)
+      std::memcpy(tab, materialise_table(),
+                  sizeof(int32_t) * std::min<int64_t>(tab_cap, 4 * TAB_DIGITS * TAB_N));
     if (pitab)
       std::memcpy(pitab, pi_table(), sizeof(int32_t) * std::min<int64_t>(pitab_cap, 4 * PI_MAX * PI_MAX));
-    if (sizes2) { sizes2[0] = TAB_R; sizes2[1] = PI_MAX; }
+    if (sizes4) {
+      sizes4[0] = TAB_W; sizes4[1] = PI_MAX; sizes4[2] = TAB_DIGITS; sizes4[3] = TAB_DIGIT_BITS;
+    }
   });
 }
 

@@ -143,6 +143,63 @@ struct LdsTables {
   int4 pi[PI_MAX * PI_MAX];     // pi^a pibar^b
 };
 
+// acc + e * pi^{A,B} * R * 2^sh for one table digit e (|e| < 2^30).
+// zero: the lane adds nothing (a vanished term, or no digit at this place).
+// Uniform: try the wave-uniform shift path first.
+template <bool Uniform>
+__device__ __forceinline__ Acc4 add_term_digit(Acc4 acc, int4 e, bool zero, uint32_t flags, int A,
+                                               int Bq, const LdsTables *lt,
+                                               const cword *__restrict__ th, int sh,
+                                               uint32_t &err) {
+  const int32_t keep = (int32_t)zero - 1;
+  e.x &= keep; e.y &= keep; e.z &= keep; e.w &= keep;
+  int64_t x0, x1, x2, x3;
+  if ((flags & TF_PI) && (A | Bq)) {
+    if (A >= PI_MAX || Bq >= PI_MAX) {
+      err |= ERR_PI_RANGE;
+      e = make_int4(0, 0, 0, 0);
+    }
+    // table digit (< 2^30) x pi power (< 2^24): signed 32 x 32 -> 64
+    // products, four per coefficient, exact in int64
+    const int4 p = lt->pi[min(A, PI_MAX - 1) * PI_MAX + min(Bq, PI_MAX - 1)];
+    x0 = (int64_t)e.x * p.x + (int64_t)e.y * p.w - (int64_t)e.z * p.z + (int64_t)e.w * p.y;
+    x1 = (int64_t)e.x * p.y + (int64_t)e.y * p.x + (int64_t)e.z * p.w + (int64_t)e.w * p.z;
+    x2 = (int64_t)e.x * p.z + (int64_t)e.y * p.y + (int64_t)e.z * p.x - (int64_t)e.w * p.w;
+    x3 = (int64_t)e.x * p.w - (int64_t)e.y * p.z - (int64_t)e.z * p.y + (int64_t)e.w * p.x;
+  } else {
+    x0 = e.x; x1 = e.y; x2 = e.z; x3 = e.w;
+  }
+  if (sh < 0 || sh > SHIFT_MAX) {
+    if (x0 | x1 | x2 | x3) err |= ERR_SHIFT;
+    x0 = x1 = x2 = x3 = 0;
+    sh = 0;
+  }
+  if (flags & TF_RESIDUAL) {
+    // a floatfactor that does not factor: one Z[w] multiply in 128 bits
+    acc = add_residual_term(acc, x0, x1, x2, x3, (int32_t)th[TH_R0], (int32_t)th[TH_R1],
+                            (int32_t)th[TH_R2], (int32_t)th[TH_R3], sh);
+  } else {
+    // add X * 2^sh into the four 128-bit sums.  The shift is almost always
+    // the same on every lane (it varies only through phase-pair records),
+    // so the common path adds 32-bit limbs at a wave-uniform offset.
+    const int sh0 = __builtin_amdgcn_readfirstlane(sh);
+    if (zero) sh = sh0;  // a zero addend adds nothing at any shift
+    if (Uniform && __builtin_amdgcn_ballot_w64(sh != sh0) == 0) {
+      const int rr = sh0 & 31, L = sh0 >> 5;
+      add_shifted_uniform(acc.a, x0, rr, L);
+      add_shifted_uniform(acc.b, x1, rr, L);
+      add_shifted_uniform(acc.c, x2, rr, L);
+      add_shifted_uniform(acc.d, x3, rr, L);
+    } else {
+      acc.a += (i128_t)((u128_t)(i128_t)x0 << sh);
+      acc.b += (i128_t)((u128_t)(i128_t)x1 << sh);
+      acc.c += (i128_t)((u128_t)(i128_t)x2 << sh);
+      acc.d += (i128_t)((u128_t)(i128_t)x3 << sh);
+    }
+  }
+  return acc;
+}
+
 // -------------------------------------------------------------- one graph
 // Exact amplitude of one graph for this lane's parameter words pv[0..W32):
 // acc = sum_terms value * 2^{-e0}; the caller scales by 2^{GH_E0}.
@@ -218,61 +275,30 @@ __device__ __forceinline__ void eval_graph(const cword *__restrict__ P,
     const int q = V >> 2, r = V & 3;
     const int n2 = N + 2 * q;
     const int s2 = (S + 2 * q) & 7;
+    // table row n': the LDS copy for the central rows, else the global
+    // table, whose rows beyond TAB_R spread over TAB_DIGITS planes of
+    // base-2^31 digits (each < 2^30); digit d adds at shift sh + 31 d.
+    const bool vanished = dead & 1u;
+    const int gi = ((n2 + TAB_W) * 4 + r) * 8 + s2;
+    const bool in_tab = (unsigned)(n2 + TAB_W) <= 2u * TAB_W;
     int4 e;
     if ((unsigned)(n2 + TAB_LDS_R) <= 2u * TAB_LDS_R) {
       e = lt->tab[((n2 + TAB_LDS_R) * 4 + r) * 8 + s2];
-    } else if ((unsigned)(n2 + TAB_R) <= 2u * TAB_R) {
-      e = gtab[((n2 + TAB_R) * 4 + r) * 8 + s2];
+    } else if (in_tab) {
+      e = gtab[gi];
     } else {
-      err |= ERR_TAB_RANGE;
+      if (!vanished) err |= ERR_TAB_RANGE;
       e = make_int4(0, 0, 0, 0);
     }
-    const int32_t keep = (int32_t)(dead & 1u) - 1;  // 0 for a vanished term
-    e.x &= keep; e.y &= keep; e.z &= keep; e.w &= keep;
-    int64_t x0, x1, x2, x3;
-    if ((flags & TF_PI) && (A | Bq)) {
-      if (A >= PI_MAX || Bq >= PI_MAX) {
-        err |= ERR_PI_RANGE;
-        e = make_int4(0, 0, 0, 0);
-      }
-      // table entry (< 2^30) x pi power (< 2^24): signed 32 x 32 -> 64
-      // products, four per coefficient, exact in int64
-      const int4 p = lt->pi[min(A, PI_MAX - 1) * PI_MAX + min(Bq, PI_MAX - 1)];
-      x0 = (int64_t)e.x * p.x + (int64_t)e.y * p.w - (int64_t)e.z * p.z + (int64_t)e.w * p.y;
-      x1 = (int64_t)e.x * p.y + (int64_t)e.y * p.x + (int64_t)e.z * p.w + (int64_t)e.w * p.z;
-      x2 = (int64_t)e.x * p.z + (int64_t)e.y * p.y + (int64_t)e.z * p.x - (int64_t)e.w * p.w;
-      x3 = (int64_t)e.x * p.w - (int64_t)e.y * p.z - (int64_t)e.z * p.y + (int64_t)e.w * p.x;
-    } else {
-      x0 = e.x; x1 = e.y; x2 = e.z; x3 = e.w;
-    }
-    int sh = q + E;
-    if (sh < 0 || sh > SHIFT_MAX) {
-      if (x0 | x1 | x2 | x3) err |= ERR_SHIFT;
-      x0 = x1 = x2 = x3 = 0;
-      sh = 0;
-    }
-    if (flags & TF_RESIDUAL) {
-      // a floatfactor that does not factor: one Z[w] multiply in 128 bits,
-      // out of line so its registers do not burden the common path
-      acc = add_residual_term(acc, x0, x1, x2, x3, (int32_t)th[TH_R0], (int32_t)th[TH_R1],
-                              (int32_t)th[TH_R2], (int32_t)th[TH_R3], sh);
-    } else {
-      // add X * 2^sh into the four 128-bit sums.  The shift is almost always
-      // the same on every lane (it varies only through phase-pair records),
-      // so the common path adds 32-bit limbs at a wave-uniform offset.
-      const int sh0 = __builtin_amdgcn_readfirstlane(sh);
-      if (dead & 1u) sh = sh0;  // a vanished term adds zero at any shift
-      if (__builtin_amdgcn_ballot_w64(sh != sh0) == 0) {
-        const int rr = sh0 & 31, L = sh0 >> 5;
-        add_shifted_uniform(acc.a, x0, rr, L);
-        add_shifted_uniform(acc.b, x1, rr, L);
-        add_shifted_uniform(acc.c, x2, rr, L);
-        add_shifted_uniform(acc.d, x3, rr, L);
-      } else {
-        acc.a += (i128_t)((u128_t)(i128_t)x0 << sh);
-        acc.b += (i128_t)((u128_t)(i128_t)x1 << sh);
-        acc.c += (i128_t)((u128_t)(i128_t)x2 << sh);
-        acc.d += (i128_t)((u128_t)(i128_t)x3 << sh);
+    const int sh = q + E;
+    acc = add_term_digit<true>(acc, e, vanished, flags, A, Bq, lt, th, sh, err);
+    const bool wide = in_tab && !vanished && (unsigned)(n2 + TAB_R) > 2u * TAB_R;
+    if (__builtin_amdgcn_ballot_w64(wide)) {
+      // a rare tail: some lane's |n'| exceeds the single-digit rows
+#pragma nounroll
+      for (int d = 1; d < TAB_DIGITS; ++d) {
+        const int4 ed = wide ? gtab[d * TAB_N + gi] : make_int4(0, 0, 0, 0);
+        acc = add_term_digit<false>(acc, ed, !wide, flags, A, Bq, lt, th, sh + TAB_DIGIT_BITS * d, err);
       }
     }
     th = th + TH_DWORDS + th[TH_SIZE];
@@ -395,7 +421,7 @@ __device__ __forceinline__ void run_shot(const ShotArgs &a, int comp, int64_t s,
 
 // stage the central rows of the materialisation table and the pi powers
 __device__ inline void load_tables(LdsTables *t, const int4 *gtab, const int4 *gpi) {
-  const int4 *src = gtab + (TAB_R - TAB_LDS_R) * 32;
+  const int4 *src = gtab + (TAB_W - TAB_LDS_R) * 32;
   for (int i = threadIdx.x; i < TAB_LDS_N; i += blockDim.x) t->tab[i] = src[i];
   for (int i = threadIdx.x; i < PI_MAX * PI_MAX; i += blockDim.x) t->pi[i] = gpi[i];
   __syncthreads();

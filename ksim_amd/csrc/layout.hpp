@@ -31,11 +31,17 @@ constexpr int PK_OS = 29;
 constexpr int NBIAS = 3;
 constexpr int CHUNK_MAX = 63;
 
-// Materialisation table: w^s (1+w)^r (1+sqrt2)^n for |n| <= TAB_R, r < 4,
-// s < 8, int32 coefficients in the basis (1, w, i, conj w).
+// Materialisation table: w^s (1+w)^r (1+sqrt2)^n for |n| <= TAB_W, r < 4,
+// s < 8, coefficients in the basis (1, w, i, conj w), each written as
+// TAB_DIGITS balanced base-2^31 digits (every digit below 2^30 in magnitude)
+// held in TAB_DIGITS planes of TAB_N int32 x 4 entries.  Rows |n| <= TAB_R
+// are a single digit (planes 1.. are zero there).
 constexpr int TAB_R = 22;
-constexpr int TAB_N = (2 * TAB_R + 1) * 4 * 8;  // entries of 4 x int32
-inline int tab_index(int n, int r, int s) { return ((n + TAB_R) * 4 + r) * 8 + s; }
+constexpr int TAB_W = 64;
+constexpr int TAB_DIGITS = 3;
+constexpr int TAB_DIGIT_BITS = 31;
+constexpr int TAB_N = (2 * TAB_W + 1) * 4 * 8;  // entries per digit plane
+inline int tab_index(int n, int r, int s) { return ((n + TAB_W) * 4 + r) * 8 + s; }
 // the rows |n| <= TAB_LDS_R are staged in LDS per block; the outer rows are
 // read from global memory (rare)
 constexpr int TAB_LDS_R = 17;

@@ -133,21 +133,52 @@ int ceil_log2(int64_t x) {
 }  // namespace
 
 const int32_t *materialise_table() {
-  // w^s (1+w)^r (1+sqrt2)^n as int32 x 4, n in [-TAB_R, TAB_R]
+  // TAB_DIGITS planes of w^s (1+w)^r (1+sqrt2)^n, n in [-TAB_W, TAB_W], as
+  // balanced base-2^31 digits
   static std::vector<int32_t> tab;
   static std::once_flag once;
   std::call_once(once, [] {
-    tab.assign((size_t)TAB_N * 4, 0);
-    for (int n = -TAB_R; n <= TAB_R; ++n)
+    tab.assign((size_t)TAB_DIGITS * TAB_N * 4, 0);
+    for (int n = -TAB_W; n <= TAB_W; ++n)
       for (int r = 0; r < 4; ++r)
         for (int s = 0; s < 8; ++s) {
-          Zw x = zw_mul(zw_mul(zw_unit(s), zw_pow(zw_lambda(), r)), zw_unit_pow(n));
-          if (zw_maxabs(x) >= ((i128)1 << 30)) throw std::logic_error("table entry too large");
-          int32_t *e = &tab[(size_t)tab_index(n, r, s) * 4];
-          e[0] = (int32_t)x.a; e[1] = (int32_t)x.b; e[2] = (int32_t)x.c; e[3] = (int32_t)x.d;
+          const Zw x = zw_mul(zw_mul(zw_unit(s), zw_pow(zw_lambda(), r)), zw_unit_pow(n));
+          const i128 c[4] = {x.a, x.b, x.c, x.d};
+          for (int k = 0; k < 4; ++k) {
+            i128 v = c[k];
+            for (int d = 0; d < TAB_DIGITS; ++d) {
+              const i128 lo = v & (((i128)1 << TAB_DIGIT_BITS) - 1);
+              const i128 dig = lo >= ((i128)1 << (TAB_DIGIT_BITS - 1)) ? lo - ((i128)1 << TAB_DIGIT_BITS) : lo;
+              if (dig >= ((i128)1 << 30) || dig < -((i128)1 << 30))
+                throw std::logic_error("table digit too large");
+              tab[((size_t)d * TAB_N + tab_index(n, r, s)) * 4 + k] = (int32_t)dig;
+              v = (v - dig) >> TAB_DIGIT_BITS;
+            }
+            if (v != 0) throw std::logic_error("table entry needs more digits");
+            if (std::abs(n) <= TAB_R && c[k] != (i128)tab[(size_t)tab_index(n, r, s) * 4 + k])
+              throw std::logic_error("central table row is not a single digit");
+          }
         }
   });
   return tab.data();
+}
+
+// bits of the largest table coefficient over rows |n| <= nmax
+static int table_bits(int nmax) {
+  static std::vector<int> bits;
+  static std::once_flag once;
+  std::call_once(once, [] {
+    bits.assign(TAB_W + 1, 0);
+    for (int n = -TAB_W; n <= TAB_W; ++n)
+      for (int r = 0; r < 4; ++r)
+        for (int s = 0; s < 8; ++s) {
+          const Zw x = zw_mul(zw_mul(zw_unit(s), zw_pow(zw_lambda(), r)), zw_unit_pow(n));
+          int &b = bits[std::abs(n)];
+          b = std::max(b, i128_bits(zw_maxabs(x)));
+        }
+    for (int n = 1; n <= TAB_W; ++n) bits[n] = std::max(bits[n], bits[n - 1]);
+  });
+  return bits[std::min(nmax, TAB_W)];
 }
 
 const int32_t *pi_table() {
@@ -281,6 +312,18 @@ void pack_graph(const ksim_graph_view &g, std::vector<uint32_t> &out, uint32_t *
         "the exact sampler does not accept them");
   if (g.G < 0 || g.P < 0) throw std::invalid_argument("negative graph dimensions");
   const int W32 = std::max(1, (g.P + 31) / 32);
+  {
+    // algorithmic work of the reference formulation (zx_eval.hpp:102-178):
+    // per term, (2W+1) ops per mask with W = ceil(P/64) 64-bit words,
+    // M = n_np + T_hp + 2 T_pp + 2 n_ph masks, 4 ops per node-phase
+    // factor, 12 per phase pair and 48 for the fixed factors and the sum
+    const double W = (double)((g.P + 63) / 64);
+    for (int t = 0; t < g.G; ++t) {
+      const double nnp = g.np_counts ? g.np_counts[t] : 0, nph = g.ph_counts ? g.ph_counts[t] : 0;
+      const double M = nnp + g.T_hp + 2.0 * g.T_pp + 2.0 * nph;
+      st.ops += (2 * W + 1) * M + 4 * nnp + 12 * nph + 48;
+    }
+  }
   std::vector<TermBuild> terms;
   terms.reserve(g.G);
   for (int t = 0; t < g.G; ++t) {
@@ -289,40 +332,58 @@ void pack_graph(const ksim_graph_view &g, std::vector<uint32_t> &out, uint32_t *
     if (!tb.zero) terms.push_back(std::move(tb));
   }
   // ---- exponent ranges and the static alignment exponent
-  struct Range { int vmin, vmax, amax, bmax, emin, emax; };
+  struct Range { int vmin, vmax, nmin, nmax, amax, bmax, emin, emax; };
   std::vector<Range> rg(terms.size());
   int e0 = 0, emax_all = 0;
-  int xbits = 0;
+  int xbits = INT32_MIN;  // max over terms of bits + emax
+  auto lo4 = [](int a, int b, int c, int d) { return std::min(std::min(a, b), std::min(c, d)); };
+  auto hi4 = [](int a, int b, int c, int d) { return std::max(std::max(a, b), std::max(c, d)); };
   for (size_t k = 0; k < terms.size(); ++k) {
     const TermBuild &tb = terms[k];
-    Range r{tb.V, tb.V, tb.A, tb.B, 0, 0};
+    Range r{tb.V, tb.V, tb.N, tb.N, tb.A, tb.B, 0, 0};
     for (const LinRec &l : tb.lin) {
       r.vmin += std::min(l.e0.v, l.e1.v); r.vmax += std::max(l.e0.v, l.e1.v);
+      r.nmin += std::min(l.e0.n, l.e1.n); r.nmax += std::max(l.e0.n, l.e1.n);
       r.amax += std::max(l.e0.a, l.e1.a); r.bmax += std::max(l.e0.b, l.e1.b);
     }
     for (const PairRec &p : tb.pair) {
-      r.vmin += std::min(std::min(p.e00.v, p.e10.v), std::min(p.e01.v, p.e11.v));
-      r.vmax += std::max(std::max(p.e00.v, p.e10.v), std::max(p.e01.v, p.e11.v));
-      r.amax += std::max(std::max(p.e00.a, p.e10.a), std::max(p.e01.a, p.e11.a));
-      r.bmax += std::max(std::max(p.e00.b, p.e10.b), std::max(p.e01.b, p.e11.b));
+      r.vmin += lo4(p.e00.v, p.e10.v, p.e01.v, p.e11.v);
+      r.vmax += hi4(p.e00.v, p.e10.v, p.e01.v, p.e11.v);
+      r.nmin += lo4(p.e00.n, p.e10.n, p.e01.n, p.e11.n);
+      r.nmax += hi4(p.e00.n, p.e10.n, p.e01.n, p.e11.n);
+      r.amax += hi4(p.e00.a, p.e10.a, p.e01.a, p.e11.a);
+      r.bmax += hi4(p.e00.b, p.e10.b, p.e01.b, p.e11.b);
     }
     r.emin = tb.E + (r.vmin >> 2);
     r.emax = tb.E + (r.vmax >> 2);
     rg[k] = r;
     if (k == 0 || r.emin < e0) e0 = r.emin;
     if (k == 0 || r.emax > emax_all) emax_all = r.emax;
-    // table entry (< 2^30) x pi power (< 2^24, four products) stays in int64;
-    // a residual floatfactor multiplies in 128 bits
-    int bits = 30;
+    // |coefficients| of w^s (1+w)^r (1+sqrt2)^n' with n' = n + 2 floor(v/4):
+    // below 2^30 inside the table range, about 2.414^|n'| * 8 beyond it
+    const int n2max = std::max(std::abs(r.nmin + 2 * (r.vmin >> 2)),
+                               std::abs(r.nmax + 2 * (r.vmax >> 2)));
+    if (n2max > TAB_W)
+      throw std::invalid_argument("(1+sqrt2) exponent range beyond +-" + std::to_string(TAB_W));
+    // a row beyond TAB_R adds as several digits whose partial sums stay
+    // within twice the entry
+    int bits = n2max <= TAB_R ? 30 : table_bits(n2max) + 1;
+    st.n2_bound = std::max(st.n2_bound, n2max);
+    st.wide_terms += n2max > TAB_R;
     if (r.amax > 0 || r.bmax > 0) bits += pi_table_bits(r.amax, r.bmax) + 2;
     if (tb.residual) bits += i128_bits(zw_maxabs(tb.R)) + 2;
-    xbits = std::max(xbits, bits);
+    // |term * 2^-e0| < 2^(bits + emax - e0): the largest such bound, plus
+    // the term count, must stay inside the signed 128-bit sum
+    xbits = std::max(xbits, bits + r.emax);
   }
   const int max_shift = terms.empty() ? 0 : emax_all - e0;
   if (!terms.empty()) {
-    if (xbits + max_shift + ceil_log2((int64_t)terms.size()) + 1 > SHIFT_MAX)
+    const int need = xbits - e0 + ceil_log2((int64_t)terms.size()) + 1;
+    st.sum_bits = std::max(st.sum_bits, need + 1);
+    if (need > SHIFT_MAX)
       throw std::invalid_argument(
-          "term exponent spread exceeds the 128-bit exact sum of the sampler");
+          "term magnitudes exceed the 128-bit exact sum of the sampler (" +
+          std::to_string(need + 1) + " bits needed)");
   }
   // ---- emit
   ghdr[GH_NTERMS] = (uint32_t)terms.size();
@@ -510,7 +571,13 @@ std::vector<uint32_t> pack_program(const ksim_graph_view *graphs, const int32_t 
       st.graphs += 1;
       st.terms += j.st.terms;
       st.records += j.st.records;
+      st.n2_bound = std::max(st.n2_bound, j.st.n2_bound);
+      st.wide_terms += j.st.wide_terms;
+      st.sum_bits = std::max(st.sum_bits, j.st.sum_bits);
+      st.ops += j.st.ops;
     }
+    // exact magnitude, compare and running-probability update per output
+    st.ops += 32.0 * std::max(0, ng - 1);
     gi += ng;
     fi += F;
     oi += n_out;

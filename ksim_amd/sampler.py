@@ -204,28 +204,48 @@ class ProgramArgs:
                 _lib.ptr(self.dest, C.c_int32), self.num_outputs)
 
 
-def pack_words(prog: FlatProgram) -> np.ndarray:
-    """The device program words of ``prog`` (host-only packing)."""
+_STAT_KEYS = ("graphs", "terms", "records", "words", "ops_per_shot", "n2_bound", "wide_terms",
+              "sum_bits")
+
+
+def _stats_dict(st: np.ndarray) -> dict:
+    d = {k: (float(v) if k == "ops_per_shot" else int(v)) for k, v in zip(_STAT_KEYS, st)}
+    d["device_bytes"] = d["words"] * 4
+    return d
+
+
+def pack_words(prog: FlatProgram, stats: dict | None = None) -> np.ndarray:
+    """The device program words of ``prog`` (host-only packing); ``stats``,
+    when given, receives the packer's figures (as HipProgramSampler.stats)."""
     lib = _lib.load()
     pa = ProgramArgs(prog)
     n = C.c_int64()
-    _lib.check(lib.ksim_program_pack(*pa.args(), None, 0, C.byref(n)))
+    _lib.check(lib.ksim_program_pack(*pa.args(), None, 0, C.byref(n), None))
     w = np.zeros(max(n.value, 1), dtype=np.uint32)
-    _lib.check(lib.ksim_program_pack(*pa.args(), _lib.ptr(w, C.c_uint32), w.size, C.byref(n)))
+    st = np.zeros(8, np.float64)
+    _lib.check(lib.ksim_program_pack(*pa.args(), _lib.ptr(w, C.c_uint32), w.size, C.byref(n),
+                                     _lib.ptr(st, C.c_double)))
+    if stats is not None:
+        stats.update(_stats_dict(st))
     return w[:n.value]
 
 
 def lookup_tables():
-    """(tab (TAB_N, 4) int32, pitab (PI_MAX*PI_MAX, 4) int32, TAB_R, PI_MAX)."""
+    """Materialisation tables of the kernels.
+
+    Returns (tab (D, (2W+1)*32, 4) int32 digit planes, pitab (PI_MAX*PI_MAX, 4)
+    int32, W, PI_MAX, D, G): entry ((n+W)*4+r)*8+s of plane d is digit d, in
+    balanced base 2^G, of the coefficients of w^s (1+w)^r (1+sqrt2)^n.
+    """
     lib = _lib.load()
-    sizes = np.zeros(2, np.int32)
+    sizes = np.zeros(4, np.int32)
     _lib.check(lib.ksim_tables(None, 0, None, 0, _lib.ptr(sizes, C.c_int32)))
-    R, PM = int(sizes[0]), int(sizes[1])
-    tab = np.zeros(((2 * R + 1) * 32, 4), np.int32)
+    W, PM, D, G = (int(x) for x in sizes)
+    tab = np.zeros((D, (2 * W + 1) * 32, 4), np.int32)
     pit = np.zeros((PM * PM, 4), np.int32)
     _lib.check(lib.ksim_tables(_lib.ptr(tab, C.c_int32), tab.size, _lib.ptr(pit, C.c_int32),
                                pit.size, None))
-    return tab, pit, R, PM
+    return tab, pit, W, PM, D, G
 
 
 class HipProgramSampler:
@@ -251,7 +271,7 @@ class HipProgramSampler:
             _lib.ptr(Fpc_a, C.c_int32), self.n_comp, num_f, _lib.ptr(dest_a, C.c_int32),
             self.num_outputs, C.byref(h)))
         self._native = _Native(h)
-        self.ops_per_shot = algorithmic_ops(prog)
+        self.ops_per_shot = self.stats()["ops_per_shot"]
 
     def _prep_f(self, f_params) -> np.ndarray:
         f = np.ascontiguousarray(f_params, dtype=np.uint8)
@@ -311,11 +331,9 @@ class HipProgramSampler:
         return self._native._h
 
     def stats(self) -> dict:
-        st = np.zeros(5, dtype=np.float64)
+        st = np.zeros(8, dtype=np.float64)
         _lib.check(_lib.load().ksim_program_stats(self._native._h, _lib.ptr(st, C.c_double)))
-        return dict(graphs=int(st[0]), terms=int(st[1]), records=int(st[2]),
-                    words=int(st[3]), device_bytes=int(st[3]) * 4,
-                    ops_per_shot=self.ops_per_shot)
+        return _stats_dict(st)
 
 
 KokkosProgramSampler = HipProgramSampler

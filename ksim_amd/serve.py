@@ -83,6 +83,7 @@ class ServingLoop:
                  rate: float = 2.5e13, streams: int = 3):
         import torch
         from .batch import device_info
+        torch.cuda.init()  # stream creation alone does not initialise torch's device state
         self.torch = torch
         self.lib = _lib.load()
         self.hbm_budget = int(hbm_budget)

@@ -278,6 +278,35 @@ def shape_program(n_q: int, t: int, F: int, n_out: int, seed: int,
     return FlatProgram(comps, order, n_out, n_out)
 
 
+def wide_exponent_program(n_out: int = 3, G: int = 6, T: int = 110, F: int = 6,
+                          seed: int = 0) -> FlatProgram:
+    """Terms whose (1 + sqrt2) exponent n' = N + 2 floor(V/4) reaches about
+    +-T/2, far beyond the single-digit table rows.  Every node factor
+    1 + w^k with odd k carries one power of (1 + w); k in {1, 7} adds no
+    (1 + sqrt2) power and k in {3, 5} adds (1 + sqrt2)^-1, so a term whose
+    ~T live phases come from one pair sits near n' = +T/2 or -T/2.  Sparse
+    masks (a parity flip turns k into k + 4) spread the lanes around it, and
+    power2 rescales each term to magnitude ~1.  The terms of one graph share
+    a pair (odd graphs {3, 5}), so one graph's terms need similar scales."""
+    rng = np.random.default_rng(seed)
+    graphs = []
+    pairs = (np.array([1, 7], np.uint8), np.array([3, 5], np.uint8))
+    for i in range(n_out + 1):
+        P = F + i
+        g = empty_graph(G, P, Tnp=T)
+        g.np_phases[:] = rng.choice(pairs[i % 2], size=(G, T))
+        if P:
+            g.np_params[:] = _bernoulli_u8(rng, (G, T, P), 0.05)
+        g.np_counts[:] = rng.integers(T - 10, T + 1, size=G)
+        g.pf_phase_idx[:] = rng.integers(0, 8, size=G)
+        # |1 + w^k| = 1.848 (k = 1, 7) or 0.765 (k = 3, 5)
+        mag = math.log2(1.8478) if i % 2 == 0 else math.log2(0.7654)
+        g.pf_power2[:] = (-np.round(g.np_counts * mag)).astype(np.int32) \
+            + rng.integers(-2, 1, size=G).astype(np.int32)
+        graphs.append(g)
+    return single_program(graphs, F=F)
+
+
 def random_f(B: int, F: int, p: float, seed: int) -> np.ndarray:
     rng = np.random.default_rng(seed)
     return (rng.random((B, F)) < p).astype(np.uint8)

@@ -50,7 +50,8 @@ def popc_and(pv, words):
 class Packed:
     def __init__(self, words: np.ndarray, tables):
         self.w = [int(x) for x in words]
-        self.tab, self.pitab, self.R, self.PM = tables
+        self.tab, self.pitab, self.W, self.PM, self.D, self.G = tables
+        self.max_n2 = 0  # largest |n'| of a live term seen by eval_graph
 
     def comp(self, c):
         off = self.w[PH_DWORDS + c]
@@ -110,9 +111,13 @@ class Packed:
                 S += x >> PK_OS
             q, r = V >> 2, V & 3
             n2, s2 = N + 2 * q, (S + 2 * q) & 7
-            if not -self.R <= n2 <= self.R:
+            if not -self.W <= n2 <= self.W:
                 raise OverflowError("table range")
-            X = tuple(int(v) for v in self.tab[((n2 + self.R) * 4 + r) * 8 + s2])
+            if not dead & 1:
+                self.max_n2 = max(self.max_n2, abs(n2))
+            gi = ((n2 + self.W) * 4 + r) * 8 + s2
+            X = tuple(sum(int(self.tab[d, gi, k]) << (self.G * d) for d in range(self.D))
+                      for k in range(4))
             if h[TH["FLAGS"]] & TF_PI and (A or B):
                 X = zmul(X, tuple(int(v) for v in self.pitab[A * self.PM + B]))
             if h[TH["FLAGS"]] & TF_RESIDUAL:

@@ -55,6 +55,21 @@ def test_eval_exact_shapes(shape):
     assert gpu_exact_matches(prog, 8, 2) > 0
 
 
+def test_wide_table_rows():
+    # |n'| up to ~60: terms beyond the single-digit rows add digit by digit
+    log("eval_exact wide rows")
+    prog = synth.wide_exponent_program(seed=5)
+    s = HipProgramSampler(prog)
+    st = s.stats()
+    assert st["wide_terms"] > 0 and st["n2_bound"] > 46
+    assert gpu_exact_matches(prog, 64, 3) > 0
+    log("streams wide rows")
+    B = 256
+    f = synth.random_f(B, 6, 0.5, 9)
+    U = np.random.default_rng(9).random((B, prog.num_outputs))
+    assert np.array_equal(s.sample_uniforms(f, U), exact.sample_program(prog, f, U))
+
+
 def test_eval_abs_matches_oracle_and_reference(golden_files):
     for fn in golden_files:
         d = np.load(fn)

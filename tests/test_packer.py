@@ -41,9 +41,9 @@ def check_program(prog, tables, rows=10, seed=0, all_rows=False):
 
 
 def test_tables_are_the_closed_forms(tables):
-    tab, pit, R, PM = tables
+    tab, pit, W, PM, D, G = tables
     lam, up, um = (1, 1, 0, 0), (1, 1, 0, 1), (-1, 1, 0, 1)
-    for n in (-R, -3, 0, 5, R):
+    for n in (-W, -40, -23, -22, -3, 0, 5, 22, 23, 47, W):
         for r in range(4):
             for s in range(8):
                 x = exact.UNIT[s]
@@ -51,7 +51,23 @@ def test_tables_are_the_closed_forms(tables):
                     x = exact.zmul(x, lam)
                 for _ in range(abs(n)):
                     x = exact.zmul(x, up if n > 0 else um)
-                assert tuple(int(v) for v in tab[((n + R) * 4 + r) * 8 + s]) == x
+                gi = ((n + W) * 4 + r) * 8 + s
+                digits = tab[:, gi, :].astype(np.int64)
+                assert np.all(np.abs(digits) < 2 ** 30)
+                got = tuple(sum(int(digits[d, k]) << (G * d) for d in range(D)) for k in range(4))
+                assert got == x, (n, r, s)
+                if abs(n) <= 22:  # central rows are a single digit
+                    assert not digits[1:].any()
+    # pi^a pibar^b
+    pi, pib = (1, 1, 0, -1), (1, -1, 0, 1)  # 1 +- i sqrt2, i sqrt2 = w - conj w
+    for a in (0, 1, 7, PM - 1):
+        for b in (0, 3, PM - 1):
+            x = (1, 0, 0, 0)
+            for _ in range(a):
+                x = exact.zmul(x, pi)
+            for _ in range(b):
+                x = exact.zmul(x, pib)
+            assert tuple(int(v) for v in pit[a * PM + b]) == x
 
 
 def test_golden_programs(tables, golden_files):
@@ -168,3 +184,37 @@ def test_high_t_count(tables, t):
         d = graph_to_dict(g)
         bits = rng.integers(0, 2, size=g.np_params.shape[2])
         assert exact.canonical(*pk.eval_graph(0, i, bits)) == exact.graph_value(d, bits)
+
+
+def test_wide_rows(tables):
+    # n' = N + 2 (V >> 2) far beyond the single-digit rows: such terms add as
+    # several base-2^31 digits
+    prog = synth.wide_exponent_program(seed=5)
+    st = {}
+    pk = Packed(pack_words(prog, st), tables)
+    assert st["wide_terms"] > 0 and st["n2_bound"] > 40
+    rng = np.random.default_rng(5)
+    comp = prog.components[0]
+    for i, g in enumerate(comp.graphs):
+        d = graph_to_dict(g)
+        for _ in range(4):
+            bits = rng.integers(0, 2, size=g.np_params.shape[2])
+            assert exact.canonical(*pk.eval_graph(0, i, bits)) == exact.graph_value(d, bits)
+    assert pk.max_n2 > 46  # the third digit plane is used too
+
+
+def test_rejects_beyond_the_widest_row():
+    prog = synth.wide_exponent_program(n_out=1, G=2, T=300, seed=1)
+    with pytest.raises(ValueError):
+        pack_words(prog)
+
+
+def test_pack_stats_ops_match_the_formula():
+    from ksim_amd.sampler import algorithmic_ops
+    for prog in (synth.shape_program(n_q=8, t=8, F=6, n_out=5, seed=1),
+                 synth.config1_program(), synth.htkh_program(3, noisy_f=True)):
+        st = {}
+        w = pack_words(prog, st)
+        assert st["words"] == len(w)
+        assert st["ops_per_shot"] == algorithmic_ops(prog)
+        assert 0 < st["sum_bits"] <= 128

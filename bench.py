@@ -34,13 +34,27 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
-# MI355X peaks.  VALU: 256 CUs x 64 lanes x 2.4 GHz = 39.3e12 32-bit lane-ops/s
-# (the published 157.3 TFLOP/s FP32 vector peak counts 2 flops per packed-FMA
-# half, i.e. 4 per lane-op); an algorithmic 64-bit integer add/logic op takes
-# two 32-bit lane-ops.
+# MI355X peaks.  VALU: 256 CUs x 64 lanes x 2.4 GHz = 39.3e12 lane-ops/s, the
+# issue rate of 32-bit integer VALU ops (and of fp64 FMAs: the published
+# 78.6 TFLOP/s fp64 vector peak counts 2 flops per FMA).  The roofline
+# divides algorithmic ops (the reference formulation's 64-bit integer ops,
+# counted by the packer) by this peak: one lane-op per algorithmic op is the
+# bound this kernel can reach, since its bit-sliced masks and packed exponent
+# adds spend about one 32-bit op per 64-bit op of the reference formulation.
+# INT64_PEAK (two lane-ops per 64-bit add or logic op) is the rate of a kernel
+# that executes the reference's op mix literally; it is reported beside it.
 VALU32_PEAK = 256 * 64 * 2.4e9
 INT64_PEAK = VALU32_PEAK / 2
 HBM_PEAK = 8.0e12
+
+
+def valu_roofline(ops_per_s, bytes_per_s, traffic=None):
+    """Roofline object of a VALU-bound launch: algorithmic ops/s against the
+    VALU lane-op peak, with the int64-op peak and HBM rate beside it."""
+    return dict(bound="valu", achieved=ops_per_s, peak=VALU32_PEAK, unit="op/s",
+                frac=ops_per_s / VALU32_PEAK, traffic=traffic,
+                frac_int64_peak=ops_per_s / INT64_PEAK,
+                hbm_GBps=bytes_per_s / 1e9, hbm_frac=bytes_per_s / HBM_PEAK)
 
 
 def log(*a):
@@ -157,9 +171,7 @@ def plain_result(name, steps, warmup, with_cpu=True):
     res = dict(workload=name, desc=w["desc"], shots=w["B"], ms_per_step=ms,
                shots_per_s=shots_s, us_per_shot=1e6 / shots_s,
                kernel_ms=kern * 1e3, ops_per_shot=pl.ops, stats=pl.s.stats(),
-               roofline=dict(bound="valu_int64", achieved=achieved, peak=INT64_PEAK,
-                             unit="int64-op/s", frac=achieved / INT64_PEAK, traffic=None,
-                             hbm_GBps=pl.bytes_per_shot * w["B"] / kern / 1e9))
+               roofline=valu_roofline(achieved, pl.bytes_per_shot * w["B"] / kern))
     if with_cpu:
         # about 10-20 s of CPU work
         probe = cpu_baseline_port(w["prog"], w["F"], 200)
@@ -213,8 +225,7 @@ def batched_result(steps, warmup, n_programs=64):
                 batched_shots_per_s=batch_sps, speedup=batch_sps / plain_sps,
                 plain_ms=wall_p / steps * 1e3, batched_ms=wall_b / steps * 1e3,
                 batched_equals_plain=bool(same), ops_total=batch.ops,
-                roofline=dict(bound="valu_int64", achieved=achieved, peak=INT64_PEAK,
-                              unit="int64-op/s", frac=achieved / INT64_PEAK, traffic=None))
+                roofline=valu_roofline(achieved, batch.bytes / kern_b))
 
 
 def tcount_sweep(ts=(0, 4, 8, 12, 16, 20, 24, 32, 40), shots=100_000, n_q=30):
@@ -236,7 +247,7 @@ def tcount_sweep(ts=(0, 4, 8, 12, 16, 20, 24, 32, 40), shots=100_000, n_q=30):
         rows.append(dict(t=t, terms_mean=float(np.mean([g.pf_phase_idx.shape[0]
                                                          for g in prog.components[0].graphs])),
                          shots_per_s=shots * 2 / wall, us_per_shot=wall / 2 / shots * 1e6,
-                         kernel_ms=kern * 1e3, valu_int64_frac=pl.ops * shots / kern / INT64_PEAK))
+                         kernel_ms=kern * 1e3, valu_frac=pl.ops * shots / kern / VALU32_PEAK))
         log(f"[bench] T sweep t={t}: {rows[-1]['us_per_shot']:.4f} us/shot")
         del pl
         torch.cuda.empty_cache()
@@ -295,6 +306,9 @@ def main():
                     help="config1 | config2 | config3 | config4 (batched)")
     ap.add_argument("--suite", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="launch one elementwise int32 copy kernel of this size first, so "
+                         "counter runs can scale FETCH_SIZE / WRITE_SIZE against known bytes")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         sys.exit("bench.py runs on one GPU only: this sampler build has no multi-GPU path "
@@ -302,6 +316,14 @@ def main():
     import torch
     assert torch.cuda.is_available(), "no GPU"
     torch.cuda.set_device(0)
+    if args.calib_copy_mib:
+        n = args.calib_copy_mib << 18  # int32 elements
+        src = torch.ones(n, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.neg(src, out=dst)  # reads and writes n * 4 bytes
+        torch.cuda.synchronize()
+        log(f"[bench] calibration copy: {n * 4} bytes read, {n * 4} bytes written")
+        del src, dst
     if args.workload == "config4":
         r4 = batched_result(args.steps, args.warmup)
         print(json.dumps({"metric": "config4 batched", **r4}), flush=True)

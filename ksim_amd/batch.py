@@ -60,6 +60,9 @@ class BatchedSampler:
         self._h = h
         self.total_shots = int(Bs.sum())
         self.ops = float(sum(s.ops_per_shot * int(b) for s, b in zip(samplers, Bs)))
+        # unavoidable HBM bytes: f rows in, output rows out
+        self.bytes = float(sum((int(fk.shape[1]) + s.num_outputs) * int(b)
+                               for s, fk, b in zip(samplers, f, Bs)))
         info = device_info()
         # persistent grid: enough 256-thread blocks for every CU to hold
         # several (LDS and VGPRs bound occupancy at ~4-6 blocks per CU)

@@ -94,6 +94,7 @@ class ServingLoop:
         self.tile_shots = int(tile_shots)
         self.max_items = int(max_items)
         self.blocks = int(blocks or device_info()["cus"] * 6)
+        self.min_items = min(self.blocks * 4, max_items)  # 4 waves per 256-thread block
         h = C.c_void_p()
         _lib.check(self.lib.ksim_sched_create(self.max_slots, self.max_items, C.byref(h)))
         self._h = h
@@ -238,8 +239,13 @@ class ServingLoop:
                 if len(inflight) < len(self.streams) and any(a.tiles for a in active):
                     budget = self.rate * self.step_s
                     items, taken, est = [], [], 0.0
-                    # round-robin over active requests, one tile each per pass
-                    while est < budget and len(items) < self.max_items:
+                    # round-robin over active requests, one tile each per pass,
+                    # until the step holds the estimated work of step_ms and at
+                    # least one tile per wave of the persistent grid (a tile
+                    # is one wave's work: fewer tiles leave SIMDs idle)
+                    def short():
+                        return est < budget or len(items) < self.min_items
+                    while short() and len(items) < self.max_items:
                         progressed = False
                         for a in active:
                             if not a.tiles:
@@ -250,7 +256,7 @@ class ServingLoop:
                             a.outstanding += 1
                             est += a.sampler.ops_per_shot * cnt / max(a.sampler.n_comp, 1)
                             progressed = True
-                            if est >= budget or len(items) >= self.max_items:
+                            if not short() or len(items) >= self.max_items:
                                 break
                         if not progressed:
                             break

@@ -207,7 +207,8 @@ def batched_result(steps, warmup, n_programs=64):
             _lib.check(lib.ksim_program_sample_device(s.handle, f.data_ptr() if f.numel() else None,
                                                       f.shape[0], f.shape[1], 0, sd, None,
                                                       o.data_ptr(), st))
-    batch = BatchedSampler(samplers, fs, seeds)
+    # launch shape from the autotuner (timed on this mix, outside the timed region)
+    batch = BatchedSampler.tuned(samplers, fs, seeds, use_cache=False, log=log)
     total = batch.total_shots
     wall_p, dev_p = time_steps(plain_step, steps, warmup)
     for s in samplers:
@@ -225,6 +226,9 @@ def batched_result(steps, warmup, n_programs=64):
                 batched_shots_per_s=batch_sps, speedup=batch_sps / plain_sps,
                 plain_ms=wall_p / steps * 1e3, batched_ms=wall_b / steps * 1e3,
                 batched_equals_plain=bool(same), ops_total=batch.ops,
+                tuned=dict(wave_shots=batch.tune["wave_shots"],
+                           blocks_per_cu=batch.tune["blocks_per_cu"],
+                           tune_s=batch.tune["tune_s"]),
                 roofline=valu_roofline(achieved, batch.bytes / kern_b))
 
 

@@ -38,7 +38,7 @@ class BatchedSampler:
     """
 
     def __init__(self, samplers: list[HipProgramSampler], f: list, seeds: list[int],
-                 wave_shots: int = 64):
+                 wave_shots: int = 128, blocks_per_cu: int = 6):
         import torch
         torch.cuda.init()
         self.torch = torch
@@ -65,8 +65,9 @@ class BatchedSampler:
                                for s, fk, b in zip(samplers, f, Bs)))
         info = device_info()
         # persistent grid: enough 256-thread blocks for every CU to hold
-        # several (LDS and VGPRs bound occupancy at ~4-6 blocks per CU)
-        self.blocks = info["cus"] * 4
+        # as many as registers allow (5 waves per SIMD, 5 blocks per CU)
+        self.wave_shots = int(wave_shots)
+        self.blocks = info["cus"] * int(blocks_per_cu)
 
     def run(self, use_graph: bool = True, blocks: int | None = None, stream=None):
         st = stream if stream is not None else self.torch.cuda.current_stream().cuda_stream
@@ -75,6 +76,16 @@ class BatchedSampler:
 
     def check(self):
         _lib.check(_lib.load().ksim_batch_check(self._h))
+
+    @classmethod
+    def tuned(cls, samplers, f, seeds, **kw):
+        """A batch with the launch shape the autotuner picks for this mix
+        (ksim_amd.autotune; cached per device and batch signature)."""
+        from .autotune import tune_batch
+        t = tune_batch(samplers, f, seeds, **kw)
+        b = cls(samplers, f, seeds, wave_shots=t["wave_shots"], blocks_per_cu=t["blocks_per_cu"])
+        b.tune = t
+        return b
 
     def __del__(self):
         h, self._h = getattr(self, "_h", None), None

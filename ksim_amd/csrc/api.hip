@@ -99,6 +99,13 @@ std::string err_text(uint32_t e) {
 namespace ksim {
 
 constexpr int kBlock = 256;
+// waves per SIMD the sampling kernels are compiled for: caps the VGPRs per
+// lane (6 -> 80, 5 -> 96, 4 -> 128).  At 5 the hot loops hold no spills
+// (at 6 the term loop spills to scratch), which measured fastest on the
+// plain and the batched paths.
+#ifndef KSIM_WAVES_PER_EU
+#define KSIM_WAVES_PER_EU 5
+#endif
 
 __global__ void k_fill_u32(uint32_t *p, uint32_t v, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -112,7 +119,7 @@ __global__ void k_fill_u8(uint8_t *p, uint8_t v, int64_t n) {
 
 // Plain path: one lane per shot, grid (ceil(B/256), n_comp).
 template <int WL>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_sample(ShotArgs a, const int4 *__restrict__ gtab,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KSIM_WAVES_PER_EU))) void k_sample(ShotArgs a, const int4 *__restrict__ gtab,
                                                   const int4 *__restrict__ pitab) {
   __shared__ LdsTables lt;
   load_tables(&lt, gtab, pitab);
@@ -195,7 +202,7 @@ __global__ void k_reset_cursor(uint32_t *cursor, uint32_t *clear) {
 }
 
 template <int WL>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_batch(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KSIM_WAVES_PER_EU))) void k_batch(
     const BatchProg *__restrict__ progs, const BatchItem *__restrict__ items, uint32_t n_items,
     uint32_t *cursor, uint32_t *err, const int4 *__restrict__ gtab,
     const int4 *__restrict__ pitab) {

@@ -10,12 +10,14 @@ from ksim_amd.sampler import HipProgramSampler
 pytestmark = pytest.mark.gpu
 
 
-def test_batch_equals_alone():
-    import torch
-    specs = [dict(n_q=5, t=4, F=3, n_out=5, seed=1, B=700),
+SPECS = [dict(n_q=5, t=4, F=3, n_out=5, seed=1, B=700),
              dict(n_q=8, t=8, F=10, n_out=9, seed=2, B=130),
              dict(n_q=6, t=0, F=40, n_out=12, seed=3, B=1000),
-             dict(n_q=4, t=6, F=0, n_out=3, seed=4, B=65)]
+         dict(n_q=4, t=6, F=0, n_out=3, seed=4, B=65)]
+
+
+def build(specs):
+    import torch
     samplers, fs, seeds, alone = [], [], [], []
     for k, sp in enumerate(specs):
         prog = synth.shape_program(sp["n_q"], sp["t"], sp["F"], sp["n_out"], sp["seed"],
@@ -26,7 +28,20 @@ def test_batch_equals_alone():
         fs.append(torch.from_numpy(f).cuda())
         seeds.append(1000 + k)
         alone.append(s.sample(f, 1000 + k))
-    for wave_shots in (64, 128):
+    return samplers, fs, seeds, alone
+
+
+def check_same(b, samplers, alone):
+    for k in range(len(samplers)):
+        got = b.out[k].cpu().numpy()[:, :samplers[k].num_outputs]
+        assert np.array_equal(got, alone[k]), k
+
+
+def test_batch_equals_alone():
+    import torch
+    specs = SPECS
+    samplers, fs, seeds, alone = build(specs)
+    for wave_shots in (64, 128, 256):
         b = BatchedSampler(samplers, fs, seeds, wave_shots=wave_shots)
         for use_graph in (False, True, True):
             for o in b.out:
@@ -37,3 +52,24 @@ def test_batch_equals_alone():
             for k in range(len(specs)):
                 got = b.out[k].cpu().numpy()[:, :samplers[k].num_outputs]
                 assert np.array_equal(got, alone[k]), (wave_shots, use_graph, k)
+
+
+def test_autotuned_batch_equals_alone(tmp_path, monkeypatch):
+    import torch
+    from ksim_amd import autotune
+    monkeypatch.setenv("KSIM_TUNE_CACHE", str(tmp_path / "tune.json"))
+    samplers, fs, seeds, alone = build(SPECS)
+    b = BatchedSampler.tuned(samplers, fs, seeds, reps=1, wave_shots=(64, 128),
+                             blocks_per_cu=(2, 5))
+    assert not b.tune["cached"] and len(b.tune["trials"]) == 4
+    from ksim_amd.batch import device_info
+    cus = device_info()["cus"]
+    assert b.wave_shots in (64, 128) and b.blocks in (2 * cus, 5 * cus)
+    b.run()
+    torch.cuda.synchronize()
+    b.check()
+    check_same(b, samplers, alone)
+    # second call: served from the cache file
+    again = autotune.tune_batch(samplers, fs, seeds, reps=1, wave_shots=(64, 128),
+                                blocks_per_cu=(2, 5))
+    assert again["cached"] and again["wave_shots"] == b.wave_shots

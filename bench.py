@@ -181,13 +181,13 @@ def plain_result(name, steps, warmup, with_cpu=True):
     return res
 
 
-def batched_result(steps, warmup, n_programs=64):
+def batched_result(steps, warmup, n_programs=64, tune=True):
     """config4: the same 64 programs sampled (a) by one plain launch each,
     back to back on one stream, and (b) by one persistent batched launch
     replayed from a hipGraph; both with resident inputs."""
     import torch
     from ksim_amd import synth
-    from ksim_amd.batch import BatchedSampler
+    from ksim_amd.batch import BatchedSampler, device_info
     from ksim_amd.sampler import HipProgramSampler
     specs = synth.config4_specs(n_programs=n_programs)
     log(f"[bench] config4: building {len(specs)} programs")
@@ -208,7 +208,12 @@ def batched_result(steps, warmup, n_programs=64):
                                                       f.shape[0], f.shape[1], 0, sd, None,
                                                       o.data_ptr(), st))
     # launch shape from the autotuner (timed on this mix, outside the timed region)
-    batch = BatchedSampler.tuned(samplers, fs, seeds, use_cache=False, log=log)
+    if tune:
+        batch = BatchedSampler.tuned(samplers, fs, seeds, use_cache=False, log=log)
+    else:
+        batch = BatchedSampler(samplers, fs, seeds)
+        batch.tune = dict(wave_shots=batch.wave_shots,
+                          blocks_per_cu=batch.blocks // device_info()["cus"], tune_s=0.0)
     total = batch.total_shots
     wall_p, dev_p = time_steps(plain_step, steps, warmup)
     for s in samplers:
@@ -310,6 +315,8 @@ def main():
                     help="config1 | config2 | config3 | config4 (batched)")
     ap.add_argument("--suite", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="config4: default launch shape instead of the autotuner's")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="launch one elementwise int32 copy kernel of this size first, so "
                          "counter runs can scale FETCH_SIZE / WRITE_SIZE against known bytes")
@@ -329,7 +336,7 @@ def main():
         log(f"[bench] calibration copy: {n * 4} bytes read, {n * 4} bytes written")
         del src, dst
     if args.workload == "config4":
-        r4 = batched_result(args.steps, args.warmup)
+        r4 = batched_result(args.steps, args.warmup, tune=not args.no_tune)
         print(json.dumps({"metric": "config4 batched", **r4}), flush=True)
         return
     if args.workload == "config5":

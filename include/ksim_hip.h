@@ -146,9 +146,9 @@ int ksim_program_sample_device(ksim_program *p, const uint8_t *d_f, int64_t B,
                                int32_t f_cols, int64_t shot_base, uint64_t seed,
                                const double *d_U, uint8_t *d_out, void *stream);
 
-/* Launch shape of the plain kernel: threads per block (64..1024, multiple of
- * 64) and waves per shot group (1 = each wave evaluates all terms of its 64
- * shots; k > 1 = k waves split each graph's terms and reduce through LDS). */
+/* Launch shape of the plain kernel: threads per block and lanes per shot
+ * (1 = each lane evaluates every term of its own shot).  Only (256, 1) is
+ * implemented; other shapes return KSIM_E_INVALID. */
 int ksim_program_set_tiles(ksim_program *p, int32_t block_threads,
                            int32_t term_split);
 

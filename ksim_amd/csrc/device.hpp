@@ -339,6 +339,17 @@ __device__ __forceinline__ void set_param_bit(uint32_t (&pv)[WL], int pos, uint3
   }
 }
 
+// parameter bit `pos` (wave-uniform) of the lane's words
+template <int WL>
+__device__ __forceinline__ uint32_t get_param_bit(const uint32_t (&pv)[WL], int pos) {
+  const int word = __builtin_amdgcn_readfirstlane(pos >> 5);
+  uint32_t x = 0;
+#pragma unroll
+  for (int w = 0; w < WL; ++w)
+    if (w == word) x = pv[w];
+  return (x >> (pos & 31)) & 1u;
+}
+
 struct ShotArgs {
   const uint32_t *prog;
   const uint8_t *f;   // (B, num_f)
@@ -396,11 +407,10 @@ __device__ __forceinline__ void run_shot(const ShotArgs &a, int comp, int64_t s,
   uint64_t rs = rng_seed(a.seed + 0x9E37ull * (uint64_t)comp, (uint64_t)(a.shot_base + s));
   uint32_t err = 0;
   Acc4 acc;
-  const double *urow = a.U ? a.U + s * (int64_t)a.tot_out + colbase : nullptr;
-  uint8_t *orow = a.out + s * (int64_t)a.num_outputs;
   double prev = 0.0;
   // graph 0 gives the normaliser, graph i+1 the probability of output i = 1
-  // (one evaluation site, so the graph code is inlined once)
+  // (one evaluation site, so the graph code is inlined once); the decided
+  // bits stay in pv (parameter F + i) until the row is written at the end
   for (int i = 0; i < ng; ++i) {
     const cword *gh = ghs + (size_t)i * GH_DWORDS;
     if (i > 0) set_param_bit<WL>(pv, F + i - 1, 1u);  // trying bit
@@ -410,12 +420,16 @@ __device__ __forceinline__ void run_shot(const ShotArgs &a, int comp, int64_t s,
       prev = p1;
       continue;
     }
-    const double u = urow ? urow[i - 1] : rng_uniform(rs);
+    const double u =
+        a.U ? a.U[s * (int64_t)a.tot_out + colbase + i - 1] : rng_uniform(rs);
     const uint32_t bit = (u < __ddiv_rn(p1, prev)) ? 1u : 0u;
     set_param_bit<WL>(pv, F + i - 1, bit);
-    orow[dest[i - 1]] = (uint8_t)bit;
     prev = bit ? p1 : __dsub_rn(prev, p1);
   }
+  // the output row in one burst of stores, so each cache line of the rows is
+  // completed while it is resident
+  uint8_t *orow = a.out + s * (int64_t)a.num_outputs;
+  for (int i = 1; i < ng; ++i) orow[dest[i - 1]] = (uint8_t)get_param_bit<WL>(pv, F + i - 1);
   if (err) atomicOr(a.err, err);
 }
 

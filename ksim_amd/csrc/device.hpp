@@ -200,6 +200,17 @@ __device__ __forceinline__ Acc4 add_term_digit(Acc4 acc, int4 e, bool zero, uint
   return acc;
 }
 
+// XOR over the lane's parameter words of (word & mask word): its popcount
+// has the parity of <params, mask> (one AND and W32-1 AND-XORs, one popcount)
+template <int W32, int WL>
+__device__ __forceinline__ uint32_t masked_xor(const uint32_t (&pv)[WL],
+                                               const cword *__restrict__ m) {
+  uint32_t t = pv[0] & m[0];
+#pragma unroll
+  for (int w = 1; w < W32; ++w) t ^= pv[w] & m[w];
+  return t;
+}
+
 // -------------------------------------------------------------- one graph
 // Exact amplitude of one graph for this lane's parameter words pv[0..W32):
 // acc = sum_terms value * 2^{-e0}; the caller scales by 2^{GH_E0}.
@@ -223,20 +234,13 @@ __device__ __forceinline__ void eval_graph(const cword *__restrict__ P,
     // PROJ: the term vanishes unless every projector parity matches
     uint32_t dead = 0;
     for (uint32_t k = 0; k < nproj; ++k, rp += W32 + 1) {
-      uint32_t c = rp[W32];
-#pragma unroll
-      for (int w = 0; w < W32; ++w) c += __popc(pv[w] & rp[w]);
-      dead |= c;
+      dead |= rp[W32] + __popc(masked_xor<W32>(pv, rp));
     }
     // PPX: (-1)^{psi phi}
     uint32_t ex = 0;
     for (uint32_t k = 0; k < nppx; ++k, rp += 2 * W32 + 2) {
-      uint32_t cs = rp[2 * W32], cf = rp[2 * W32 + 1];
-#pragma unroll
-      for (int w = 0; w < W32; ++w) {
-        cs += __popc(pv[w] & rp[w]);
-        cf += __popc(pv[w] & rp[W32 + w]);
-      }
+      const uint32_t cs = rp[2 * W32] + __popc(masked_xor<W32>(pv, rp));
+      const uint32_t cf = rp[2 * W32 + 1] + __popc(masked_xor<W32>(pv, rp + W32));
       ex ^= cs & cf;
     }
     S += (int)((ex & 1u) << 2);
@@ -248,18 +252,12 @@ __device__ __forceinline__ void eval_graph(const cword *__restrict__ P,
       // LIN records, unrolled so each scalar load fetches several records
 #pragma unroll 4
       for (uint32_t k = 0; k < nl; ++k, rp += W32 + 1) {
-        uint32_t c = 0;
-#pragma unroll
-        for (int w = 0; w < W32; ++w) c += __popc(pv[w] & rp[w]);
+        const uint32_t c = __popc(masked_xor<W32>(pv, rp));
         x += (0u - (c & 1u)) & rp[W32];
       }
       for (uint32_t k = 0; k < npr; ++k, rp += 2 * W32 + 3) {
-        uint32_t ca = 0, cb = 0;
-#pragma unroll
-        for (int w = 0; w < W32; ++w) {
-          ca += __popc(pv[w] & rp[w]);
-          cb += __popc(pv[w] & rp[W32 + w]);
-        }
+        const uint32_t ca = __popc(masked_xor<W32>(pv, rp));
+        const uint32_t cb = __popc(masked_xor<W32>(pv, rp + W32));
         const uint32_t ma = 0u - (ca & 1u), mb = 0u - (cb & 1u);
         x += (ma & rp[2 * W32]) + (mb & rp[2 * W32 + 1]) + (ma & mb & rp[2 * W32 + 2]);
       }

@@ -167,6 +167,39 @@ int ksim_program_stats(const ksim_program *p, double *stats8);
 
 void ksim_program_destroy(ksim_program *p);
 
+/* ---- wire format: a packed program as one validated blob ---------------- */
+
+/* Program blob layout, version 1 (little-endian): "KSIMPRG\0", u32 version,
+ * u32 header bytes (128), u64 word count, u64 FNV-1a 64 of the words,
+ * 8 f64 stats (ksim_program_stats order), u32 n_comp / num_f / num_outputs /
+ * total outputs / lane words, 12 zero bytes, then the u32 program words.
+ * Replaces the reference's per-process packing of the graph arrays
+ * (src/cpp/ksim/kokkos_sim.cpp:91-151, src/python/ksim/sample/sampler.py:21-43)
+ * with a format that uploads without re-packing.  Each writer stores up to
+ * `cap` bytes to `out` and the full size to *n_bytes (out may be NULL). */
+int ksim_program_serialize(const ksim_program *p, uint8_t *out, int64_t cap,
+                           int64_t *n_bytes);
+
+/* Host-only: pack a FlatProgram (arguments as ksim_program_create) straight
+ * into a blob. */
+int ksim_program_pack_blob(const ksim_graph_view *graphs, const int32_t *graphs_per_comp,
+                           const int32_t *f_selection, const int32_t *F_per_comp,
+                           int32_t n_comp, int32_t num_f, const int32_t *dest_col,
+                           int32_t num_outputs, uint8_t *out, int64_t cap,
+                           int64_t *n_bytes);
+
+/* Validate a blob (magic, version, sizes, checksum, and every offset, count
+ * and record size the kernels follow); stats8 (optional) receives its stats.
+ * Host-only. */
+int ksim_blob_check(const uint8_t *blob, int64_t n, double *stats8);
+
+/* Validate and upload a blob as a program (no packing). */
+int ksim_program_create_from_blob(const uint8_t *blob, int64_t n, ksim_program **out);
+
+/* [n_comp, num_f, num_outputs, total outputs, lane words, program words,
+ * device, 0] of a program. */
+int ksim_program_info(const ksim_program *p, int64_t *info8);
+
 /* ---- batched persistent path ------------------------------------------ */
 
 typedef struct ksim_batch ksim_batch;

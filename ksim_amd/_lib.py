@@ -74,6 +74,14 @@ EXPORTS = {
                                     i32p, C.c_int32, C.POINTER(C.c_uint32), C.c_int64, i64p,
                                     C.POINTER(C.c_double)]),
     "ksim_tables": (C.c_int, [i32p, C.c_int64, i32p, C.c_int64, i32p]),
+    "ksim_program_serialize": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint8), C.c_int64, i64p]),
+    "ksim_program_pack_blob": (C.c_int, [C.POINTER(GraphView), i32p, i32p, i32p, C.c_int32,
+                                         C.c_int32, i32p, C.c_int32, C.POINTER(C.c_uint8),
+                                         C.c_int64, i64p]),
+    "ksim_blob_check": (C.c_int, [C.POINTER(C.c_uint8), C.c_int64, C.POINTER(C.c_double)]),
+    "ksim_program_create_from_blob": (C.c_int, [C.POINTER(C.c_uint8), C.c_int64,
+                                                C.POINTER(C.c_void_p)]),
+    "ksim_program_info": (C.c_int, [C.c_void_p, i64p]),
     "ksim_last_error": (C.c_char_p, []),
     "ksim_device_info": (C.c_int, [i64p]),
 }

@@ -19,6 +19,7 @@
 #include "device.hpp"
 #include "layout.hpp"
 #include "pack.hpp"
+#include "wire.hpp"
 
 using namespace ksim;
 
@@ -308,14 +309,11 @@ void launch_sample(ksim_program *p, const uint8_t *d_f, int64_t B, int32_t f_col
   hip_check(hipGetLastError(), "k_sample launch");
 }
 
-ksim_program *create_program(const ksim_graph_view *graphs, const int32_t *graphs_per_comp,
-                             const int32_t *f_selection, const int32_t *F_per_comp,
-                             int32_t n_comp, int32_t num_f, const int32_t *dest_col,
-                             int32_t num_outputs) {
-  if (!graphs && n_comp > 0) throw std::invalid_argument("graphs is NULL");
-  PackStats st;
-  std::vector<uint32_t> words = pack_program(graphs, graphs_per_comp, f_selection, F_per_comp,
-                                             n_comp, num_f, dest_col, num_outputs, st);
+// upload packed (and validated) words as a program object
+ksim_program *upload_program(const std::vector<uint32_t> &words, const PackStats &st) {
+  const int n_comp = (int)words[PH_NCOMP];
+  const int num_f = (int)words[PH_NUMF];
+  const int num_outputs = (int)words[PH_NUMOUT];
   auto *p = new ksim_program();
   try {
     hip_check(hipGetDevice(&p->device), "hipGetDevice");
@@ -329,8 +327,8 @@ ksim_program *create_program(const ksim_graph_view *graphs, const int32_t *graph
     p->num_f = num_f;
     p->num_outputs = num_outputs;
     p->tot_out = (int)words[PH_TOTOUT];
-    p->wmax = st.wmax;
-    p->wl = lane_words(st.wmax);
+    p->wmax = (int)words[PH_WMAX];
+    p->wl = lane_words(p->wmax);
     p->st = st;
     for (int c = 0; c < n_comp; ++c) {
       const uint32_t *ch = &words[words[PH_DWORDS + c]];
@@ -347,6 +345,37 @@ ksim_program *create_program(const ksim_graph_view *graphs, const int32_t *graph
     throw;
   }
   return p;
+}
+
+ksim_program *create_program(const ksim_graph_view *graphs, const int32_t *graphs_per_comp,
+                             const int32_t *f_selection, const int32_t *F_per_comp,
+                             int32_t n_comp, int32_t num_f, const int32_t *dest_col,
+                             int32_t num_outputs) {
+  if (!graphs && n_comp > 0) throw std::invalid_argument("graphs is NULL");
+  PackStats st;
+  std::vector<uint32_t> words = pack_program(graphs, graphs_per_comp, f_selection, F_per_comp,
+                                             n_comp, num_f, dest_col, num_outputs, st);
+  return upload_program(words, st);
+}
+
+void stats_to8(const PackStats &st, size_t n_words, double *s8) {
+  const double v[8] = {(double)st.graphs, (double)st.terms, (double)st.records, (double)n_words,
+                       st.ops, (double)st.n2_bound, (double)st.wide_terms, (double)st.sum_bits};
+  std::memcpy(s8, v, sizeof(v));
+}
+
+PackStats stats_from8(const double *s8, int wmax) {
+  PackStats st;
+  st.graphs = (size_t)s8[0]; st.terms = (size_t)s8[1]; st.records = (size_t)s8[2];
+  st.ops = s8[4]; st.n2_bound = (int)s8[5]; st.wide_terms = (size_t)s8[6];
+  st.sum_bits = (int)s8[7]; st.wmax = wmax;
+  return st;
+}
+
+void copy_out(const std::vector<uint8_t> &b, uint8_t *out, int64_t cap, int64_t *n_bytes) {
+  if (!n_bytes) throw std::invalid_argument("n_bytes is NULL");
+  *n_bytes = (int64_t)b.size();
+  if (out && cap > 0) std::memcpy(out, b.data(), std::min<size_t>(b.size(), (size_t)cap));
 }
 
 }  // namespace
@@ -591,6 +620,58 @@ int ksim_program_pack(const ksim_graph_view *graphs, const int32_t *graphs_per_c
     }
     if (words && cap > 0)
       std::memcpy(words, w.data(), sizeof(uint32_t) * std::min<size_t>(w.size(), (size_t)cap));
+  });
+}
+
+int ksim_program_pack_blob(const ksim_graph_view *graphs, const int32_t *graphs_per_comp,
+                           const int32_t *f_selection, const int32_t *F_per_comp,
+                           int32_t n_comp, int32_t num_f, const int32_t *dest_col,
+                           int32_t num_outputs, uint8_t *out, int64_t cap, int64_t *n_bytes) {
+  return guarded([&] {
+    PackStats st;
+    std::vector<uint32_t> w = pack_program(graphs, graphs_per_comp, f_selection, F_per_comp,
+                                           n_comp, num_f, dest_col, num_outputs, st);
+    double s8[8];
+    stats_to8(st, w.size(), s8);
+    copy_out(blob_from_words(w, s8), out, cap, n_bytes);
+  });
+}
+
+int ksim_blob_check(const uint8_t *blob, int64_t n, double *stats8) {
+  return guarded([&] {
+    if (n < 0) throw std::invalid_argument("negative blob size");
+    (void)words_from_blob(blob, (size_t)n, stats8);
+  });
+}
+
+int ksim_program_serialize(const ksim_program *p, uint8_t *out, int64_t cap, int64_t *n_bytes) {
+  return guarded([&] {
+    if (!p) throw std::invalid_argument("program is NULL");
+    std::vector<uint32_t> w(p->n_words);
+    hip_check(hipMemcpy(w.data(), p->d_prog, p->n_words * 4, hipMemcpyDeviceToHost),
+              "download program");
+    double s8[8];
+    stats_to8(p->st, w.size(), s8);
+    copy_out(blob_from_words(w, s8), out, cap, n_bytes);
+  });
+}
+
+int ksim_program_create_from_blob(const uint8_t *blob, int64_t n, ksim_program **out) {
+  return guarded([&] {
+    if (!out) throw std::invalid_argument("out is NULL");
+    if (n < 0) throw std::invalid_argument("negative blob size");
+    double s8[8];
+    std::vector<uint32_t> w = words_from_blob(blob, (size_t)n, s8);
+    *out = upload_program(w, stats_from8(s8, (int)w[PH_WMAX]));
+  });
+}
+
+int ksim_program_info(const ksim_program *p, int64_t *info8) {
+  return guarded([&] {
+    if (!p || !info8) throw std::invalid_argument("NULL argument");
+    const int64_t v[8] = {p->n_comp, p->num_f, p->num_outputs, p->tot_out,
+                          p->wmax, (int64_t)p->n_words, p->device, 0};
+    std::memcpy(info8, v, sizeof(v));
   });
 }
 

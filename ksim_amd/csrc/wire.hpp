@@ -1,0 +1,36 @@
+// wire.hpp — the on-disk / on-wire form of a packed program.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "pack.hpp"
+
+namespace ksim {
+
+// Blob layout (little-endian), version 1:
+//   [0, 8)     magic "KSIMPRG\0"
+//   [8, 12)    u32 version
+//   [12, 16)   u32 header bytes (128)
+//   [16, 24)   u64 n_words
+//   [24, 32)   u64 FNV-1a 64 of the program words (as bytes)
+//   [32, 96)   f64 x 8: the packer's stats (ksim_program_stats order)
+//   [96, 116)  u32 x 5: n_comp, num_f, num_outputs, total outputs, lane words
+//   [116, 128) zero
+//   [128, ...) u32 program words
+constexpr uint32_t WIRE_VERSION = 1;
+constexpr size_t WIRE_HEADER = 128;
+
+uint64_t fnv1a64(const void *data, size_t n);
+
+std::vector<uint8_t> blob_from_words(const std::vector<uint32_t> &words, const double *stats8);
+
+// Parse and fully validate a blob: every offset, count and size a kernel
+// follows stays inside the words, so a corrupt or hostile blob is rejected
+// here (std::invalid_argument) instead of reaching the device.
+std::vector<uint32_t> words_from_blob(const uint8_t *blob, size_t n, double *stats8);
+
+// The structural check alone (also run on freshly packed words in debug use).
+void validate_words(const uint32_t *w, size_t n);
+
+}  // namespace ksim

@@ -230,6 +230,27 @@ def pack_words(prog: FlatProgram, stats: dict | None = None) -> np.ndarray:
     return w[:n.value]
 
 
+def program_blob(prog: FlatProgram) -> bytes:
+    """Pack ``prog`` into the wire format on the host (no device needed)."""
+    lib = _lib.load()
+    pa = ProgramArgs(prog)
+    n = C.c_int64()
+    _lib.check(lib.ksim_program_pack_blob(*pa.args()[:8], None, 0, C.byref(n)))
+    out = np.zeros(n.value, np.uint8)
+    _lib.check(lib.ksim_program_pack_blob(*pa.args()[:8], _lib.ptr(out, C.c_uint8), out.size,
+                                          C.byref(n)))
+    return out.tobytes()
+
+
+def blob_stats(blob: bytes) -> dict:
+    """Validate a program blob and return its stats (ValueError if rejected)."""
+    buf = np.frombuffer(blob, dtype=np.uint8)
+    st = np.zeros(8, np.float64)
+    _lib.check(_lib.load().ksim_blob_check(_lib.ptr(buf, C.c_uint8) if buf.size else None,
+                                           buf.size, _lib.ptr(st, C.c_double)))
+    return _stats_dict(st)
+
+
 def lookup_tables():
     """Materialisation tables of the kernels.
 
@@ -257,21 +278,40 @@ class HipProgramSampler:
 
     def __init__(self, prog: FlatProgram):
         lib = _lib.load()
-        self.prog = prog
         pa = ProgramArgs(prog)
-        self.n_comp = pa.n_comp
-        self.num_outputs = pa.num_outputs
-        self.min_num_f = pa.num_f
-        self.tot_out = pa.tot_out
-        views, gpc_a, fsel_a, Fpc_a, num_f, dest_a = (pa.views, pa.gpc, pa.fsel, pa.Fpc,
-                                                      pa.num_f, pa.dest)
         h = C.c_void_p()
-        _lib.check(lib.ksim_program_create(
-            views, _lib.ptr(gpc_a, C.c_int32), _lib.ptr(fsel_a, C.c_int32),
-            _lib.ptr(Fpc_a, C.c_int32), self.n_comp, num_f, _lib.ptr(dest_a, C.c_int32),
-            self.num_outputs, C.byref(h)))
+        _lib.check(lib.ksim_program_create(*pa.args()[:8], C.byref(h)))
+        self._adopt(h, prog)
+
+    def _adopt(self, h, prog):
+        self.prog = prog
         self._native = _Native(h)
+        info = np.zeros(8, np.int64)
+        _lib.check(_lib.load().ksim_program_info(h, _lib.ptr(info, C.c_int64)))
+        self.n_comp, self.min_num_f, self.num_outputs, self.tot_out = (int(x) for x in info[:4])
         self.ops_per_shot = self.stats()["ops_per_shot"]
+
+    @classmethod
+    def from_bytes(cls, blob: bytes) -> "HipProgramSampler":
+        """Upload a program blob (``to_bytes`` / :func:`program_blob`)
+        without re-packing; the blob is validated first.  ``prog`` is None."""
+        buf = np.frombuffer(blob, dtype=np.uint8)
+        h = C.c_void_p()
+        _lib.check(_lib.load().ksim_program_create_from_blob(
+            _lib.ptr(buf, C.c_uint8) if buf.size else None, buf.size, C.byref(h)))
+        self = cls.__new__(cls)
+        self._adopt(h, None)
+        return self
+
+    def to_bytes(self) -> bytes:
+        """The program as a versioned, checksummed blob."""
+        lib = _lib.load()
+        n = C.c_int64()
+        _lib.check(lib.ksim_program_serialize(self._native._h, None, 0, C.byref(n)))
+        out = np.zeros(n.value, np.uint8)
+        _lib.check(lib.ksim_program_serialize(self._native._h, _lib.ptr(out, C.c_uint8),
+                                              out.size, C.byref(n)))
+        return out.tobytes()
 
     def _prep_f(self, f_params) -> np.ndarray:
         f = np.ascontiguousarray(f_params, dtype=np.uint8)

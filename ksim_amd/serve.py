@@ -23,6 +23,7 @@ from __future__ import annotations
 
 import ctypes as C
 import dataclasses
+import hashlib
 import heapq
 import math
 import time
@@ -31,14 +32,16 @@ from collections import OrderedDict, deque
 import numpy as np
 
 from . import _lib
-from .sampler import HipProgramSampler
+from .sampler import HipProgramSampler, program_blob
 
 
 @dataclasses.dataclass
 class Request:
-    """One sampling request: ``prog`` (a FlatProgram, or a sampler already
-    built for it) under ``key``, its f matrix (shots, f_cols) uint8, a seed
-    and an arrival time in seconds from the start of the run."""
+    """One sampling request: ``prog`` (a FlatProgram, a program blob, or a
+    sampler already built for it) under ``key``, its f matrix (shots, f_cols)
+    uint8, a seed and an arrival time in seconds from the start of the run.
+    With ``key=None`` the program is keyed by its content (the hash of its
+    packed blob), so identical programs stay resident once."""
     key: object
     prog: object
     f: np.ndarray
@@ -112,12 +115,34 @@ class ServingLoop:
         self.rate = float(rate)
 
     # ------------------------------------------------------------ programs
+    @staticmethod
+    def content_key(blob: bytes) -> str:
+        """Residency key of a program blob: programs with the same packed
+        words share one resident copy, whoever submits them."""
+        return "blob:" + hashlib.sha256(blob).hexdigest()[:32]
+
+    def _resolve(self, r):
+        """Give a request without a key its content key (packing a
+        FlatProgram into a blob on the host once)."""
+        if r.key is None:
+            if not isinstance(r.prog, (bytes, bytearray, HipProgramSampler)):
+                r.prog = program_blob(r.prog)
+            if isinstance(r.prog, HipProgramSampler):
+                r.prog = r.prog.to_bytes()
+            r.key = self.content_key(bytes(r.prog))
+        return r.key
+
     def _program(self, key, prog):
         ent = self.programs.get(key)
         if ent is not None:
             self.programs.move_to_end(key)
             return ent
-        sampler = prog if isinstance(prog, HipProgramSampler) else HipProgramSampler(prog)
+        if isinstance(prog, HipProgramSampler):
+            sampler = prog
+        elif isinstance(prog, (bytes, bytearray)):
+            sampler = HipProgramSampler.from_bytes(bytes(prog))
+        else:
+            sampler = HipProgramSampler(prog)
         nbytes = int(self.lib.ksim_program_device_bytes(sampler.handle))
         ent = [sampler, nbytes, 0]
         self.programs[key] = ent
@@ -190,7 +215,7 @@ class ServingLoop:
                 while pend_i < len(pending) and (not realtime or pending[pend_i].arrival <= t) \
                         and free_slots:
                     r = pending[pend_i]
-                    ent = self.programs.get(r.key)
+                    ent = self.programs.get(self._resolve(r))
                     need = r.shots * (r.f.shape[1] + 64)
                     if ent is None:
                         need += 0  # program bytes are only known after packing

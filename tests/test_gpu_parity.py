@@ -195,3 +195,19 @@ def test_shot_range_split_is_identical():
     torch.cuda.synchronize()
     assert torch.equal(full, parts)
     assert torch.equal(full.cpu(), torch.from_numpy(s.sample(f.cpu().numpy(), 99)))
+
+
+def test_program_blob_round_trip():
+    from ksim_amd.sampler import program_blob
+    prog = synth.shape_program(n_q=7, t=6, F=9, n_out=6, seed=11, n_components=2)
+    s = HipProgramSampler(prog)
+    blob = s.to_bytes()
+    assert blob == program_blob(prog)
+    t = HipProgramSampler.from_bytes(blob)
+    assert (t.n_comp, t.num_outputs, t.tot_out, t.min_num_f) == \
+        (s.n_comp, s.num_outputs, s.tot_out, s.min_num_f)
+    assert t.ops_per_shot == s.ops_per_shot
+    f = synth.random_f(777, 9, 0.5, 4)
+    assert np.array_equal(t.sample(f, 21), s.sample(f, 21))
+    with pytest.raises(ValueError):
+        HipProgramSampler.from_bytes(blob[:-4])

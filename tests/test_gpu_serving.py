@@ -49,3 +49,24 @@ def test_poisson_stream_latencies():
     assert stats["p50_ms"] is not None and stats["p99_ms"] >= stats["p50_ms"] >= 0
     assert all(r.done >= r.arrival for r in reqs)
     loop.close()
+
+
+def test_content_keyed_residency():
+    # requests without a key: FlatPrograms, blobs and samplers of the same
+    # program share one resident copy, and results match a plain launch
+    from ksim_amd.sampler import program_blob
+    prog_a = synth.shape_program(n_q=5, t=4, F=3, n_out=5, seed=1)
+    prog_b = synth.shape_program(n_q=6, t=2, F=4, n_out=3, seed=2)
+    sa, sb = HipProgramSampler(prog_a), HipProgramSampler(prog_b)
+    forms = [prog_a, program_blob(prog_a), sa, prog_b, program_blob(prog_b)]
+    reqs = []
+    for i, p in enumerate(forms * 2):
+        f = synth.random_f(200 + 37 * i, 4, 0.3, 70 + i)
+        reqs.append(Request(key=None, prog=p, f=f, seed=40 + i))
+    loop = ServingLoop(max_slots=4, step_ms=0.2, hbm_budget=1 << 30)
+    loop.run(reqs, realtime=False)
+    assert len(loop.programs) == 2
+    for r in reqs:
+        ref = sa if r.key == loop.content_key(program_blob(prog_a)) else sb
+        assert np.array_equal(r.out, ref.sample(r.f, r.seed))
+    loop.close()

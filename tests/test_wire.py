@@ -1,0 +1,121 @@
+"""Host-only: the program wire format.  A blob holds exactly the packed
+words, round-trips through validation, and every structural corruption a
+kernel could follow out of bounds is rejected before upload."""
+import struct
+
+import numpy as np
+import pytest
+
+from ksim_amd import synth
+from ksim_amd.sampler import blob_stats, pack_words, program_blob
+
+HDR = 128
+
+
+def fnv1a64(b: bytes) -> int:
+    h = 0xcbf29ce484222325
+    for x in b:
+        h ^= x
+        h = (h * 0x100000001b3) & 0xFFFFFFFFFFFFFFFF
+    return h
+
+
+def words_of(blob: bytes) -> np.ndarray:
+    return np.frombuffer(blob[HDR:], dtype=np.uint32).copy()
+
+
+def with_words(blob: bytes, w: np.ndarray) -> bytes:
+    """Replace the words and re-seal the checksum, so only the structural
+    validation can catch what was changed."""
+    body = w.astype(np.uint32).tobytes()
+    head = bytearray(blob[:HDR])
+    head[16:24] = struct.pack("<Q", len(w))
+    head[24:32] = struct.pack("<Q", fnv1a64(body))
+    return bytes(head) + body
+
+
+@pytest.fixture(scope="module")
+def prog():
+    return synth.shape_program(n_q=6, t=6, F=8, n_out=4, seed=3, n_components=2)
+
+
+@pytest.fixture(scope="module")
+def blob(prog):
+    return program_blob(prog)
+
+
+def test_blob_holds_the_packed_words(prog, blob):
+    st = {}
+    w = pack_words(prog, st)
+    assert blob[:8] == b"KSIMPRG\0"
+    assert struct.unpack("<II", blob[8:16]) == (1, HDR)
+    assert np.array_equal(words_of(blob), w)
+    assert struct.unpack("<Q", blob[24:32])[0] == fnv1a64(w.tobytes())
+    got = blob_stats(blob)
+    for k in ("graphs", "terms", "records", "words", "ops_per_shot", "sum_bits"):
+        assert got[k] == st[k], k
+
+
+def test_blob_of_every_golden_program(golden_files):
+    from tests.golden.golden_io import load_program
+    for fn in golden_files:
+        b = program_blob(load_program(np.load(fn)))
+        assert blob_stats(b)["words"] == (len(b) - HDR) // 4
+
+
+@pytest.mark.parametrize("case", ["magic", "version", "truncated", "extra", "checksum", "header"])
+def test_rejects_damaged_envelope(blob, case):
+    b = bytearray(blob)
+    if case == "magic":
+        b[0] ^= 1
+    elif case == "version":
+        b[8] = 2
+    elif case == "truncated":
+        b = b[:-4]
+    elif case == "extra":
+        b += b"\0\0\0\0"
+    elif case == "checksum":
+        b[HDR + 40] ^= 0x10
+    elif case == "header":
+        b[96] ^= 1  # n_comp field disagrees with the words
+    with pytest.raises(ValueError):
+        blob_stats(bytes(b))
+
+
+def structural_cases(w):
+    """(name, mutated words) pairs, each a corruption the validator must catch."""
+    PH, CH, GH, TH = 8, 16, 8, 16
+    c0 = int(w[PH])                       # component 0 header offset
+    fsel, dest, graphs = int(w[c0 + 5]), int(w[c0 + 6]), int(w[c0 + 7])
+    g1 = graphs + GH                      # graph 1 header
+    t0 = int(w[g1 + 3])                   # its first term
+    out = []
+
+    def mut(name, idx, val):
+        v = w.copy()
+        v[idx] = val
+        out.append((name, v))
+
+    mut("lane words", 4, 9)
+    mut("component offset", PH, len(w) + 5)
+    mut("graph count", c0 + 0, 0)
+    mut("f_selection", fsel, int(w[1]) + 3)
+    mut("output column", dest, int(w[2]))
+    mut("graph mask words", g1 + 1, 9)
+    mut("term offset", g1 + 3, len(w) - 4)
+    mut("term count", g1 + 0, int(w[g1 + 0]) + 1000)
+    mut("term size", t0 + 14, int(w[t0 + 14]) + 1)
+    mut("term flags", t0 + 3, 8)
+    mut("chunk count", t0 + 2, int(w[t0 + 2]) + 1000)
+    return out
+
+
+def test_rejects_structural_corruption(blob):
+    w = words_of(blob)
+    cases = structural_cases(w)
+    assert len(cases) == 11
+    for name, v in cases:
+        with pytest.raises(ValueError):
+            blob_stats(with_words(blob, v))
+        # the unchanged words still pass (the re-sealing itself is sound)
+    blob_stats(with_words(blob, w))

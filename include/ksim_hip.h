@@ -146,6 +146,19 @@ int ksim_program_sample_device(ksim_program *p, const uint8_t *d_f, int64_t B,
                                int32_t f_cols, int64_t shot_base, uint64_t seed,
                                const double *d_U, uint8_t *d_out, void *stream);
 
+/* Packed output epilogue: rows of ceil(num_outputs / 32) little-endian u32
+ * words, bit c of a row = output column c (after the output_order
+ * permutation), i.e. numpy's packbits(bitorder="little") of the u8 row padded
+ * to whole words — 8x less D2H than the u8 rows of sampler.py:65-67.  Same
+ * shots and stream as ksim_program_sample / _sample_device.  The device
+ * variant zeroes d_out itself (on `stream`) when rows are assembled from
+ * several components. */
+int ksim_program_sample_packed(ksim_program *p, const uint8_t *f_params, int64_t B,
+                               int32_t f_cols, uint64_t seed, uint32_t *out);
+int ksim_program_sample_device_packed(ksim_program *p, const uint8_t *d_f, int64_t B,
+                                      int32_t f_cols, int64_t shot_base, uint64_t seed,
+                                      const double *d_U, uint32_t *d_out, void *stream);
+
 /* Launch shape of the plain kernel: threads per block and lanes per shot
  * (1 = each lane evaluates every term of its own shot).  Only (256, 1) is
  * implemented; other shapes return KSIM_E_INVALID. */
@@ -197,7 +210,7 @@ int ksim_blob_check(const uint8_t *blob, int64_t n, double *stats8);
 int ksim_program_create_from_blob(const uint8_t *blob, int64_t n, ksim_program **out);
 
 /* [n_comp, num_f, num_outputs, total outputs, lane words, program words,
- * device, 0] of a program. */
+ * device, packed output row words] of a program. */
 int ksim_program_info(const ksim_program *p, int64_t *info8);
 
 /* ---- batched persistent path ------------------------------------------ */

@@ -82,6 +82,11 @@ EXPORTS = {
     "ksim_program_create_from_blob": (C.c_int, [C.POINTER(C.c_uint8), C.c_int64,
                                                 C.POINTER(C.c_void_p)]),
     "ksim_program_info": (C.c_int, [C.c_void_p, i64p]),
+    "ksim_program_sample_packed": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint8), C.c_int64,
+                                             C.c_int32, C.c_uint64, C.POINTER(C.c_uint32)]),
+    "ksim_program_sample_device_packed": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64,
+                                                    C.c_int32, C.c_int64, C.c_uint64,
+                                                    C.c_void_p, C.c_void_p, C.c_void_p]),
     "ksim_last_error": (C.c_char_p, []),
     "ksim_device_info": (C.c_int, [i64p]),
 }

@@ -221,6 +221,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KSIM_WAV
     a.prog = bp.prog; a.f = bp.f; a.U = nullptr; a.out = bp.out; a.err = err;
     a.B = bp.B; a.shot_base = 0; a.seed = bp.seed;
     a.num_f = bp.num_f; a.num_outputs = bp.num_outputs; a.tot_out = bp.tot_out; a.pad = 0;
+    a.out_words = 0; a.out_atomic = 0;
     for (int sub = 0; sub < it.count; sub += 64) {
       const int64_t s = it.shot0 + sub + lane;
       if (sub + lane < it.count && s < bp.B) run_shot<WL>(a, it.comp, s, &lt, gtab);
@@ -241,6 +242,8 @@ struct ksim_program {
   std::vector<std::vector<uint32_t>> graph_hdr;  // host copy of graph headers per comp
   PackStats st;
   int block_threads = kBlock;
+  int out_words = 0;              // packed output row words: ceil(num_outputs / 32)
+  bool packed_full_rows = false;  // one component writes whole packed rows in order
   int term_split = 1;
   // staging for the host-pointer API
   uint8_t *d_f = nullptr;
@@ -289,7 +292,7 @@ void check_err_sync(ksim_program *p, hipStream_t st) {
 
 void launch_sample(ksim_program *p, const uint8_t *d_f, int64_t B, int32_t f_cols,
                    int64_t shot_base, uint64_t seed, const double *d_U, uint8_t *d_out,
-                   hipStream_t st) {
+                   hipStream_t st, int out_words = 0) {
   if (f_cols < p->num_f) throw std::invalid_argument("f_params has fewer columns than f_selection needs");
   if (B <= 0 || p->n_comp == 0) return;
   DevTables t = tables_for_current_device();
@@ -297,6 +300,15 @@ void launch_sample(ksim_program *p, const uint8_t *d_f, int64_t B, int32_t f_col
   a.prog = p->d_prog; a.f = d_f; a.U = d_U; a.out = d_out; a.err = p->d_err;
   a.B = B; a.shot_base = shot_base; a.seed = seed;
   a.num_f = f_cols; a.num_outputs = p->num_outputs; a.tot_out = p->tot_out; a.pad = 0;
+  a.out_words = out_words;
+  a.out_atomic = p->n_comp > 1 ? 1 : 0;
+  if (out_words > 0 && !p->packed_full_rows) {
+    // rows assembled from several components or scattered bits: OR-ed into
+    // zeroed words (a fill kernel, so the launch stays graph-capturable)
+    const int64_t n = B * (int64_t)out_words;
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                       (uint32_t *)d_out, 0u, n);
+  }
   const int64_t nb = (B + kBlock - 1) / kBlock;
   if (nb > 0x7fffffff) throw std::invalid_argument("too many shots for one launch");
   dim3 grid((unsigned)nb, (unsigned)p->n_comp);
@@ -329,6 +341,11 @@ ksim_program *upload_program(const std::vector<uint32_t> &words, const PackStats
     p->tot_out = (int)words[PH_TOTOUT];
     p->wmax = (int)words[PH_WMAX];
     p->wl = lane_words(p->wmax);
+    p->out_words = (num_outputs + 31) / 32;
+    if (n_comp == 1) {
+      const uint32_t *c0 = &words[words[PH_DWORDS]];
+      p->packed_full_rows = c0[CH_DEST0] == 0 && (int)c0[CH_NOUT] == num_outputs;
+    }
     p->st = st;
     for (int c = 0; c < n_comp; ++c) {
       const uint32_t *ch = &words[words[PH_DWORDS + c]];
@@ -430,6 +447,20 @@ int ksim_program_sample_device(ksim_program *p, const uint8_t *d_f, int64_t B, i
   });
 }
 
+int ksim_program_sample_device_packed(ksim_program *p, const uint8_t *d_f, int64_t B,
+                                      int32_t f_cols, int64_t shot_base, uint64_t seed,
+                                      const double *d_U, uint32_t *d_out, void *stream) {
+  return guarded([&] {
+    if (!p) throw std::invalid_argument("program is NULL");
+    if (B < 0) throw std::invalid_argument("B < 0");
+    if (B > 0 && ((!d_f && f_cols > 0) || !d_out)) throw std::invalid_argument("NULL buffer");
+    if (p->out_words == 0) return;
+    hipStream_t st = (hipStream_t)stream;
+    reset_err_async(p, st);
+    launch_sample(p, d_f, B, f_cols, shot_base, seed, d_U, (uint8_t *)d_out, st, p->out_words);
+  });
+}
+
 int ksim_program_check(ksim_program *p, void *stream) {
   return guarded([&] {
     if (!p) throw std::invalid_argument("program is NULL");
@@ -438,14 +469,15 @@ int ksim_program_check(ksim_program *p, void *stream) {
 }
 
 static void sample_host(ksim_program *p, const uint8_t *f, int64_t B, int32_t f_cols,
-                        uint64_t seed, const double *U, uint8_t *out, int out_cols) {
+                        uint64_t seed, const double *U, uint8_t *out, int out_cols,
+                        bool packed = false) {
   if (B < 0) throw std::invalid_argument("B < 0");
   if (f_cols < p->num_f) throw std::invalid_argument("f_params has fewer columns than f_selection needs");
   if (B == 0) return;
   if (!out || (!f && f_cols > 0)) throw std::invalid_argument("NULL buffer");
   hipStream_t st = nullptr;
   const size_t fb = (size_t)B * f_cols;
-  const size_t ob = (size_t)B * out_cols;
+  const size_t ob = packed ? (size_t)B * p->out_words * 4 : (size_t)B * out_cols;
   ensure(p->d_f, p->f_cap, fb);
   ensure(p->d_out, p->out_cap, ob);
   if (fb) hip_check(hipMemcpyAsync(p->d_f, f, fb, hipMemcpyHostToDevice, st), "H2D f");
@@ -456,12 +488,21 @@ static void sample_host(ksim_program *p, const uint8_t *f, int64_t B, int32_t f_
     if (ub) hip_check(hipMemcpyAsync(p->d_U, U, ub, hipMemcpyHostToDevice, st), "H2D U");
     dU = p->d_U;
   }
-  if (out_cols != p->num_outputs || p->tot_out < p->num_outputs)
+  if (!packed && (out_cols != p->num_outputs || p->tot_out < p->num_outputs))
     fill_u8_async(p->d_out, 0, (int64_t)ob, st);
   reset_err_async(p, st);
-  launch_sample(p, p->d_f, B, f_cols, 0, seed, dU, p->d_out, st);
+  launch_sample(p, p->d_f, B, f_cols, 0, seed, dU, p->d_out, st, packed ? p->out_words : 0);
   hip_check(hipMemcpyAsync(out, p->d_out, ob, hipMemcpyDeviceToHost, st), "D2H out");
   check_err_sync(p, st);
+}
+
+int ksim_program_sample_packed(ksim_program *p, const uint8_t *f_params, int64_t B,
+                               int32_t f_cols, uint64_t seed, uint32_t *out) {
+  return guarded([&] {
+    if (!p) throw std::invalid_argument("program is NULL");
+    if (p->out_words == 0) return;
+    sample_host(p, f_params, B, f_cols, seed, nullptr, (uint8_t *)out, 0, true);
+  });
 }
 
 int ksim_program_sample(ksim_program *p, const uint8_t *f_params, int64_t B, int32_t f_cols,
@@ -670,7 +711,7 @@ int ksim_program_info(const ksim_program *p, int64_t *info8) {
   return guarded([&] {
     if (!p || !info8) throw std::invalid_argument("NULL argument");
     const int64_t v[8] = {p->n_comp, p->num_f, p->num_outputs, p->tot_out,
-                          p->wmax, (int64_t)p->n_words, p->device, 0};
+                          p->wmax, (int64_t)p->n_words, p->device, p->out_words};
     std::memcpy(info8, v, sizeof(v));
   });
 }

@@ -352,12 +352,64 @@ struct ShotArgs {
   const uint32_t *prog;
   const uint8_t *f;   // (B, num_f)
   const double *U;    // (B, tot_out) or null
-  uint8_t *out;       // (B, num_outputs)
+  uint8_t *out;       // (B, num_outputs) u8, or (B, out_words) u32 bit rows
   uint32_t *err;
   int64_t B, shot_base;
   uint64_t seed;
-  int32_t num_f, num_outputs, tot_out, pad;
+  int32_t num_f, num_outputs, tot_out;
+  int32_t out_words;  // 0: one byte per output; > 0: packed rows of this many words
+  int32_t out_atomic; // packed rows shared by several components: OR words in
+  int32_t pad;
 };
+
+// 32 bits of the lane's parameter words starting at bit `off` (wave-uniform,
+// may be negative or run past the words: missing bits read as 0)
+template <int WL>
+__device__ __forceinline__ uint32_t param_bits32(const uint32_t (&pv)[WL], int off) {
+  const int w0 = off >> 5, r = off & 31;
+  uint32_t lo = 0, hi = 0;
+#pragma unroll
+  for (int w = 0; w < WL; ++w) {
+    lo = (w == w0) ? pv[w] : lo;
+    hi = (w == w0 + 1) ? pv[w] : hi;
+  }
+  return r ? (lo >> r) | (hi << (32 - r)) : lo;
+}
+
+// write the shot's output row as packed bits: bit c of the row = output
+// column c (little-endian words, so the byte view is numpy's
+// packbits(bitorder="little") of the u8 row, padded to whole words)
+template <int WL>
+__device__ __forceinline__ void write_packed_row(const ShotArgs &a, const uint32_t (&pv)[WL],
+                                                 const cword *__restrict__ ch,
+                                                 const cword *__restrict__ dest, int F,
+                                                 int nout, int64_t s) {
+  uint32_t *orow = (uint32_t *)a.out + s * (int64_t)a.out_words;
+  const uint32_t dest0 = ch[CH_DEST0];
+  if (dest0 != 0xffffffffu) {
+    // contiguous columns [dest0, dest0 + nout): row word k holds parameter
+    // bits F - dest0 + 32 k .. + 31, masked to the component's columns
+    const int k0 = (int)dest0 >> 5, k1 = ((int)dest0 + nout - 1) >> 5;
+    for (int k = k0; k <= k1; ++k) {
+      uint32_t v = param_bits32<WL>(pv, F - (int)dest0 + 32 * k);
+      const int lo = max((int)dest0 - 32 * k, 0), hi = min((int)dest0 + nout - 32 * k, 32);
+      const uint32_t m = (hi >= 32 ? 0xffffffffu : ((1u << hi) - 1u)) & ~((1u << lo) - 1u);
+      v &= m;
+      if (a.out_atomic) {
+        if (v) atomicOr(orow + k, v);
+      } else {
+        orow[k] = v;
+      }
+    }
+    return;
+  }
+  // any column order: one bit per output, OR-ed into its word
+  for (int i = 0; i < nout; ++i) {
+    const uint32_t c = dest[i];
+    const uint32_t bit = param_bits32<WL>(pv, F + i) & 1u;
+    if (bit) atomicOr(orow + (c >> 5), 1u << (c & 31));
+  }
+}
 
 // The whole autoregressive loop of one shot of component `comp`
 // (src/cpp/ksim/kokkos_sim.cpp:214-230, src/python/ksim/sample/evaluate.py:76-85).
@@ -426,8 +478,12 @@ __device__ __forceinline__ void run_shot(const ShotArgs &a, int comp, int64_t s,
   }
   // the output row in one burst of stores, so each cache line of the rows is
   // completed while it is resident
-  uint8_t *orow = a.out + s * (int64_t)a.num_outputs;
-  for (int i = 1; i < ng; ++i) orow[dest[i - 1]] = (uint8_t)get_param_bit<WL>(pv, F + i - 1);
+  if (a.out_words > 0) {
+    write_packed_row<WL>(a, pv, ch, dest, F, ng - 1, s);
+  } else {
+    uint8_t *orow = a.out + s * (int64_t)a.num_outputs;
+    for (int i = 1; i < ng; ++i) orow[dest[i - 1]] = (uint8_t)get_param_bit<WL>(pv, F + i - 1);
+  }
   if (err) atomicOr(a.err, err);
 }
 

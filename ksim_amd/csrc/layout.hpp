@@ -75,7 +75,8 @@ enum CompHdr : int {
   CH_NGRAPHS = 0, CH_F, CH_NOUT, CH_WMAX, CH_COLBASE, CH_FSEL_OFF,
   CH_DEST_OFF, CH_GRAPH_OFF,
   CH_FSEL0,  // first f column when f_selection is contiguous, else 0xffffffff
-  CH_PAD0, CH_PAD1, CH_PAD2, CH_PAD3, CH_PAD4, CH_PAD5, CH_PAD6,
+  CH_DEST0,  // first output column when dest is contiguous, else 0xffffffff
+  CH_PAD1, CH_PAD2, CH_PAD3, CH_PAD4, CH_PAD5, CH_PAD6,
   CH_DWORDS
 };
 

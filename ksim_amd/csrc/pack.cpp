@@ -552,11 +552,14 @@ std::vector<uint32_t> pack_program(const ksim_graph_view *graphs, const int32_t 
     }
     w[ch + CH_FSEL0] = (F > 0 && contig) ? (uint32_t)f_selection[fi] : 0xffffffffu;
     w[ch + CH_DEST_OFF] = (uint32_t)w.size();
+    bool dcontig = true;
     for (int j = 0; j < n_out; ++j) {
       const int d = dest_col[oi + j];
       if (d < 0 || d >= num_outputs) throw std::invalid_argument("dest_col out of range");
+      if (d != dest_col[oi] + j) dcontig = false;
       w.push_back((uint32_t)d);
     }
+    w[ch + CH_DEST0] = (n_out > 0 && dcontig) ? (uint32_t)dest_col[oi] : 0xffffffffu;
     const size_t gh = w.size();
     w[ch + CH_GRAPH_OFF] = (uint32_t)gh;
     w.resize(gh + (size_t)ng * GH_DWORDS, 0u);

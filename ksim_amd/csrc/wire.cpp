@@ -85,6 +85,11 @@ void validate_words(const uint32_t *w, size_t n) {
     need((uint64_t)ch[CH_DEST_OFF] + nout, "output columns");
     for (uint32_t i = 0; i < nout; ++i)
       if (w[ch[CH_DEST_OFF] + i] >= num_out) bad("output column beyond num_outputs");
+    if (ch[CH_DEST0] != 0xffffffffu) {
+      if ((uint64_t)ch[CH_DEST0] + nout > num_out) bad("contiguous output columns beyond num_outputs");
+      for (uint32_t i = 0; i < nout; ++i)
+        if (w[ch[CH_DEST_OFF] + i] != ch[CH_DEST0] + i) bad("output columns are not contiguous");
+    }
     need((uint64_t)ch[CH_GRAPH_OFF] + (uint64_t)ng * GH_DWORDS, "graph headers");
     for (uint32_t g = 0; g < ng; ++g) {
       const uint32_t *gh = w + ch[CH_GRAPH_OFF] + (size_t)g * GH_DWORDS;

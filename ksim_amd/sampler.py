@@ -230,6 +230,22 @@ def pack_words(prog: FlatProgram, stats: dict | None = None) -> np.ndarray:
     return w[:n.value]
 
 
+def unpack_outputs(packed: np.ndarray, num_outputs: int) -> np.ndarray:
+    """(B, W) uint32 packed rows -> (B, num_outputs) uint8 (the layout of
+    ``HipProgramSampler.sample``)."""
+    packed = np.ascontiguousarray(packed, dtype="<u4")
+    B = packed.shape[0]
+    return np.unpackbits(packed.view(np.uint8).reshape(B, -1), axis=1, count=num_outputs,
+                         bitorder="little")
+
+
+def split_detectors(out: np.ndarray, num_detectors: int):
+    """(detector bits, observable bits) of a result, as the reference's
+    ``run`` returns them (src/python/ksim/run.py:75-76)."""
+    nd = int(num_detectors)
+    return out[:, :nd].astype(np.uint8), out[:, nd:].astype(np.uint8)
+
+
 def program_blob(prog: FlatProgram) -> bytes:
     """Pack ``prog`` into the wire format on the host (no device needed)."""
     lib = _lib.load()
@@ -289,6 +305,7 @@ class HipProgramSampler:
         info = np.zeros(8, np.int64)
         _lib.check(_lib.load().ksim_program_info(h, _lib.ptr(info, C.c_int64)))
         self.n_comp, self.min_num_f, self.num_outputs, self.tot_out = (int(x) for x in info[:4])
+        self.out_words = int(info[7])
         self.ops_per_shot = self.stats()["ops_per_shot"]
 
     @classmethod
@@ -328,6 +345,20 @@ class HipProgramSampler:
         out = np.zeros((B, self.num_outputs), dtype=np.uint8)
         _lib.check(_lib.load().ksim_program_sample(self._native._h, _lib.ptr(f, C.c_uint8), B,
                                                     f.shape[1], seed, _lib.ptr(out, C.c_uint8)))
+        return out
+
+    def sample_packed(self, f_params: np.ndarray, seed: int) -> np.ndarray:
+        """``sample`` with bit-packed rows: (B, ceil(num_outputs/32)) uint32,
+        bit c of a row = output column c; :func:`unpack_outputs` restores
+        the u8 array."""
+        f = self._prep_f(f_params)
+        seed = _check_seed(seed, self.n_comp)
+        B = f.shape[0]
+        out = np.zeros((B, self.out_words), dtype=np.uint32)
+        if self.out_words:
+            _lib.check(_lib.load().ksim_program_sample_packed(
+                self._native._h, _lib.ptr(f, C.c_uint8), B, f.shape[1], seed,
+                _lib.ptr(out, C.c_uint32)))
         return out
 
     def sample_uniforms(self, f_params: np.ndarray, uniforms: np.ndarray) -> np.ndarray:

@@ -211,3 +211,24 @@ def test_program_blob_round_trip():
     assert np.array_equal(t.sample(f, 21), s.sample(f, 21))
     with pytest.raises(ValueError):
         HipProgramSampler.from_bytes(blob[:-4])
+
+
+@pytest.mark.parametrize("which", ["single", "multi", "permuted", "wide"])
+def test_packed_rows_equal_byte_rows(which):
+    from ksim_amd.sampler import unpack_outputs
+    if which == "single":
+        prog = synth.shape_program(n_q=6, t=4, F=8, n_out=7, seed=2)
+    elif which == "multi":
+        prog = synth.shape_program(n_q=6, t=4, F=8, n_out=20, seed=3, n_components=3)
+    elif which == "permuted":
+        prog = synth.product_program([synth.htkh_program(1), synth.htkh_program(3, noisy_f=True),
+                                      synth.shape_program(n_q=5, t=2, F=3, n_out=4, seed=4)])
+    else:  # more than 64 outputs: several row words, contiguous columns
+        prog = synth.shape_program(n_q=4, t=0, F=10, n_out=70, seed=5)
+    s = HipProgramSampler(prog)
+    B = 333
+    f = synth.random_f(B, max(s.min_num_f, 1), 0.4, 8)
+    want = s.sample(f, 99)
+    packed = s.sample_packed(f, 99)
+    assert packed.shape == (B, (s.num_outputs + 31) // 32)
+    assert np.array_equal(unpack_outputs(packed, s.num_outputs), want)

@@ -119,3 +119,17 @@ def test_rejects_structural_corruption(blob):
             blob_stats(with_words(blob, v))
         # the unchanged words still pass (the re-sealing itself is sound)
     blob_stats(with_words(blob, w))
+
+
+def test_unpack_outputs_and_split():
+    from ksim_amd.sampler import split_detectors, unpack_outputs
+    rng = np.random.default_rng(0)
+    out = rng.integers(0, 2, size=(9, 45)).astype(np.uint8)
+    # pack as the kernel does: bit c of the row = column c, little-endian u32 words
+    words = np.zeros((9, 2), np.uint32)
+    for c in range(45):
+        words[:, c // 32] |= out[:, c].astype(np.uint32) << np.uint32(c % 32)
+    assert np.array_equal(unpack_outputs(words, 45), out)
+    det, obs = split_detectors(out, 40)
+    assert det.shape == (9, 40) and obs.shape == (9, 5)
+    assert np.array_equal(np.concatenate([det, obs], axis=1), out)

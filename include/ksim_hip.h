@@ -213,6 +213,53 @@ int ksim_program_create_from_blob(const uint8_t *blob, int64_t n, ksim_program *
  * device, packed output row words] of a program. */
 int ksim_program_info(const ksim_program *p, int64_t *info8);
 
+/* ---- device-side noise: the f bits drawn on the GPU ---------------------- */
+
+typedef struct ksim_noise ksim_noise;
+
+/* A noise model: n_channels channels, channel c with n_outcomes[c] (a power
+ * of two) outcome probabilities in `probs` (concatenated, outcome 0 = no
+ * error) and, for outcomes 1.., the f bits each flips as rows of num_f bytes
+ * in `patterns` (concatenated over channels, n_outcomes[c] - 1 rows each).
+ * Replaces ChannelSampler (src/python/ksim/compile/channels.py:503-658):
+ * same per-channel fire probability and conditional outcome distribution,
+ * drawn per shot on the device from a SplitMix64 stream keyed by
+ * (seed, shot index) instead of numpy's PCG64 on the host.  The tables are
+ * built on the host here and uploaded on first device use. */
+int ksim_noise_create(const double *probs, const int32_t *n_outcomes, const uint8_t *patterns,
+                      int32_t n_channels, int32_t num_f, ksim_noise **out);
+
+/* [channels kept, num_f, pattern words per row, cdf entries]. */
+int ksim_noise_info(const ksim_noise *nz, int64_t *info4);
+
+/* Copy the sampling tables (sizes from ksim_noise_info; any pointer may be
+ * NULL): skip (C+1 u64), per kept channel acc (u64), n_out, off_cdf,
+ * off_pat (u32) and its input index `source` (i32), cdf (u64), pat (u32). */
+int ksim_noise_tables(const ksim_noise *nz, uint64_t *skip, uint64_t *acc, uint32_t *n_out,
+                      uint32_t *off_cdf, uint32_t *off_pat, uint64_t *cdf, uint32_t *pat,
+                      int32_t *source);
+
+/* Draw f rows for shots [shot_base, shot_base + B): d_out (B, f_cols) u8,
+ * f_cols >= num_f, columns [0, num_f) written. */
+int ksim_noise_sample_device(ksim_noise *nz, int64_t B, int64_t shot_base, uint64_t seed,
+                             uint8_t *d_out, int32_t f_cols, void *stream);
+
+/* Host variant: out (B, num_f) u8, the rows of ksim_noise_sample_device. */
+int ksim_noise_sample(ksim_noise *nz, int64_t B, int64_t shot_base, uint64_t seed,
+                      uint8_t *out);
+
+/* Sample a program with its f bits drawn inside the kernel (no f matrix in
+ * HBM, no host noise): the same result as ksim_program_sample_device on the
+ * rows ksim_noise_sample_device(nz, B, shot_base, noise_seed) produces.
+ * The noise model must cover the program's f bits. */
+int ksim_program_sample_noisy_device(ksim_program *p, ksim_noise *nz, int64_t B,
+                                     int64_t shot_base, uint64_t seed, uint64_t noise_seed,
+                                     const double *d_U, uint8_t *d_out, void *stream);
+int ksim_program_sample_noisy(ksim_program *p, ksim_noise *nz, int64_t B, uint64_t seed,
+                              uint64_t noise_seed, uint8_t *out);
+
+void ksim_noise_destroy(ksim_noise *nz);
+
 /* ---- batched persistent path ------------------------------------------ */
 
 typedef struct ksim_batch ksim_batch;

@@ -82,6 +82,23 @@ EXPORTS = {
     "ksim_program_create_from_blob": (C.c_int, [C.POINTER(C.c_uint8), C.c_int64,
                                                 C.POINTER(C.c_void_p)]),
     "ksim_program_info": (C.c_int, [C.c_void_p, i64p]),
+    "ksim_noise_create": (C.c_int, [C.POINTER(C.c_double), i32p, C.POINTER(C.c_uint8), C.c_int32,
+                                    C.c_int32, C.POINTER(C.c_void_p)]),
+    "ksim_noise_info": (C.c_int, [C.c_void_p, i64p]),
+    "ksim_noise_tables": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                    C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                    C.POINTER(C.c_uint32), C.POINTER(C.c_uint64),
+                                    C.POINTER(C.c_uint32), i32p]),
+    "ksim_noise_sample_device": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_uint64,
+                                           C.c_void_p, C.c_int32, C.c_void_p]),
+    "ksim_noise_sample": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_uint64,
+                                    C.POINTER(C.c_uint8)]),
+    "ksim_program_sample_noisy_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int64,
+                                                   C.c_uint64, C.c_uint64, C.c_void_p,
+                                                   C.c_void_p, C.c_void_p]),
+    "ksim_program_sample_noisy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_uint64,
+                                            C.c_uint64, C.POINTER(C.c_uint8)]),
+    "ksim_noise_destroy": (None, [C.c_void_p]),
     "ksim_program_sample_packed": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint8), C.c_int64,
                                              C.c_int32, C.c_uint64, C.POINTER(C.c_uint32)]),
     "ksim_program_sample_device_packed": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64,

@@ -30,7 +30,7 @@ DEVICE = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-pa
 
 def sources():
     return [os.path.join(CSRC, "api.hip"), os.path.join(CSRC, "pack.cpp"),
-            os.path.join(CSRC, "wire.cpp")]
+            os.path.join(CSRC, "wire.cpp"), os.path.join(CSRC, "noise.cpp")]
 
 
 def build(verbose: bool = False, out: str = LIB, extra: list | None = None) -> str:

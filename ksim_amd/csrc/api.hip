@@ -118,15 +118,50 @@ __global__ void k_fill_u8(uint8_t *p, uint8_t v, int64_t n) {
   if (i < n) p[i] = v;
 }
 
-// Plain path: one lane per shot, grid (ceil(B/256), n_comp).
-template <int WL>
+// Plain path: one lane per shot, grid (ceil(B/256), n_comp).  NOISE: the
+// f bits are drawn in the prologue instead of read.
+template <int WL, bool NOISE = false>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KSIM_WAVES_PER_EU))) void k_sample(ShotArgs a, const int4 *__restrict__ gtab,
                                                   const int4 *__restrict__ pitab) {
   __shared__ LdsTables lt;
   load_tables(&lt, gtab, pitab);
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= a.B) return;
-  run_shot<WL>(a, (int)blockIdx.y, s, &lt, gtab);
+  run_shot<WL, NOISE>(a, (int)blockIdx.y, s, &lt, gtab);
+}
+
+// Noise rows: lane per shot writes its f row (B, f_cols) u8, columns
+// [0, num_f), 128 f bits per pass over the shot's noise stream.
+__global__ __launch_bounds__(kBlock) void k_noise_rows(NoiseDev nz, int64_t B, int64_t shot_base,
+                                                      int32_t num_f, int32_t f_cols,
+                                                      uint8_t *__restrict__ out) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= B) return;
+  uint8_t *row = out + s * (int64_t)f_cols;
+  const bool dw = ((f_cols | (int32_t)(uintptr_t)out) & 3) == 0;
+  for (int q = 0; 128 * q < num_f; ++q) {
+    uint32_t buf[4] = {0u, 0u, 0u, 0u};
+    if (nz.C > 0)
+      noise_walk(nz, noise_stream(nz.seed, (uint64_t)(shot_base + s)), [&](int, uint32_t r) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          if (4 * q + k < nz.wf) buf[k] ^= nz.pat[r + 4 * q + k];
+      });
+    const int n = min(128, num_f - 128 * q);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      for (int j = 0; j < 32; j += 4) {
+        const int col = 128 * q + 32 * k + j;
+        if (32 * k + j >= n) break;
+        const uint32_t b = (buf[k] >> j) & 0xfu;
+        if (dw && 32 * k + j + 4 <= n) {
+          *(uint32_t *)(row + col) = (b * 0x00204081u) & 0x01010101u;  // 4 bits -> 4 bytes
+        } else {
+          for (int t = 0; t < 4 && 32 * k + j + t < n; ++t) row[col + t] = (uint8_t)((b >> t) & 1u);
+        }
+      }
+    }
+  }
 }
 
 // Explicit parameter rows: exact value (mode 0) or |value| (mode 1) of one graph.
@@ -222,6 +257,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KSIM_WAV
     a.B = bp.B; a.shot_base = 0; a.seed = bp.seed;
     a.num_f = bp.num_f; a.num_outputs = bp.num_outputs; a.tot_out = bp.tot_out; a.pad = 0;
     a.out_words = 0; a.out_atomic = 0;
+    a.noise = NoiseDev{};
     for (int sub = 0; sub < it.count; sub += 64) {
       const int64_t s = it.shot0 + sub + lane;
       if (sub + lane < it.count && s < bp.B) run_shot<WL>(a, it.comp, s, &lt, gtab);
@@ -292,8 +328,9 @@ void check_err_sync(ksim_program *p, hipStream_t st) {
 
 void launch_sample(ksim_program *p, const uint8_t *d_f, int64_t B, int32_t f_cols,
                    int64_t shot_base, uint64_t seed, const double *d_U, uint8_t *d_out,
-                   hipStream_t st, int out_words = 0) {
-  if (f_cols < p->num_f) throw std::invalid_argument("f_params has fewer columns than f_selection needs");
+                   hipStream_t st, int out_words = 0, const NoiseDev *noise = nullptr) {
+  if (!noise && f_cols < p->num_f)
+    throw std::invalid_argument("f_params has fewer columns than f_selection needs");
   if (B <= 0 || p->n_comp == 0) return;
   DevTables t = tables_for_current_device();
   ShotArgs a;
@@ -302,6 +339,7 @@ void launch_sample(ksim_program *p, const uint8_t *d_f, int64_t B, int32_t f_col
   a.num_f = f_cols; a.num_outputs = p->num_outputs; a.tot_out = p->tot_out; a.pad = 0;
   a.out_words = out_words;
   a.out_atomic = p->n_comp > 1 ? 1 : 0;
+  a.noise = noise ? *noise : NoiseDev{};
   if (out_words > 0 && !p->packed_full_rows) {
     // rows assembled from several components or scattered bits: OR-ed into
     // zeroed words (a fill kernel, so the launch stays graph-capturable)
@@ -312,11 +350,16 @@ void launch_sample(ksim_program *p, const uint8_t *d_f, int64_t B, int32_t f_col
   const int64_t nb = (B + kBlock - 1) / kBlock;
   if (nb > 0x7fffffff) throw std::invalid_argument("too many shots for one launch");
   dim3 grid((unsigned)nb, (unsigned)p->n_comp);
-  switch (p->wl) {
-    case 1: hipLaunchKernelGGL(k_sample<1>, grid, dim3(kBlock), 0, st, a, t.tab, t.pitab); break;
-    case 2: hipLaunchKernelGGL(k_sample<2>, grid, dim3(kBlock), 0, st, a, t.tab, t.pitab); break;
-    case 4: hipLaunchKernelGGL(k_sample<4>, grid, dim3(kBlock), 0, st, a, t.tab, t.pitab); break;
-    default: hipLaunchKernelGGL(k_sample<8>, grid, dim3(kBlock), 0, st, a, t.tab, t.pitab); break;
+  const bool nz = a.noise.C > 0;
+  switch (p->wl * 2 + (nz ? 1 : 0)) {
+    case 2: hipLaunchKernelGGL((k_sample<1, false>), grid, dim3(kBlock), 0, st, a, t.tab, t.pitab); break;
+    case 3: hipLaunchKernelGGL((k_sample<1, true>), grid, dim3(kBlock), 0, st, a, t.tab, t.pitab); break;
+    case 4: hipLaunchKernelGGL((k_sample<2, false>), grid, dim3(kBlock), 0, st, a, t.tab, t.pitab); break;
+    case 5: hipLaunchKernelGGL((k_sample<2, true>), grid, dim3(kBlock), 0, st, a, t.tab, t.pitab); break;
+    case 8: hipLaunchKernelGGL((k_sample<4, false>), grid, dim3(kBlock), 0, st, a, t.tab, t.pitab); break;
+    case 9: hipLaunchKernelGGL((k_sample<4, true>), grid, dim3(kBlock), 0, st, a, t.tab, t.pitab); break;
+    case 16: hipLaunchKernelGGL((k_sample<8, false>), grid, dim3(kBlock), 0, st, a, t.tab, t.pitab); break;
+    default: hipLaunchKernelGGL((k_sample<8, true>), grid, dim3(kBlock), 0, st, a, t.tab, t.pitab); break;
   }
   hip_check(hipGetLastError(), "k_sample launch");
 }
@@ -1043,6 +1086,198 @@ void ksim_sched_destroy(ksim_sched *q) {
   }
   if (q->d_cursor) (void)hipFree(q->d_cursor);
   delete q;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ noise
+struct ksim_noise {
+  int device = -1;  // device holding the tables (uploaded on first device use)
+  NoiseHost h;
+  uint64_t *d_skip = nullptr;
+  NoiseChan *d_chan = nullptr;
+  uint64_t *d_cdf = nullptr;
+  uint32_t *d_pat = nullptr;
+  uint8_t *d_rows = nullptr;  // staging for the host-pointer API
+  size_t rows_cap = 0;
+
+  NoiseDev dev(uint64_t seed) const {
+    NoiseDev d;
+    d.skip = d_skip; d.chan = d_chan; d.cdf = d_cdf; d.pat = d_pat;
+    d.seed = seed; d.C = h.C; d.wf = h.wf;
+    return d;
+  }
+};
+
+namespace {
+
+template <class T>
+T *upload_vec(const std::vector<T> &v, const char *what) {
+  T *d = nullptr;
+  const size_t n = std::max<size_t>(v.size(), 1) * sizeof(T);
+  hip_check(hipMalloc(&d, n), what);
+  if (!v.empty()) hip_check(hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), what);
+  return d;
+}
+
+// upload the tables to the current device on first use (a noise model is
+// built on the host, so it can be inspected without a device)
+void noise_on_device(ksim_noise *nz) {
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  if (nz->device == dev) return;
+  if (nz->device >= 0) throw std::invalid_argument("noise model used on a second device");
+  nz->d_skip = upload_vec(nz->h.skip, "upload noise skip table");
+  nz->d_chan = upload_vec(nz->h.chan, "upload noise channels");
+  nz->d_cdf = upload_vec(nz->h.cdf, "upload noise cdf");
+  nz->d_pat = upload_vec(nz->h.pat, "upload noise patterns");
+  nz->device = dev;
+}
+
+void launch_noise_rows(ksim_noise *nz, int64_t B, int64_t shot_base, uint64_t seed,
+                       uint8_t *d_out, int32_t f_cols, hipStream_t st) {
+  if (f_cols < nz->h.num_f) throw std::invalid_argument("f_cols is smaller than the noise model's f bits");
+  if (B <= 0 || nz->h.num_f == 0) return;
+  noise_on_device(nz);
+  const int64_t nb = (B + kBlock - 1) / kBlock;
+  if (nb > 0x7fffffff) throw std::invalid_argument("too many shots for one launch");
+  hipLaunchKernelGGL(k_noise_rows, dim3((unsigned)nb), dim3(kBlock), 0, st, nz->dev(seed), B,
+                     shot_base, nz->h.num_f, f_cols, d_out);
+  hip_check(hipGetLastError(), "k_noise_rows launch");
+}
+
+void sample_noisy_device(ksim_program *p, ksim_noise *nz, int64_t B, int64_t shot_base,
+                         uint64_t seed, uint64_t noise_seed, const double *d_U, uint8_t *d_out,
+                         hipStream_t st) {
+  if (!p || !nz) throw std::invalid_argument("NULL argument");
+  if (B < 0) throw std::invalid_argument("B < 0");
+  if (nz->h.num_f < p->num_f)
+    throw std::invalid_argument("the noise model has fewer f bits than the program selects");
+  if (B > 0 && !d_out) throw std::invalid_argument("NULL buffer");
+  reset_err_async(p, st);
+  noise_on_device(nz);
+  const NoiseDev nd = nz->dev(noise_seed);
+  if (nd.C == 0) {
+    // nothing can fire: every f bit is 0
+    const int32_t fc = std::max(p->num_f, 1);
+    ensure(p->d_f, p->f_cap, (size_t)B * fc);
+    fill_u8_async(p->d_f, 0, B * fc, st);
+    launch_sample(p, p->d_f, B, fc, shot_base, seed, d_U, d_out, st);
+  } else {
+    launch_sample(p, nullptr, B, 0, shot_base, seed, d_U, d_out, st, 0, &nd);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int ksim_noise_create(const double *probs, const int32_t *n_outcomes, const uint8_t *patterns,
+                      int32_t n_channels, int32_t num_f, ksim_noise **out) {
+  return guarded([&] {
+    if (!out) throw std::invalid_argument("out is NULL");
+    auto *nz = new ksim_noise();
+    try {
+      nz->h = build_noise(probs, n_outcomes, patterns, n_channels, num_f);
+    } catch (...) {
+      ksim_noise_destroy(nz);
+      throw;
+    }
+    *out = nz;
+  });
+}
+
+int ksim_noise_info(const ksim_noise *nz, int64_t *info4) {
+  return guarded([&] {
+    if (!nz || !info4) throw std::invalid_argument("NULL argument");
+    info4[0] = nz->h.C;
+    info4[1] = nz->h.num_f;
+    info4[2] = nz->h.wf;
+    info4[3] = (int64_t)nz->h.cdf.size();
+  });
+}
+
+int ksim_noise_tables(const ksim_noise *nz, uint64_t *skip, uint64_t *acc, uint32_t *n_out,
+                      uint32_t *off_cdf, uint32_t *off_pat, uint64_t *cdf, uint32_t *pat,
+                      int32_t *source) {
+  return guarded([&] {
+    if (!nz) throw std::invalid_argument("noise is NULL");
+    const NoiseHost &h = nz->h;
+    if (skip) std::memcpy(skip, h.skip.data(), h.skip.size() * 8);
+    for (int c = 0; c < h.C; ++c) {
+      if (acc) acc[c] = h.chan[c].acc;
+      if (n_out) n_out[c] = h.chan[c].n_out;
+      if (off_cdf) off_cdf[c] = h.chan[c].off_cdf;
+      if (off_pat) off_pat[c] = h.chan[c].off_pat;
+      if (source) source[c] = h.source[c];
+    }
+    if (cdf && !h.cdf.empty()) std::memcpy(cdf, h.cdf.data(), h.cdf.size() * 8);
+    if (pat && !h.pat.empty()) std::memcpy(pat, h.pat.data(), h.pat.size() * 4);
+  });
+}
+
+int ksim_noise_sample_device(ksim_noise *nz, int64_t B, int64_t shot_base, uint64_t seed,
+                             uint8_t *d_out, int32_t f_cols, void *stream) {
+  return guarded([&] {
+    if (!nz) throw std::invalid_argument("noise is NULL");
+    if (B < 0) throw std::invalid_argument("B < 0");
+    if (B > 0 && nz->h.num_f > 0 && !d_out) throw std::invalid_argument("NULL buffer");
+    launch_noise_rows(nz, B, shot_base, seed, d_out, f_cols, (hipStream_t)stream);
+  });
+}
+
+int ksim_noise_sample(ksim_noise *nz, int64_t B, int64_t shot_base, uint64_t seed,
+                      uint8_t *out) {
+  return guarded([&] {
+    if (!nz) throw std::invalid_argument("noise is NULL");
+    if (B < 0) throw std::invalid_argument("B < 0");
+    const size_t n = (size_t)B * nz->h.num_f;
+    if (n == 0) return;
+    if (!out) throw std::invalid_argument("NULL buffer");
+    if (n > nz->rows_cap) {
+      if (nz->d_rows) hip_check(hipFree(nz->d_rows), "hipFree");
+      nz->d_rows = nullptr;
+      nz->rows_cap = 0;
+      hip_check(hipMalloc(&nz->d_rows, n), "hipMalloc(noise rows)");
+      nz->rows_cap = n;
+    }
+    launch_noise_rows(nz, B, shot_base, seed, nz->d_rows, nz->h.num_f, nullptr);
+    hip_check(hipMemcpy(out, nz->d_rows, n, hipMemcpyDeviceToHost), "D2H noise rows");
+  });
+}
+
+int ksim_program_sample_noisy_device(ksim_program *p, ksim_noise *nz, int64_t B,
+                                     int64_t shot_base, uint64_t seed, uint64_t noise_seed,
+                                     const double *d_U, uint8_t *d_out, void *stream) {
+  return guarded([&] {
+    sample_noisy_device(p, nz, B, shot_base, seed, noise_seed, d_U, d_out, (hipStream_t)stream);
+  });
+}
+
+int ksim_program_sample_noisy(ksim_program *p, ksim_noise *nz, int64_t B, uint64_t seed,
+                              uint64_t noise_seed, uint8_t *out) {
+  return guarded([&] {
+    if (!p || !nz) throw std::invalid_argument("NULL argument");
+    if (B < 0) throw std::invalid_argument("B < 0");
+    if (B == 0) return;
+    if (!out) throw std::invalid_argument("NULL buffer");
+    const size_t ob = (size_t)B * p->num_outputs;
+    ensure(p->d_out, p->out_cap, ob);
+    if (p->tot_out < p->num_outputs) fill_u8_async(p->d_out, 0, (int64_t)ob, nullptr);
+    sample_noisy_device(p, nz, B, 0, seed, noise_seed, nullptr, p->d_out, nullptr);
+    hip_check(hipMemcpy(out, p->d_out, ob, hipMemcpyDeviceToHost), "D2H out");
+    check_err_sync(p, nullptr);
+  });
+}
+
+void ksim_noise_destroy(ksim_noise *nz) {
+  if (!nz) return;
+  if (nz->d_skip) (void)hipFree(nz->d_skip);
+  if (nz->d_chan) (void)hipFree(nz->d_chan);
+  if (nz->d_cdf) (void)hipFree(nz->d_cdf);
+  if (nz->d_pat) (void)hipFree(nz->d_pat);
+  if (nz->d_rows) (void)hipFree(nz->d_rows);
+  delete nz;
 }
 
 }  // extern "C"

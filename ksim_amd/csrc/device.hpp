@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include "layout.hpp"
+#include "noise.hpp"
 
 namespace ksim {
 
@@ -348,6 +349,59 @@ __device__ __forceinline__ uint32_t get_param_bit(const uint32_t (&pv)[WL], int 
   return (x >> (pos & 31)) & 1u;
 }
 
+// Device view of a noise model (tables built by build_noise, noise.hpp).
+struct NoiseDev {
+  const uint64_t *skip;    // C + 1
+  const NoiseChan *chan;   // C
+  const uint64_t *cdf;
+  const uint32_t *pat;     // wf words per outcome row
+  uint64_t seed;
+  int32_t C, wf;
+};
+
+// SplitMix64 (S. Vigna): the per-shot noise stream.  next() advances the
+// state by the golden-ratio increment and returns the mixed value.
+__host__ __device__ inline uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+__host__ __device__ inline uint64_t splitmix_next(uint64_t &x) {
+  x += 0x9e3779b97f4a7c15ull;
+  return mix64(x);
+}
+// start of shot `shot`'s stream under `seed`
+__host__ __device__ inline uint64_t noise_stream(uint64_t seed, uint64_t shot) {
+  return mix64(seed ^ mix64(shot + 0x632be59bd9b4e019ull));
+}
+
+// Walk the channels of one shot: calls fire(channel, outcome) for every
+// channel that fires, in channel order.
+template <class Fire>
+__device__ __forceinline__ void noise_walk(const NoiseDev &nz, uint64_t x, Fire &&fire) {
+  int c = 0;
+  while (c < nz.C) {
+    // number of channels skipped before the next candidate: the largest
+    // k <= C - c with u < skip[k]  (P(k or more skipped) = (1 - p_max)^k)
+    const uint64_t u = splitmix_next(x);
+    int lo = 0, hi = nz.C - c;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (u < nz.skip[mid]) lo = mid; else hi = mid - 1;
+    }
+    c += lo;
+    if (c >= nz.C) break;
+    const NoiseChan ch = nz.chan[c];
+    if (splitmix_next(x) <= ch.acc) {
+      const uint64_t v = splitmix_next(x);
+      uint32_t o = 0;
+      while (o + 1 < ch.n_out && v > nz.cdf[ch.off_cdf + o]) ++o;
+      fire(c, ch.off_pat + o * (uint32_t)nz.wf);
+    }
+    ++c;
+  }
+}
+
 struct ShotArgs {
   const uint32_t *prog;
   const uint8_t *f;   // (B, num_f)
@@ -360,6 +414,7 @@ struct ShotArgs {
   int32_t out_words;  // 0: one byte per output; > 0: packed rows of this many words
   int32_t out_atomic; // packed rows shared by several components: OR words in
   int32_t pad;
+  NoiseDev noise;     // noise.C > 0: f bits drawn on the device, f unused
 };
 
 // 32 bits of the lane's parameter words starting at bit `off` (wave-uniform,
@@ -413,7 +468,7 @@ __device__ __forceinline__ void write_packed_row(const ShotArgs &a, const uint32
 
 // The whole autoregressive loop of one shot of component `comp`
 // (src/cpp/ksim/kokkos_sim.cpp:214-230, src/python/ksim/sample/evaluate.py:76-85).
-template <int WL>
+template <int WL, bool NOISE = false>
 __device__ __forceinline__ void run_shot(const ShotArgs &a, int comp, int64_t s,
                                          const LdsTables *lt, const int4 *__restrict__ gtab) {
   const cword *__restrict__ P = as_const(a.prog);
@@ -426,7 +481,36 @@ __device__ __forceinline__ void run_shot(const ShotArgs &a, int comp, int64_t s,
   const uint8_t *frow = a.f + s * (int64_t)a.num_f;
   uint32_t pv[WL];
   const uint32_t fsel0 = ch[CH_FSEL0];
-  if (fsel0 != 0xffffffffu && (((uint32_t)a.num_f | fsel0 | (uint32_t)(uintptr_t)a.f) & 3u) == 0) {
+  if constexpr (NOISE) {
+    // f bits drawn here: XOR of the fired outcomes' patterns, seen through
+    // this component's f_selection
+#pragma unroll
+    for (int w = 0; w < WL; ++w) pv[w] = 0;
+    noise_walk(a.noise, noise_stream(a.noise.seed, (uint64_t)(a.shot_base + s)),
+               [&](int, uint32_t row) {
+                 const uint32_t *pr = a.noise.pat + row;
+                 if (fsel0 != 0xffffffffu) {
+#pragma unroll
+                   for (int w = 0; w < WL; ++w) {
+                     const int off = (int)fsel0 + 32 * w;
+                     const int wi = off >> 5, r = off & 31;
+                     uint32_t lo = wi < a.noise.wf ? pr[wi] : 0u;
+                     uint32_t hi = wi + 1 < a.noise.wf ? pr[wi + 1] : 0u;
+                     uint32_t v = r ? (lo >> r) | (hi << (32 - r)) : lo;
+                     const int left = F - 32 * w;
+                     v &= left >= 32 ? 0xffffffffu : (left > 0 ? (1u << left) - 1u : 0u);
+                     pv[w] ^= v;
+                   }
+                 } else {
+                   for (int j = 0; j < F; ++j) {
+                     const uint32_t b = fsel[j];
+                     const uint32_t bit = (pr[b >> 5] >> (b & 31)) & 1u;
+                     set_param_bit<WL>(pv, j, get_param_bit<WL>(pv, j) ^ bit);
+                   }
+                 }
+               });
+  } else if (fsel0 != 0xffffffffu &&
+             (((uint32_t)a.num_f | fsel0 | (uint32_t)(uintptr_t)a.f) & 3u) == 0) {
     // contiguous selection on a dword-aligned row: 4 f bytes per load,
     // bit 0 of each byte gathered by one multiply (bits 24..27 of the product)
     const uint32_t *row4 = (const uint32_t *)(frow + fsel0);

@@ -347,6 +347,44 @@ class HipProgramSampler:
                                                     f.shape[1], seed, _lib.ptr(out, C.c_uint8)))
         return out
 
+    def sample_noisy(self, noise, shots: int, seed: int, noise_seed: int | None = None,
+                     shot_base: int = 0) -> np.ndarray:
+        """Sample ``shots`` shots with the f bits drawn inside the kernel from
+        ``noise`` (a :class:`ksim_amd.noise.NoiseModel`, streams keyed by
+        ``noise_seed``, default the model's seed): the result of ``sample`` on
+        ``noise.sample(shots, shot_base)`` with no f matrix anywhere."""
+        seed = _check_seed(seed, self.n_comp)
+        B = int(shots)
+        out = np.zeros((B, self.num_outputs), dtype=np.uint8)
+        ns = noise.seed if noise_seed is None else int(noise_seed)
+        if B and shot_base == 0:
+            _lib.check(_lib.load().ksim_program_sample_noisy(
+                self._native._h, noise.handle, B, seed, ns, _lib.ptr(out, C.c_uint8)))
+        elif B:
+            import torch
+            d_out = torch.zeros((B, max(self.num_outputs, 1)), dtype=torch.uint8, device="cuda")
+            self.sample_noisy_device(noise, d_out, seed, ns, shot_base)
+            self.check()
+            out = d_out.cpu().numpy()[:, :self.num_outputs]
+        return out
+
+    def sample_noisy_device(self, noise, d_out, seed: int, noise_seed: int | None = None,
+                            shot_base: int = 0, stream=None):
+        """Resident variant: d_out a (B, num_outputs) uint8 CUDA tensor."""
+        import torch
+        st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        ns = noise.seed if noise_seed is None else int(noise_seed)
+        _lib.check(_lib.load().ksim_program_sample_noisy_device(
+            self._native._h, noise.handle, int(d_out.shape[0]), int(shot_base), int(seed), ns,
+            None, d_out.data_ptr() if d_out.numel() else None, st))
+
+    def check(self, stream=None):
+        """Wait for the program's launches on ``stream`` and raise a device
+        range error, if any."""
+        import torch
+        st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        _lib.check(_lib.load().ksim_program_check(self._native._h, st))
+
     def sample_packed(self, f_params: np.ndarray, seed: int) -> np.ndarray:
         """``sample`` with bit-packed rows: (B, ceil(num_outputs/32)) uint32,
         bit c of a row = output column c; :func:`unpack_outputs` restores

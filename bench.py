@@ -181,6 +181,68 @@ def plain_result(name, steps, warmup, with_cpu=True):
     return res
 
 
+def noise_result(prog, B, steps, warmup, with_cpu=True):
+    """config2 with its noise: f bits drawn on the device, (a) fused into the
+    sampling kernel, (b) as f rows then a plain launch; beside it the host
+    path of the reference (numpy channel sampler restated, timed on a
+    bounded sample, plus the H2D copy of the f rows)."""
+    import torch
+    from ksim_amd import _lib, synth
+    from ksim_amd.noise import NoiseModel
+    from ksim_amd.sampler import HipProgramSampler
+    probs, T = synth.config2_noise()
+    nm = NoiseModel(probs, T, seed=102)
+    s = HipProgramSampler(prog)
+    lib = _lib.load()
+    out = torch.zeros((B, s.num_outputs), dtype=torch.uint8, device="cuda")
+    f = torch.zeros((B, nm.num_f), dtype=torch.uint8, device="cuda")
+
+    def fused():
+        st = torch.cuda.current_stream().cuda_stream
+        _lib.check(lib.ksim_program_sample_noisy_device(s.handle, nm.handle, B, 0, 12345, 102,
+                                                        None, out.data_ptr(), st))
+
+    def rows_then_sample():
+        st = torch.cuda.current_stream().cuda_stream
+        _lib.check(lib.ksim_noise_sample_device(nm.handle, B, 0, 102, f.data_ptr(), nm.num_f, st))
+        _lib.check(lib.ksim_program_sample_device(s.handle, f.data_ptr(), B, nm.num_f, 0, 12345,
+                                                  None, out.data_ptr(), st))
+
+    def rows_only():
+        st = torch.cuda.current_stream().cuda_stream
+        _lib.check(lib.ksim_noise_sample_device(nm.handle, B, 0, 102, f.data_ptr(), nm.num_f, st))
+
+    _, dev_f = time_steps(fused, steps, warmup)
+    s.check()
+    fused_out = out.clone()
+    _, dev_r = time_steps(rows_then_sample, steps, warmup)
+    s.check()
+    same = bool(torch.equal(fused_out, out))
+    _, dev_n = time_steps(rows_only, steps, warmup)
+    res = dict(channels=len(probs), e_bits=int(T.shape[1]), f_bits=int(T.shape[0]), shots=B,
+               fused_kernel_ms=dev_f / steps * 1e3, fused_shots_per_s=B * steps / dev_f,
+               rows_then_sample_ms=dev_r / steps * 1e3, noise_rows_ms=dev_n / steps * 1e3,
+               fused_equals_rows_then_sample=same)
+    if with_cpu:
+        from oracle import numpy_ref
+        n = 200_000
+        t0 = time.perf_counter()
+        rows = numpy_ref.channel_sample(probs, T, n, np.random.default_rng(0))
+        dt = time.perf_counter() - t0
+        host = torch.from_numpy(np.zeros((B, nm.num_f), np.uint8)).pin_memory()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        f.copy_(host, non_blocking=True)
+        torch.cuda.synchronize()
+        h2d = time.perf_counter() - t1
+        res["host_path"] = dict(
+            kind="port", cores=1, sample=f"{n} shots, numpy restatement of ChannelSampler.sample",
+            noise_us_per_shot=dt / n * 1e6, h2d_ms_for_all_shots=h2d * 1e3,
+            noise_plus_h2d_ms_for_all_shots=dt / n * B * 1e3 + h2d * 1e3,
+            mean_f_weight=float(rows.sum(axis=1).mean()))
+    return res
+
+
 def batched_result(steps, warmup, n_programs=64, tune=True):
     """config4: the same 64 programs sampled (a) by one plain launch each,
     back to back on one stream, and (b) by one persistent batched launch
@@ -343,6 +405,10 @@ def main():
         print(json.dumps({"metric": "config5 serving", **serving_result()}), flush=True)
         return
     res = plain_result(args.workload, args.steps, args.warmup, with_cpu=not args.no_cpu)
+    if args.workload == "config2":
+        w2 = make_workload("config2")
+        res["noise"] = noise_result(w2["prog"], w2["B"], args.steps, args.warmup,
+                                    with_cpu=not args.no_cpu)
     line = {
         "metric": "shots/s & \u00b5s/shot vs T-count, plain vs batched (exact); p50/p99 latency",
         "value": res["shots_per_s"],
@@ -363,6 +429,8 @@ def main():
         "roofline": res["roofline"],
         "cpu_baseline": res.get("cpu_baseline"),
     }
+    if "noise" in res:
+        line["noise"] = res["noise"]
     if args.suite:
         line["suite"] = {}
         for name in ("config1", "config3"):

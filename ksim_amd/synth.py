@@ -326,6 +326,27 @@ def config2_program(seed: int = 2) -> FlatProgram:
     return shape_program(n_q=9, t=0, F=100, n_out=29, seed=seed)
 
 
+def config2_noise(seed: int = 102, num_f: int = 100, p: float = 1e-3):
+    """Noise of the d=5 repetition-code memory (5 rounds): DEPOLARIZE2(p)
+    after each of the 40 CX (4-bit channels, 15 equally likely Pauli pairs)
+    and X_ERROR(p) on the 25 measured qubits: 185 e-bits mapped onto
+    ``num_f`` f-bits by a seeded sparse GF(2) transform (each e-bit flips 1-3
+    f-bits), as ``(channel_probs, error_transform)``."""
+    rng = np.random.default_rng(seed)
+    probs = []
+    for _ in range(40):
+        pr = np.full(16, p / 15.0)
+        pr[0] = 1.0 - p
+        probs.append(pr)
+    for _ in range(25):
+        probs.append(np.array([1.0 - p, p]))
+    num_e = 40 * 4 + 25
+    T = np.zeros((num_f, num_e), np.uint8)
+    for e in range(num_e):
+        T[rng.choice(num_f, size=int(rng.integers(1, 4)), replace=False), e] = 1
+    return probs, T
+
+
 def config3_program(seed: int = 3) -> FlatProgram:
     """30-qubit brickwork encoder, 12 T + DEPOLARIZE1 after each T:
     n_out = 30, F = 24, G ~ 2^{0.228*12} ~ 7."""

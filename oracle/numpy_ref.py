@@ -134,3 +134,43 @@ def sample_flat(prog, f_params: np.ndarray, rng: np.random.Generator,
         outs.append(m)
     combined = np.concatenate(outs, axis=1) if outs else np.zeros((B, 0), int)
     return combined[:, np.argsort(prog.output_order)]
+
+
+def channel_sample(channel_probs, error_transform, num_samples: int,
+                   rng: np.random.Generator) -> np.ndarray:
+    """The reference's host noise sampler, restated: ``ChannelSampler.sample``
+    (src/python/ksim/compile/channels.py:622-658) with its per-channel data
+    (``_precompute_sparse``, :575-620) built without the simplification
+    passes, which leave the f distribution unchanged.  Geometric skips along
+    the shot axis, ``searchsorted`` on the conditional CDF, XOR of the
+    outcome's pattern into the rows.  Used as the CPU baseline of noise
+    generation (the stream differs from the reference's only in that
+    simplified channels consume the generator in a different order)."""
+    T = np.asarray(error_transform, dtype=np.uint8)
+    num_f = T.shape[0]
+    result = np.zeros((num_samples, num_f), dtype=np.uint8)
+    e_off = 0
+    for probs in channel_probs:
+        probs = np.asarray(probs, dtype=np.float64)
+        n = len(probs)
+        k = int(np.log2(n))
+        cols = T[:, e_off:e_off + k]
+        e_off += k
+        p_fire = 1.0 - float(probs[0])
+        if p_fire <= 1e-15 or n <= 1:
+            continue
+        cond_cdf = np.cumsum(probs[1:] / p_fire, dtype=np.float64)
+        cond_cdf /= cond_cdf[-1]
+        outcomes = np.arange(1, n)
+        bits = ((outcomes[:, None] >> np.arange(k)) & 1).astype(np.uint8)
+        pats = (bits @ cols.T % 2).astype(np.uint8)
+        expected = num_samples * p_fire
+        sigma = np.sqrt(expected * (1.0 - p_fire))
+        n_draws = int(expected + 7.0 * sigma) + 100
+        positions = np.cumsum(rng.geometric(p_fire, size=n_draws)) - 1
+        positions = positions[positions < num_samples]
+        if len(positions) == 0:
+            continue
+        idx = np.searchsorted(cond_cdf, rng.uniform(size=len(positions)))
+        result[positions] ^= pats[idx]
+    return result

@@ -349,6 +349,12 @@ __device__ __forceinline__ uint32_t get_param_bit(const uint32_t (&pv)[WL], int 
   return (x >> (pos & 31)) & 1u;
 }
 
+// four dwords at a dword-aligned address (gfx950 loads them in one
+// instruction; no 16-byte alignment is assumed)
+struct __attribute__((aligned(4))) U32x4 {
+  uint32_t v[4];
+};
+
 // Device view of a noise model (tables built by build_noise, noise.hpp).
 struct NoiseDev {
   const uint64_t *skip;    // C + 1
@@ -511,20 +517,34 @@ __device__ __forceinline__ void run_shot(const ShotArgs &a, int comp, int64_t s,
                });
   } else if (fsel0 != 0xffffffffu &&
              (((uint32_t)a.num_f | fsel0 | (uint32_t)(uintptr_t)a.f) & 3u) == 0) {
-    // contiguous selection on a dword-aligned row: 4 f bytes per load,
-    // bit 0 of each byte gathered by one multiply (bits 24..27 of the product)
-    const uint32_t *row4 = (const uint32_t *)(frow + fsel0);
+    // contiguous selection on a dword-aligned row: 16 f bytes per load,
+    // bit 0 of each byte gathered by one multiply per dword (bits 24..27 of
+    // the product)
+    const uint8_t *base = frow + fsel0;
 #pragma unroll
     for (int w = 0; w < WL; ++w) {
       uint32_t word = 0;
-      const int j0 = 32 * w;
-      const int jn = min(32, F - j0);
-      int jj = 0;
-      for (; jj + 4 <= jn; jj += 4) {
-        const uint32_t x = row4[(j0 + jj) >> 2] & 0x01010101u;
-        word |= ((x * 0x01020408u) >> 24) << jj;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int j = 32 * w + 16 * h;
+        if (j + 16 <= F) {
+          const U32x4 q = *(const U32x4 *)(base + j);
+          uint32_t b = 0;
+#pragma unroll
+          for (int k = 0; k < 4; ++k) b |= (((q.v[k] & 0x01010101u) * 0x01020408u) >> 24) << (4 * k);
+          word |= b << (16 * h);
+        } else {
+          for (int jj = max(j, 0); jj < min(j + 16, F); ++jj) {
+            if (((jj & 3) == 0) && jj + 4 <= F) {
+              const uint32_t x = *(const uint32_t *)(base + jj) & 0x01010101u;
+              word |= ((x * 0x01020408u) >> 24) << (jj & 31);
+              jj += 3;
+            } else {
+              word |= (uint32_t)(base[jj] & 1u) << (jj & 31);
+            }
+          }
+        }
       }
-      for (; jj < jn; ++jj) word |= (uint32_t)(frow[fsel0 + j0 + jj] & 1u) << jj;
       pv[w] = word;
     }
   } else {

@@ -23,6 +23,10 @@
 
 using namespace ksim;
 
+namespace ksim {
+__global__ void k_init_mag(const int4 *tab, double *mag);
+}
+
 namespace {
 
 thread_local std::string g_last_error;
@@ -63,6 +67,8 @@ int guarded(F &&f) {
 struct DevTables {
   int4 *tab = nullptr;
   int4 *pitab = nullptr;
+  double *mag = nullptr;
+  ksim::Tables view() const { return ksim::Tables{tab, pitab, mag}; }
 };
 std::mutex g_tab_mu;
 std::map<int, DevTables> g_tabs;
@@ -82,6 +88,11 @@ DevTables tables_for_current_device() {
   hip_check(hipMemcpy(t.pitab, pi_table(), sizeof(int4) * PI_MAX * PI_MAX,
                       hipMemcpyHostToDevice),
             "upload pitab");
+  hip_check(hipMalloc(&t.mag, sizeof(double) * TAB_N), "hipMalloc(mag)");
+  hipLaunchKernelGGL(ksim::k_init_mag, dim3((TAB_N + 255) / 256), dim3(256), 0, nullptr,
+                     (const int4 *)t.tab, t.mag);
+  hip_check(hipGetLastError(), "k_init_mag");
+  hip_check(hipDeviceSynchronize(), "build magnitude table");
   g_tabs[dev] = t;
   return t;
 }
@@ -118,16 +129,21 @@ __global__ void k_fill_u8(uint8_t *p, uint8_t v, int64_t n) {
   if (i < n) p[i] = v;
 }
 
+// the magnitude table: magnitude() of every materialisation-table entry
+__global__ void k_init_mag(const int4 *tab, double *mag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < TAB_N) mag[i] = table_magnitude(tab, i);
+}
+
 // Plain path: one lane per shot, grid (ceil(B/256), n_comp).  NOISE: the
 // f bits are drawn in the prologue instead of read.
 template <int WL, bool NOISE = false>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KSIM_WAVES_PER_EU))) void k_sample(ShotArgs a, const int4 *__restrict__ gtab,
-                                                  const int4 *__restrict__ pitab) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KSIM_WAVES_PER_EU))) void k_sample(ShotArgs a, Tables tb) {
   __shared__ LdsTables lt;
-  load_tables(&lt, gtab, pitab);
+  load_tables(&lt, tb);
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= a.B) return;
-  run_shot<WL, NOISE>(a, (int)blockIdx.y, s, &lt, gtab);
+  run_shot<WL, NOISE>(a, (int)blockIdx.y, s, &lt, tb);
 }
 
 // Noise rows: lane per shot writes its f row (B, f_cols) u8, columns
@@ -178,10 +194,9 @@ struct EvalArgs {
 };
 
 template <int WL>
-__global__ __launch_bounds__(kBlock) void k_eval(EvalArgs a, const int4 *__restrict__ gtab,
-                                                const int4 *__restrict__ pitab) {
+__global__ __launch_bounds__(kBlock) void k_eval(EvalArgs a, Tables tb) {
   __shared__ LdsTables lt;
-  load_tables(&lt, gtab, pitab);
+  load_tables(&lt, tb);
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= a.B) return;
   const uint8_t *row = a.params + s * (int64_t)a.P;
@@ -193,11 +208,11 @@ __global__ __launch_bounds__(kBlock) void k_eval(EvalArgs a, const int4 *__restr
     for (int jj = 0; jj < jn; ++jj) word |= (uint32_t)(row[j0 + jj] & 1u) << jj;
     pv[w] = word;
   }
-  Acc4 acc;
   uint32_t err = 0;
-  eval_graph_any<WL>(as_const(a.prog), as_const(a.gh), pv, &lt, gtab, acc, err);
   const int e0 = (int)a.gh[GH_E0];
   if (a.mode == 0) {
+    Acc4 acc;
+    eval_graph_any<WL>(as_const(a.prog), as_const(a.gh), pv, &lt, tb, acc, err);
     int64_t *c = a.coeffs + s * 8;
     const i128_t v[4] = {acc.a, acc.b, acc.c, acc.d};
     for (int k = 0; k < 4; ++k) {
@@ -206,7 +221,8 @@ __global__ __launch_bounds__(kBlock) void k_eval(EvalArgs a, const int4 *__restr
     }
     a.exp2[s] = e0;
   } else {
-    a.absout[s] = magnitude(acc, e0);
+    // the sampler's own probability path (table magnitudes, 64-bit sums)
+    a.absout[s] = graph_prob_any<WL>(as_const(a.prog), as_const(a.gh), pv, &lt, tb, err);
   }
   if (err) atomicOr(a.err, err);
 }
@@ -240,10 +256,9 @@ __global__ void k_reset_cursor(uint32_t *cursor, uint32_t *clear) {
 template <int WL>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KSIM_WAVES_PER_EU))) void k_batch(
     const BatchProg *__restrict__ progs, const BatchItem *__restrict__ items, uint32_t n_items,
-    uint32_t *cursor, uint32_t *err, const int4 *__restrict__ gtab,
-    const int4 *__restrict__ pitab) {
+    uint32_t *cursor, uint32_t *err, Tables tb) {
   __shared__ LdsTables lt;
-  load_tables(&lt, gtab, pitab);
+  load_tables(&lt, tb);
   const int lane = threadIdx.x & 63;
   for (;;) {
     uint32_t idx = 0;
@@ -260,7 +275,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KSIM_WAV
     a.noise = NoiseDev{};
     for (int sub = 0; sub < it.count; sub += 64) {
       const int64_t s = it.shot0 + sub + lane;
-      if (sub + lane < it.count && s < bp.B) run_shot<WL>(a, it.comp, s, &lt, gtab);
+      if (sub + lane < it.count && s < bp.B) run_shot<WL>(a, it.comp, s, &lt, tb);
     }
   }
 }
@@ -352,14 +367,14 @@ void launch_sample(ksim_program *p, const uint8_t *d_f, int64_t B, int32_t f_col
   dim3 grid((unsigned)nb, (unsigned)p->n_comp);
   const bool nz = a.noise.C > 0;
   switch (p->wl * 2 + (nz ? 1 : 0)) {
-    case 2: hipLaunchKernelGGL((k_sample<1, false>), grid, dim3(kBlock), 0, st, a, t.tab, t.pitab); break;
-    case 3: hipLaunchKernelGGL((k_sample<1, true>), grid, dim3(kBlock), 0, st, a, t.tab, t.pitab); break;
-    case 4: hipLaunchKernelGGL((k_sample<2, false>), grid, dim3(kBlock), 0, st, a, t.tab, t.pitab); break;
-    case 5: hipLaunchKernelGGL((k_sample<2, true>), grid, dim3(kBlock), 0, st, a, t.tab, t.pitab); break;
-    case 8: hipLaunchKernelGGL((k_sample<4, false>), grid, dim3(kBlock), 0, st, a, t.tab, t.pitab); break;
-    case 9: hipLaunchKernelGGL((k_sample<4, true>), grid, dim3(kBlock), 0, st, a, t.tab, t.pitab); break;
-    case 16: hipLaunchKernelGGL((k_sample<8, false>), grid, dim3(kBlock), 0, st, a, t.tab, t.pitab); break;
-    default: hipLaunchKernelGGL((k_sample<8, true>), grid, dim3(kBlock), 0, st, a, t.tab, t.pitab); break;
+    case 2: hipLaunchKernelGGL((k_sample<1, false>), grid, dim3(kBlock), 0, st, a, t.view()); break;
+    case 3: hipLaunchKernelGGL((k_sample<1, true>), grid, dim3(kBlock), 0, st, a, t.view()); break;
+    case 4: hipLaunchKernelGGL((k_sample<2, false>), grid, dim3(kBlock), 0, st, a, t.view()); break;
+    case 5: hipLaunchKernelGGL((k_sample<2, true>), grid, dim3(kBlock), 0, st, a, t.view()); break;
+    case 8: hipLaunchKernelGGL((k_sample<4, false>), grid, dim3(kBlock), 0, st, a, t.view()); break;
+    case 9: hipLaunchKernelGGL((k_sample<4, true>), grid, dim3(kBlock), 0, st, a, t.view()); break;
+    case 16: hipLaunchKernelGGL((k_sample<8, false>), grid, dim3(kBlock), 0, st, a, t.view()); break;
+    default: hipLaunchKernelGGL((k_sample<8, true>), grid, dim3(kBlock), 0, st, a, t.view()); break;
   }
   hip_check(hipGetLastError(), "k_sample launch");
 }
@@ -612,10 +627,10 @@ static void eval_host(ksim_program *p, int32_t comp, int32_t graph, const uint8_
   reset_err_async(p, st);
   dim3 grid((unsigned)((B + kBlock - 1) / kBlock));
   switch (wl) {
-    case 1: hipLaunchKernelGGL(k_eval<1>, grid, dim3(kBlock), 0, st, a, t.tab, t.pitab); break;
-    case 2: hipLaunchKernelGGL(k_eval<2>, grid, dim3(kBlock), 0, st, a, t.tab, t.pitab); break;
-    case 4: hipLaunchKernelGGL(k_eval<4>, grid, dim3(kBlock), 0, st, a, t.tab, t.pitab); break;
-    default: hipLaunchKernelGGL(k_eval<8>, grid, dim3(kBlock), 0, st, a, t.tab, t.pitab); break;
+    case 1: hipLaunchKernelGGL(k_eval<1>, grid, dim3(kBlock), 0, st, a, t.view()); break;
+    case 2: hipLaunchKernelGGL(k_eval<2>, grid, dim3(kBlock), 0, st, a, t.view()); break;
+    case 4: hipLaunchKernelGGL(k_eval<4>, grid, dim3(kBlock), 0, st, a, t.view()); break;
+    default: hipLaunchKernelGGL(k_eval<8>, grid, dim3(kBlock), 0, st, a, t.view()); break;
   }
   hip_check(hipGetLastError(), "k_eval launch");
   if (mode == 0) {
@@ -868,7 +883,7 @@ static void batch_enqueue(ksim_batch *b, int blocks, hipStream_t st) {
   hip_check(hipGetLastError(), "k_reset_cursor");
 #define KSIM_BATCH_LAUNCH(W)                                                                  \
   hipLaunchKernelGGL(k_batch<W>, dim3(blocks), dim3(kBlock), 0, st, b->d_progs, b->d_items,    \
-                     b->n_items, b->d_cursor, b->d_cursor + 1, t.tab, t.pitab)
+                     b->n_items, b->d_cursor, b->d_cursor + 1, t.view())
   switch (b->wl) {
     case 1: KSIM_BATCH_LAUNCH(1); break;
     case 2: KSIM_BATCH_LAUNCH(2); break;
@@ -1045,7 +1060,7 @@ int ksim_sched_launch(ksim_sched *q, const int32_t *items, int32_t n, int32_t bl
     hip_check(hipGetLastError(), "k_reset_cursor");
 #define KSIM_SCHED_LAUNCH(W)                                                                   \
   hipLaunchKernelGGL(k_batch<W>, dim3(blocks), dim3(kBlock), 0, st, q->d_progs[k],              \
-                     q->d_items[k], (uint32_t)n, cursor, q->d_cursor + 64, t.tab, t.pitab)
+                     q->d_items[k], (uint32_t)n, cursor, q->d_cursor + 64, t.view())
     switch (lane_words(wl)) {
       case 1: KSIM_SCHED_LAUNCH(1); break;
       case 2: KSIM_SCHED_LAUNCH(2); break;

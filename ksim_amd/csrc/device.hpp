@@ -142,7 +142,127 @@ __device__ __forceinline__ Acc4 add_residual_term(Acc4 acc, int64_t x0, int64_t 
 struct LdsTables {
   int4 tab[TAB_LDS_N];          // w^s (1+w)^r (1+sqrt2)^n, |n| <= TAB_LDS_R
   int4 pi[PI_MAX * PI_MAX];     // pi^a pibar^b
+  double mag[MAG_LDS_N];        // magnitude() of the entries with |n| <= MAG_LDS_R
 };
+
+// Per-device read-only tables the kernels take.
+struct Tables {
+  const int4 *tab;     // TAB_DIGITS planes of TAB_N entries
+  const int4 *pi;      // PI_MAX x PI_MAX
+  const double *mag;   // TAB_N: magnitude() of each (single-plane-summed) entry
+};
+
+// XOR over the lane's parameter words of (word & mask word): its popcount
+// has the parity of <params, mask> (one AND and W32-1 AND-XORs, one popcount)
+template <int W32, int WL>
+__device__ __forceinline__ uint32_t masked_xor(const uint32_t (&pv)[WL],
+                                               const cword *__restrict__ m) {
+  uint32_t t = pv[0] & m[0];
+#pragma unroll
+  for (int w = 1; w < W32; ++w) t ^= pv[w] & m[w];
+  return t;
+}
+
+// -------------------------------------------------------------- one term
+// Exponents of one term for this lane's parameter words pv[0..W32):
+// the term is w^S (1+w)^V (1+sqrt2)^N pi^A pibar^B [R] 2^E, or zero when
+// bit 0 of `dead` is set.  th points at the term header.
+template <int W32, int WL>
+__device__ __forceinline__ void term_exps(const cword *__restrict__ th,
+                                          const uint32_t (&pv)[WL], int &S, int &V, int &N,
+                                          int &A, int &Bq, uint32_t &dead) {
+  static_assert(W32 <= WL, "graph wider than the lane's parameter words");
+  const uint32_t nproj = th[TH_NPROJ], nppx = th[TH_NPPX], nchunk = th[TH_NCHUNK];
+  S = (int)th[TH_S]; V = (int)th[TH_V]; N = (int)th[TH_N];
+  A = (int)th[TH_A]; Bq = (int)th[TH_B];
+  const cword *__restrict__ rp = th + TH_DWORDS;
+  // PROJ: the term vanishes unless every projector parity matches
+  dead = 0;
+  for (uint32_t k = 0; k < nproj; ++k, rp += W32 + 1) {
+    dead |= rp[W32] + __popc(masked_xor<W32>(pv, rp));
+  }
+  // PPX: (-1)^{psi phi}
+  uint32_t ex = 0;
+  for (uint32_t k = 0; k < nppx; ++k, rp += 2 * W32 + 2) {
+    const uint32_t cs = rp[2 * W32] + __popc(masked_xor<W32>(pv, rp));
+    const uint32_t cf = rp[2 * W32 + 1] + __popc(masked_xor<W32>(pv, rp + W32));
+    ex ^= cs & cf;
+  }
+  S += (int)((ex & 1u) << 2);
+  // LIN / PAIR chunks: packed exponent adds
+  for (uint32_t ch = 0; ch < nchunk; ++ch) {
+    const uint32_t nl = rp[0], npr = rp[1];
+    uint32_t x = rp[2];
+    rp += CHUNK_HDR;
+    // LIN records, unrolled so each scalar load fetches several records
+#pragma unroll 4
+    for (uint32_t k = 0; k < nl; ++k, rp += W32 + 1) {
+      const uint32_t c = __popc(masked_xor<W32>(pv, rp));
+      x += (0u - (c & 1u)) & rp[W32];
+    }
+    for (uint32_t k = 0; k < npr; ++k, rp += 2 * W32 + 3) {
+      const uint32_t ca = __popc(masked_xor<W32>(pv, rp));
+      const uint32_t cb = __popc(masked_xor<W32>(pv, rp + W32));
+      const uint32_t ma = 0u - (ca & 1u), mb = 0u - (cb & 1u);
+      x += (ma & rp[2 * W32]) + (mb & rp[2 * W32 + 1]) + (ma & mb & rp[2 * W32 + 2]);
+    }
+    V += (int)((x >> PK_OV) & ((1u << PK_WV) - 1));
+    N += (int)((x >> PK_ON) & ((1u << PK_WN) - 1));
+    A += (int)((x >> PK_OA) & ((1u << PK_WA) - 1));
+    Bq += (int)((x >> PK_OB) & ((1u << PK_WB) - 1));
+    S += (int)(x >> PK_OS);
+  }
+}
+
+// materialise w^S (1+w)^V (1+sqrt2)^N pi^A pibar^B R:
+// (1+w)^4 = 2 w^2 (1+sqrt2)^2, so with q = V>>2, r = V&3 the value is
+// 2^q w^{S+2q} (1+w)^r (1+sqrt2)^{N+2q} pi^A pibar^B R.  Table row n':
+// the LDS copy for the central rows, else the global table, whose rows
+// beyond TAB_R spread over TAB_DIGITS planes of base-2^31 digits (each
+// < 2^30); digit d adds at shift sh + 31 d.
+struct TermIdx {
+  int q, r, n2, s2, gi;
+  bool in_tab;
+};
+__device__ __forceinline__ TermIdx term_index(int S, int V, int N) {
+  TermIdx t;
+  t.q = V >> 2;
+  t.r = V & 3;
+  t.n2 = N + 2 * t.q;
+  t.s2 = (S + 2 * t.q) & 7;
+  t.gi = ((t.n2 + TAB_W) * 4 + t.r) * 8 + t.s2;
+  t.in_tab = (unsigned)(t.n2 + TAB_W) <= 2u * TAB_W;
+  return t;
+}
+
+__device__ __forceinline__ int4 table_digit0(const TermIdx &ti, bool vanished, const LdsTables *lt,
+                                             const Tables &tb, uint32_t &err) {
+  if ((unsigned)(ti.n2 + TAB_LDS_R) <= 2u * TAB_LDS_R)
+    return lt->tab[((ti.n2 + TAB_LDS_R) * 4 + ti.r) * 8 + ti.s2];
+  if (ti.in_tab) return tb.tab[ti.gi];
+  if (!vanished) err |= ERR_TAB_RANGE;
+  return make_int4(0, 0, 0, 0);
+}
+
+// e * pi^A pibar^B for a table digit e (|e| < 2^30; pi powers < 2^24): four
+// signed 32 x 32 -> 64-bit products per coefficient, exact in int64
+__device__ __forceinline__ void times_pi(int4 e, uint32_t flags, int A, int Bq,
+                                         const LdsTables *lt, int64_t &x0, int64_t &x1,
+                                         int64_t &x2, int64_t &x3, uint32_t &err) {
+  if ((flags & TF_PI) && (A | Bq)) {
+    if (A >= PI_MAX || Bq >= PI_MAX) {
+      err |= ERR_PI_RANGE;
+      e = make_int4(0, 0, 0, 0);
+    }
+    const int4 p = lt->pi[min(A, PI_MAX - 1) * PI_MAX + min(Bq, PI_MAX - 1)];
+    x0 = (int64_t)e.x * p.x + (int64_t)e.y * p.w - (int64_t)e.z * p.z + (int64_t)e.w * p.y;
+    x1 = (int64_t)e.x * p.y + (int64_t)e.y * p.x + (int64_t)e.z * p.w + (int64_t)e.w * p.z;
+    x2 = (int64_t)e.x * p.z + (int64_t)e.y * p.y + (int64_t)e.z * p.x - (int64_t)e.w * p.w;
+    x3 = (int64_t)e.x * p.w - (int64_t)e.y * p.z - (int64_t)e.z * p.y + (int64_t)e.w * p.x;
+  } else {
+    x0 = e.x; x1 = e.y; x2 = e.z; x3 = e.w;
+  }
+}
 
 // acc + e * pi^{A,B} * R * 2^sh for one table digit e (|e| < 2^30).
 // zero: the lane adds nothing (a vanished term, or no digit at this place).
@@ -155,21 +275,7 @@ __device__ __forceinline__ Acc4 add_term_digit(Acc4 acc, int4 e, bool zero, uint
   const int32_t keep = (int32_t)zero - 1;
   e.x &= keep; e.y &= keep; e.z &= keep; e.w &= keep;
   int64_t x0, x1, x2, x3;
-  if ((flags & TF_PI) && (A | Bq)) {
-    if (A >= PI_MAX || Bq >= PI_MAX) {
-      err |= ERR_PI_RANGE;
-      e = make_int4(0, 0, 0, 0);
-    }
-    // table digit (< 2^30) x pi power (< 2^24): signed 32 x 32 -> 64
-    // products, four per coefficient, exact in int64
-    const int4 p = lt->pi[min(A, PI_MAX - 1) * PI_MAX + min(Bq, PI_MAX - 1)];
-    x0 = (int64_t)e.x * p.x + (int64_t)e.y * p.w - (int64_t)e.z * p.z + (int64_t)e.w * p.y;
-    x1 = (int64_t)e.x * p.y + (int64_t)e.y * p.x + (int64_t)e.z * p.w + (int64_t)e.w * p.z;
-    x2 = (int64_t)e.x * p.z + (int64_t)e.y * p.y + (int64_t)e.z * p.x - (int64_t)e.w * p.w;
-    x3 = (int64_t)e.x * p.w - (int64_t)e.y * p.z - (int64_t)e.z * p.y + (int64_t)e.w * p.x;
-  } else {
-    x0 = e.x; x1 = e.y; x2 = e.z; x3 = e.w;
-  }
+  times_pi(e, flags, A, Bq, lt, x0, x1, x2, x3, err);
   if (sh < 0 || sh > SHIFT_MAX) {
     if (x0 | x1 | x2 | x3) err |= ERR_SHIFT;
     x0 = x1 = x2 = x3 = 0;
@@ -201,140 +307,192 @@ __device__ __forceinline__ Acc4 add_term_digit(Acc4 acc, int4 e, bool zero, uint
   return acc;
 }
 
-// XOR over the lane's parameter words of (word & mask word): its popcount
-// has the parity of <params, mask> (one AND and W32-1 AND-XORs, one popcount)
-template <int W32, int WL>
-__device__ __forceinline__ uint32_t masked_xor(const uint32_t (&pv)[WL],
-                                               const cword *__restrict__ m) {
-  uint32_t t = pv[0] & m[0];
-#pragma unroll
-  for (int w = 1; w < W32; ++w) t ^= pv[w] & m[w];
-  return t;
+// Four signed 64-bit sums: the exact term sum of a GF_NARROW graph, whose
+// packer bound keeps every partial sum, shifted term and product below
+// 2^62 in magnitude.
+struct Acc64 {
+  int64_t a, b, c, d;
+};
+
+__device__ __forceinline__ void add_term_narrow(Acc64 &n, int4 e, bool zero, uint32_t flags,
+                                                int A, int Bq, const LdsTables *lt,
+                                                const cword *__restrict__ th, int sh,
+                                                uint32_t &err) {
+  const int32_t keep = (int32_t)zero - 1;
+  e.x &= keep; e.y &= keep; e.z &= keep; e.w &= keep;
+  int64_t x0, x1, x2, x3;
+  times_pi(e, flags, A, Bq, lt, x0, x1, x2, x3, err);
+  if (flags & TF_RESIDUAL)
+    zmul64(x0, x1, x2, x3, (int32_t)th[TH_R0], (int32_t)th[TH_R1], (int32_t)th[TH_R2],
+           (int32_t)th[TH_R3]);
+  if (sh < 0 || sh > 62) {
+    if (x0 | x1 | x2 | x3) err |= ERR_SHIFT;
+    x0 = x1 = x2 = x3 = 0;
+    sh = 0;
+  }
+  n.a += (int64_t)((uint64_t)x0 << sh);
+  n.b += (int64_t)((uint64_t)x1 << sh);
+  n.c += (int64_t)((uint64_t)x2 << sh);
+  n.d += (int64_t)((uint64_t)x3 << sh);
 }
 
-// -------------------------------------------------------------- one graph
-// Exact amplitude of one graph for this lane's parameter words pv[0..W32):
-// acc = sum_terms value * 2^{-e0}; the caller scales by 2^{GH_E0}.
+
+// Running state of a graph's term sum.
+struct GraphSum {
+  Acc4 acc;     // 128-bit sums (graphs without GF_NARROW)
+  Acc64 n;      // 64-bit sums (GF_NARROW)
+  double pm;    // the float64 magnitude (GF_MONO)
+};
+
+__device__ __forceinline__ void graph_sum_clear(GraphSum &g) {
+  g.acc.a = g.acc.b = g.acc.c = g.acc.d = 0;
+  g.n.a = g.n.b = g.n.c = g.n.d = 0;
+  g.pm = 0.0;
+}
+
+// Add terms [0, nt) starting at program dword `th` into g, by the graph's
+// flags: a single table magnitude (GF_MONO), 64-bit sums (GF_NARROW) or
+// 128-bit sums.
 template <int W32, int WL>
-__device__ __forceinline__ void eval_graph(const cword *__restrict__ P,
-                                           const cword *__restrict__ gh,
-                                           const uint32_t (&pv)[WL], const LdsTables *lt,
-                                           const int4 *__restrict__ gtab, Acc4 &acc,
-                                           uint32_t &err) {
-  static_assert(W32 <= WL, "graph wider than the lane's parameter words");
-  acc.a = acc.b = acc.c = acc.d = 0;
-  const uint32_t nterms = gh[GH_NTERMS];
-  const cword *__restrict__ th = P + gh[GH_TERM_OFF];
-  for (uint32_t t = 0; t < nterms; ++t) {
-    const uint32_t nproj = th[TH_NPROJ], nppx = th[TH_NPPX], nchunk = th[TH_NCHUNK];
+__device__ __forceinline__ void sum_terms(const cword *__restrict__ th, uint32_t nt,
+                                          uint32_t gflags, int e0, const uint32_t (&pv)[WL],
+                                          const LdsTables *lt, const Tables &tb, GraphSum &g,
+                                          uint32_t &err) {
+  for (uint32_t t = 0; t < nt; ++t) {
     const uint32_t flags = th[TH_FLAGS];
-    int S = (int)th[TH_S], V = (int)th[TH_V], N = (int)th[TH_N];
-    int A = (int)th[TH_A], Bq = (int)th[TH_B];
     const int E = (int)th[TH_E];
-    const cword *__restrict__ rp = th + TH_DWORDS;
-    // PROJ: the term vanishes unless every projector parity matches
-    uint32_t dead = 0;
-    for (uint32_t k = 0; k < nproj; ++k, rp += W32 + 1) {
-      dead |= rp[W32] + __popc(masked_xor<W32>(pv, rp));
-    }
-    // PPX: (-1)^{psi phi}
-    uint32_t ex = 0;
-    for (uint32_t k = 0; k < nppx; ++k, rp += 2 * W32 + 2) {
-      const uint32_t cs = rp[2 * W32] + __popc(masked_xor<W32>(pv, rp));
-      const uint32_t cf = rp[2 * W32 + 1] + __popc(masked_xor<W32>(pv, rp + W32));
-      ex ^= cs & cf;
-    }
-    S += (int)((ex & 1u) << 2);
-    // LIN / PAIR chunks: packed exponent adds
-    for (uint32_t ch = 0; ch < nchunk; ++ch) {
-      const uint32_t nl = rp[0], npr = rp[1];
-      uint32_t x = rp[2];
-      rp += CHUNK_HDR;
-      // LIN records, unrolled so each scalar load fetches several records
-#pragma unroll 4
-      for (uint32_t k = 0; k < nl; ++k, rp += W32 + 1) {
-        const uint32_t c = __popc(masked_xor<W32>(pv, rp));
-        x += (0u - (c & 1u)) & rp[W32];
-      }
-      for (uint32_t k = 0; k < npr; ++k, rp += 2 * W32 + 3) {
-        const uint32_t ca = __popc(masked_xor<W32>(pv, rp));
-        const uint32_t cb = __popc(masked_xor<W32>(pv, rp + W32));
-        const uint32_t ma = 0u - (ca & 1u), mb = 0u - (cb & 1u);
-        x += (ma & rp[2 * W32]) + (mb & rp[2 * W32 + 1]) + (ma & mb & rp[2 * W32 + 2]);
-      }
-      V += (int)((x >> PK_OV) & ((1u << PK_WV) - 1));
-      N += (int)((x >> PK_ON) & ((1u << PK_WN) - 1));
-      A += (int)((x >> PK_OA) & ((1u << PK_WA) - 1));
-      Bq += (int)((x >> PK_OB) & ((1u << PK_WB) - 1));
-      S += (int)(x >> PK_OS);
-    }
-    // materialise w^S (1+w)^V (1+sqrt2)^N pi^A pibar^B R:
-    // (1+w)^4 = 2 w^2 (1+sqrt2)^2, so with q = V>>2, r = V&3 the value is
-    // 2^q w^{S+2q} (1+w)^r (1+sqrt2)^{N+2q} pi^A pibar^B R.
-    const int q = V >> 2, r = V & 3;
-    const int n2 = N + 2 * q;
-    const int s2 = (S + 2 * q) & 7;
-    // table row n': the LDS copy for the central rows, else the global
-    // table, whose rows beyond TAB_R spread over TAB_DIGITS planes of
-    // base-2^31 digits (each < 2^30); digit d adds at shift sh + 31 d.
+    int S, V, N, A, Bq;
+    uint32_t dead;
+    term_exps<W32, WL>(th, pv, S, V, N, A, Bq, dead);
+    const TermIdx ti = term_index(S, V, N);
     const bool vanished = dead & 1u;
-    const int gi = ((n2 + TAB_W) * 4 + r) * 8 + s2;
-    const bool in_tab = (unsigned)(n2 + TAB_W) <= 2u * TAB_W;
-    int4 e;
-    if ((unsigned)(n2 + TAB_LDS_R) <= 2u * TAB_LDS_R) {
-      e = lt->tab[((n2 + TAB_LDS_R) * 4 + r) * 8 + s2];
-    } else if (in_tab) {
-      e = gtab[gi];
+    const int sh = ti.q + E;
+    if (gflags & GF_MONO) {
+      // |value| = magnitude(table entry) * 2^(sh + e0) exactly: scaling by
+      // a power of two commutes with every step of magnitude(), and the
+      // entry is below 2^31, so the 62-bit window never truncates it
+      double m;
+      if ((unsigned)(ti.n2 + MAG_LDS_R) <= 2u * MAG_LDS_R) {
+        m = lt->mag[((ti.n2 + MAG_LDS_R) * 4 + ti.r) * 8 + ti.s2];
+      } else if (ti.in_tab) {
+        m = tb.mag[ti.gi];
+      } else {
+        if (!vanished) err |= ERR_TAB_RANGE;
+        m = 0.0;
+      }
+      g.pm = vanished ? 0.0 : __builtin_ldexp(m, sh + e0);
     } else {
-      if (!vanished) err |= ERR_TAB_RANGE;
-      e = make_int4(0, 0, 0, 0);
-    }
-    const int sh = q + E;
-    acc = add_term_digit<true>(acc, e, vanished, flags, A, Bq, lt, th, sh, err);
-    const bool wide = in_tab && !vanished && (unsigned)(n2 + TAB_R) > 2u * TAB_R;
-    if (__builtin_amdgcn_ballot_w64(wide)) {
-      // a rare tail: some lane's |n'| exceeds the single-digit rows
+      const int4 e = table_digit0(ti, vanished, lt, tb, err);
+      if (gflags & GF_NARROW) {
+        add_term_narrow(g.n, e, vanished, flags, A, Bq, lt, th, sh, err);
+      } else {
+        g.acc = add_term_digit<true>(g.acc, e, vanished, flags, A, Bq, lt, th, sh, err);
+        const bool wide = ti.in_tab && !vanished && (unsigned)(ti.n2 + TAB_R) > 2u * TAB_R;
+        if (__builtin_amdgcn_ballot_w64(wide)) {
+          // a rare tail: some lane's |n'| exceeds the single-digit rows
 #pragma nounroll
-      for (int d = 1; d < TAB_DIGITS; ++d) {
-        const int4 ed = wide ? gtab[d * TAB_N + gi] : make_int4(0, 0, 0, 0);
-        acc = add_term_digit<false>(acc, ed, !wide, flags, A, Bq, lt, th, sh + TAB_DIGIT_BITS * d, err);
+          for (int d = 1; d < TAB_DIGITS; ++d) {
+            const int4 ed = wide ? tb.tab[d * TAB_N + ti.gi] : make_int4(0, 0, 0, 0);
+            g.acc = add_term_digit<false>(g.acc, ed, !wide, flags, A, Bq, lt, th,
+                                          sh + TAB_DIGIT_BITS * d, err);
+          }
+        }
       }
     }
     th = th + TH_DWORDS + th[TH_SIZE];
   }
 }
 
+__device__ __forceinline__ Acc4 widen(const Acc64 &n) {
+  Acc4 a;
+  a.a = n.a; a.b = n.b; a.c = n.c; a.d = n.d;
+  return a;
+}
+
+// |amplitude| of one graph (float64) for this lane's parameter words: the
+// value magnitude() gives on the exact term sum.
+template <int W32, int WL>
+__device__ __forceinline__ double graph_prob(const cword *__restrict__ P,
+                                             const cword *__restrict__ gh,
+                                             const uint32_t (&pv)[WL], const LdsTables *lt,
+                                             const Tables &tb, uint32_t &err) {
+  const uint32_t gflags = gh[GH_FLAGS];
+  const int e0 = (int)gh[GH_E0];
+  GraphSum g;
+  graph_sum_clear(g);
+  sum_terms<W32, WL>(P + gh[GH_TERM_OFF], gh[GH_NTERMS], gflags, e0, pv, lt, tb, g, err);
+  if (gflags & GF_MONO) return g.pm;
+  if (gflags & GF_NARROW) return magnitude(widen(g.n), e0);
+  return magnitude(g.acc, e0);
+}
+
+template <int WL>
+__device__ __forceinline__ double graph_prob_any(const cword *__restrict__ P,
+                                                 const cword *__restrict__ gh,
+                                                 const uint32_t (&pv)[WL], const LdsTables *lt,
+                                                 const Tables &tb, uint32_t &err) {
+  switch (gh[GH_W32]) {
+    case 1: return graph_prob<1, WL>(P, gh, pv, lt, tb, err);
+    case 2: if constexpr (WL >= 2) return graph_prob<2, WL>(P, gh, pv, lt, tb, err); break;
+    case 3: if constexpr (WL >= 3) return graph_prob<3, WL>(P, gh, pv, lt, tb, err); break;
+    case 4: if constexpr (WL >= 4) return graph_prob<4, WL>(P, gh, pv, lt, tb, err); break;
+    case 5: if constexpr (WL >= 5) return graph_prob<5, WL>(P, gh, pv, lt, tb, err); break;
+    case 6: if constexpr (WL >= 6) return graph_prob<6, WL>(P, gh, pv, lt, tb, err); break;
+    case 7: if constexpr (WL >= 7) return graph_prob<7, WL>(P, gh, pv, lt, tb, err); break;
+    case 8: if constexpr (WL >= 8) return graph_prob<8, WL>(P, gh, pv, lt, tb, err); break;
+    default: break;
+  }
+  return 0.0;
+}
+
+// Exact amplitude of one graph for explicit parameter words (the exact
+// path of k_eval): acc = sum_terms value * 2^{-e0} in 128 bits, every term
+// through the general path whatever the graph's flags.
+template <int W32, int WL>
+__device__ __forceinline__ void eval_graph(const cword *__restrict__ P,
+                                           const cword *__restrict__ gh,
+                                           const uint32_t (&pv)[WL], const LdsTables *lt,
+                                           const Tables &tb, Acc4 &acc, uint32_t &err) {
+  GraphSum g;
+  graph_sum_clear(g);
+  sum_terms<W32, WL>(P + gh[GH_TERM_OFF], gh[GH_NTERMS], 0u, (int)gh[GH_E0], pv, lt, tb, g, err);
+  acc = g.acc;
+}
+
 template <int WL>
 __device__ __forceinline__ void eval_graph_any(const cword *__restrict__ P,
                                                const cword *__restrict__ gh,
                                                const uint32_t (&pv)[WL], const LdsTables *lt,
-                                               const int4 *__restrict__ gtab, Acc4 &acc,
-                                               uint32_t &err) {
+                                               const Tables &tb, Acc4 &acc, uint32_t &err) {
   switch (gh[GH_W32]) {
-    case 1: eval_graph<1, WL>(P, gh, pv, lt, gtab, acc, err); break;
-    case 2: if constexpr (WL >= 2) eval_graph<2, WL>(P, gh, pv, lt, gtab, acc, err); break;
-    case 3: if constexpr (WL >= 3) eval_graph<3, WL>(P, gh, pv, lt, gtab, acc, err); break;
-    case 4: if constexpr (WL >= 4) eval_graph<4, WL>(P, gh, pv, lt, gtab, acc, err); break;
-    case 5: if constexpr (WL >= 5) eval_graph<5, WL>(P, gh, pv, lt, gtab, acc, err); break;
-    case 6: if constexpr (WL >= 6) eval_graph<6, WL>(P, gh, pv, lt, gtab, acc, err); break;
-    case 7: if constexpr (WL >= 7) eval_graph<7, WL>(P, gh, pv, lt, gtab, acc, err); break;
-    case 8: if constexpr (WL >= 8) eval_graph<8, WL>(P, gh, pv, lt, gtab, acc, err); break;
+    case 1: eval_graph<1, WL>(P, gh, pv, lt, tb, acc, err); break;
+    case 2: if constexpr (WL >= 2) eval_graph<2, WL>(P, gh, pv, lt, tb, acc, err); break;
+    case 3: if constexpr (WL >= 3) eval_graph<3, WL>(P, gh, pv, lt, tb, acc, err); break;
+    case 4: if constexpr (WL >= 4) eval_graph<4, WL>(P, gh, pv, lt, tb, acc, err); break;
+    case 5: if constexpr (WL >= 5) eval_graph<5, WL>(P, gh, pv, lt, tb, acc, err); break;
+    case 6: if constexpr (WL >= 6) eval_graph<6, WL>(P, gh, pv, lt, tb, acc, err); break;
+    case 7: if constexpr (WL >= 7) eval_graph<7, WL>(P, gh, pv, lt, tb, acc, err); break;
+    case 8: if constexpr (WL >= 8) eval_graph<8, WL>(P, gh, pv, lt, tb, acc, err); break;
     default: break;
   }
 }
 
-// set parameter bit `pos` of the lane's words to `bit` (pos is wave-uniform)
+// set parameter bit `pos` of the lane's words to `bit` (pos is wave-uniform:
+// a scalar branch picks the one word, so pv stays in registers)
 template <int WL>
 __device__ __forceinline__ void set_param_bit(uint32_t (&pv)[WL], int pos, uint32_t bit) {
-  // every word is rewritten with a word-selected mask (no indexed store, so
-  // pv stays in registers)
-  const int word = pos >> 5;
+  const int word = __builtin_amdgcn_readfirstlane(pos >> 5);
   const uint32_t m = 1u << (pos & 31);
   const uint32_t b = 0u - (bit & 1u);
-#pragma unroll
-  for (int w = 0; w < WL; ++w) {
-    const uint32_t mw = m & (0u - (uint32_t)(w == word));
-    pv[w] = (pv[w] & ~mw) | (mw & b);
+  switch (word) {
+#define KSIM_SET_WORD(w)                                   \
+  case w:                                                   \
+    if constexpr (w < WL) pv[w] = (pv[w] & ~m) | (m & b);   \
+    break;
+    KSIM_SET_WORD(0) KSIM_SET_WORD(1) KSIM_SET_WORD(2) KSIM_SET_WORD(3)
+    KSIM_SET_WORD(4) KSIM_SET_WORD(5) KSIM_SET_WORD(6) KSIM_SET_WORD(7)
+#undef KSIM_SET_WORD
+    default: break;
   }
 }
 
@@ -343,9 +501,16 @@ template <int WL>
 __device__ __forceinline__ uint32_t get_param_bit(const uint32_t (&pv)[WL], int pos) {
   const int word = __builtin_amdgcn_readfirstlane(pos >> 5);
   uint32_t x = 0;
-#pragma unroll
-  for (int w = 0; w < WL; ++w)
-    if (w == word) x = pv[w];
+  switch (word) {
+#define KSIM_GET_WORD(w)              \
+  case w:                             \
+    if constexpr (w < WL) x = pv[w];  \
+    break;
+    KSIM_GET_WORD(0) KSIM_GET_WORD(1) KSIM_GET_WORD(2) KSIM_GET_WORD(3)
+    KSIM_GET_WORD(4) KSIM_GET_WORD(5) KSIM_GET_WORD(6) KSIM_GET_WORD(7)
+#undef KSIM_GET_WORD
+    default: break;
+  }
   return (x >> (pos & 31)) & 1u;
 }
 
@@ -476,7 +641,7 @@ __device__ __forceinline__ void write_packed_row(const ShotArgs &a, const uint32
 // (src/cpp/ksim/kokkos_sim.cpp:214-230, src/python/ksim/sample/evaluate.py:76-85).
 template <int WL, bool NOISE = false>
 __device__ __forceinline__ void run_shot(const ShotArgs &a, int comp, int64_t s,
-                                         const LdsTables *lt, const int4 *__restrict__ gtab) {
+                                         const LdsTables *lt, const Tables &tb) {
   const cword *__restrict__ P = as_const(a.prog);
   const cword *__restrict__ ch = P + P[PH_DWORDS + comp];
   const int ng = (int)ch[CH_NGRAPHS], F = (int)ch[CH_F];
@@ -560,7 +725,6 @@ __device__ __forceinline__ void run_shot(const ShotArgs &a, int comp, int64_t s,
   // per-component stream: seed + 0x9E37*k (src/python/ksim/sample/sampler.py:63)
   uint64_t rs = rng_seed(a.seed + 0x9E37ull * (uint64_t)comp, (uint64_t)(a.shot_base + s));
   uint32_t err = 0;
-  Acc4 acc;
   double prev = 0.0;
   // graph 0 gives the normaliser, graph i+1 the probability of output i = 1
   // (one evaluation site, so the graph code is inlined once); the decided
@@ -568,8 +732,7 @@ __device__ __forceinline__ void run_shot(const ShotArgs &a, int comp, int64_t s,
   for (int i = 0; i < ng; ++i) {
     const cword *gh = ghs + (size_t)i * GH_DWORDS;
     if (i > 0) set_param_bit<WL>(pv, F + i - 1, 1u);  // trying bit
-    eval_graph_any<WL>(P, gh, pv, lt, gtab, acc, err);
-    const double p1 = magnitude(acc, (int)gh[GH_E0]);
+    const double p1 = graph_prob_any<WL>(P, gh, pv, lt, tb, err);
     if (i == 0) {
       prev = p1;
       continue;
@@ -591,12 +754,31 @@ __device__ __forceinline__ void run_shot(const ShotArgs &a, int comp, int64_t s,
   if (err) atomicOr(a.err, err);
 }
 
-// stage the central rows of the materialisation table and the pi powers
-__device__ inline void load_tables(LdsTables *t, const int4 *gtab, const int4 *gpi) {
-  const int4 *src = gtab + (TAB_W - TAB_LDS_R) * 32;
+// stage the central rows of the materialisation and magnitude tables and
+// the pi powers
+__device__ inline void load_tables(LdsTables *t, const Tables &tb) {
+  const int4 *src = tb.tab + (TAB_W - TAB_LDS_R) * 32;
   for (int i = threadIdx.x; i < TAB_LDS_N; i += blockDim.x) t->tab[i] = src[i];
-  for (int i = threadIdx.x; i < PI_MAX * PI_MAX; i += blockDim.x) t->pi[i] = gpi[i];
+  for (int i = threadIdx.x; i < PI_MAX * PI_MAX; i += blockDim.x) t->pi[i] = tb.pi[i];
+  const double *msrc = tb.mag + (TAB_W - MAG_LDS_R) * 32;
+  for (int i = threadIdx.x; i < MAG_LDS_N; i += blockDim.x) t->mag[i] = msrc[i];
   __syncthreads();
+}
+
+// magnitude() of every table entry (all digit planes summed), exponent 0:
+// the table GF_MONO graphs read their float64 magnitude from
+__device__ inline double table_magnitude(const int4 *tab, int i) {
+  Acc4 v;
+  v.a = v.b = v.c = v.d = 0;
+  for (int d = 0; d < TAB_DIGITS; ++d) {
+    const int4 e = tab[d * TAB_N + i];
+    const int sh = TAB_DIGIT_BITS * d;
+    v.a += (i128_t)((u128_t)(i128_t)e.x << sh);
+    v.b += (i128_t)((u128_t)(i128_t)e.y << sh);
+    v.c += (i128_t)((u128_t)(i128_t)e.z << sh);
+    v.d += (i128_t)((u128_t)(i128_t)e.w << sh);
+  }
+  return magnitude(v, 0);
 }
 
 }  // namespace ksim

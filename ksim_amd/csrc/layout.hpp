@@ -46,6 +46,10 @@ inline int tab_index(int n, int r, int s) { return ((n + TAB_W) * 4 + r) * 8 + s
 // read from global memory (rare)
 constexpr int TAB_LDS_R = 17;
 constexpr int TAB_LDS_N = (2 * TAB_LDS_R + 1) * 4 * 8;
+// float64 magnitudes |w^s (1+w)^r (1+sqrt2)^n| of the table entries (by the
+// sampler's magnitude sequence), same indexing; rows |n| <= MAG_LDS_R in LDS
+constexpr int MAG_LDS_R = 12;
+constexpr int MAG_LDS_N = (2 * MAG_LDS_R + 1) * 4 * 8;
 
 // pi^a pibar^b for a, b < PI_MAX (int32 coefficients, < 2^24)
 constexpr int PI_MAX = 16;
@@ -64,11 +68,25 @@ constexpr uint32_t TF_PI = 2u;        // A or B may be non-zero
 
 constexpr int CHUNK_HDR = 4;  // n_lin, n_pair, base, pad
 
-// graph header (dwords)
+// graph header (dwords).  The terms of a graph are stored as GH_PARTS parts
+// of about equal record work (the packer deals them out heaviest first), so
+// that 1, 2 or 4 waves can split a graph's term sum: with L waves, wave j
+// takes parts [j*GH_PARTS/L, (j+1)*GH_PARTS/L).  GH_PART_T[k] is the first
+// term of part k (k = 1..3; part 0 starts at term 0) and GH_PART_O[k] its
+// dword offset in the program.
+constexpr int GH_PARTS = 4;
 enum GraphHdr : int {
-  GH_NTERMS = 0, GH_W32, GH_E0, GH_TERM_OFF, GH_NREC, GH_PAD0, GH_PAD1, GH_PAD2,
+  GH_NTERMS = 0, GH_W32, GH_E0, GH_TERM_OFF, GH_NREC, GH_FLAGS,
+  GH_PART_T1, GH_PART_T2, GH_PART_T3,
+  GH_PART_O1, GH_PART_O2, GH_PART_O3,
+  GH_PAD0, GH_PAD1, GH_PAD2, GH_PAD3,
   GH_DWORDS
 };
+// graph flags
+constexpr uint32_t GF_MONO = 1u;    // one term without pi powers or residual: its
+                                    // float64 magnitude comes from a table
+constexpr uint32_t GF_NARROW = 2u;  // every partial sum fits a signed 64-bit
+                                    // integer (and no term uses several digits)
 
 // component header (dwords)
 enum CompHdr : int {

@@ -377,6 +377,13 @@ void pack_graph(const ksim_graph_view &g, std::vector<uint32_t> &out, uint32_t *
     xbits = std::max(xbits, bits + r.emax);
   }
   const int max_shift = terms.empty() ? 0 : emax_all - e0;
+  bool any_wide = false;
+  for (size_t k = 0; k < terms.size(); ++k) {
+    const Range &r = rg[k];
+    any_wide |= std::max(std::abs(r.nmin + 2 * (r.vmin >> 2)),
+                         std::abs(r.nmax + 2 * (r.vmax >> 2))) > TAB_R;
+  }
+  uint32_t gflags = 0;
   if (!terms.empty()) {
     const int need = xbits - e0 + ceil_log2((int64_t)terms.size()) + 1;
     st.sum_bits = std::max(st.sum_bits, need + 1);
@@ -384,14 +391,62 @@ void pack_graph(const ksim_graph_view &g, std::vector<uint32_t> &out, uint32_t *
       throw std::invalid_argument(
           "term magnitudes exceed the 128-bit exact sum of the sampler (" +
           std::to_string(need + 1) + " bits needed)");
+    // every partial sum (and every shifted term and product) is below
+    // 2^need in magnitude, so 64-bit two's-complement sums are exact
+    if (need <= 62 && !any_wide) gflags |= GF_NARROW;
+    if (terms.size() == 1 && !terms[0].residual && rg[0].amax == 0 && rg[0].bmax == 0)
+      gflags |= GF_MONO;
+  }
+  // ---- term order: GH_PARTS parts of about equal work (heaviest term to
+  // the lightest part), parts ordered so that parts {0,1} and {2,3} are
+  // also about equal (a two-way split).  The exact sum does not depend on
+  // the order of its terms.
+  std::vector<size_t> order;
+  uint32_t part_first[GH_PARTS + 1] = {0, 0, 0, 0, 0};
+  {
+    std::vector<double> wt(terms.size());
+    for (size_t k = 0; k < terms.size(); ++k) {
+      const TermBuild &tb = terms[k];
+      wt[k] = 48.0 + (double)W32 * (tb.proj.size() + 2 * tb.ppx.size() + tb.lin.size() +
+                                    2 * tb.pair.size()) +
+              3.0 * (tb.proj.size() + tb.ppx.size() + tb.lin.size() + 2 * tb.pair.size());
+    }
+    std::vector<size_t> by(terms.size());
+    for (size_t k = 0; k < by.size(); ++k) by[k] = k;
+    std::stable_sort(by.begin(), by.end(), [&](size_t x, size_t y) { return wt[x] > wt[y]; });
+    std::vector<std::vector<size_t>> bins(GH_PARTS);
+    double load[GH_PARTS] = {0, 0, 0, 0};
+    for (size_t k : by) {
+      int b = 0;
+      for (int j = 1; j < GH_PARTS; ++j)
+        if (load[j] < load[b]) b = j;
+      bins[b].push_back(k);
+      load[b] += wt[k];
+    }
+    int bo[GH_PARTS] = {0, 1, 2, 3};
+    std::stable_sort(bo, bo + GH_PARTS, [&](int x, int y) { return load[x] > load[y]; });
+    const int seq[GH_PARTS] = {bo[0], bo[3], bo[1], bo[2]};  // (max, min), (2nd, 3rd)
+    for (int p = 0; p < GH_PARTS; ++p) {
+      std::sort(bins[seq[p]].begin(), bins[seq[p]].end());
+      part_first[p] = (uint32_t)order.size();
+      order.insert(order.end(), bins[seq[p]].begin(), bins[seq[p]].end());
+    }
+    part_first[GH_PARTS] = (uint32_t)order.size();
   }
   // ---- emit
   ghdr[GH_NTERMS] = (uint32_t)terms.size();
   ghdr[GH_W32] = (uint32_t)W32;
   ghdr[GH_E0] = (uint32_t)e0;
   ghdr[GH_TERM_OFF] = (uint32_t)out.size();
+  ghdr[GH_FLAGS] = gflags;
   size_t nrec = 0;
-  for (size_t k = 0; k < terms.size(); ++k) {
+  for (size_t q = 0; q < order.size(); ++q) {
+    const size_t k = order[q];
+    for (int p = 1; p < GH_PARTS; ++p)
+      if (part_first[p] == q) {
+        ghdr[GH_PART_T1 + p - 1] = (uint32_t)q;
+        ghdr[GH_PART_O1 + p - 1] = (uint32_t)out.size();
+      }
     const TermBuild &tb = terms[k];
     const size_t hdr = out.size();
     out.resize(hdr + TH_DWORDS, 0u);
@@ -451,6 +506,11 @@ void pack_graph(const ksim_graph_view &g, std::vector<uint32_t> &out, uint32_t *
     out.insert(out.end(), rec.begin(), rec.end());
     nrec += tb.proj.size() + tb.ppx.size() + tb.lin.size() + tb.pair.size();
   }
+  for (int p = 1; p < GH_PARTS; ++p)
+    if (part_first[p] == order.size()) {  // empty trailing parts start at the end
+      ghdr[GH_PART_T1 + p - 1] = (uint32_t)order.size();
+      ghdr[GH_PART_O1 + p - 1] = (uint32_t)out.size();
+    }
   ghdr[GH_NREC] = (uint32_t)nrec;
   st.terms += terms.size();
   st.records += nrec;
@@ -566,6 +626,7 @@ std::vector<uint32_t> pack_program(const ksim_graph_view *graphs, const int32_t 
     for (int i = 0; i < ng; ++i) {
       Job &j = jobs[gi + i];
       j.hdr[GH_TERM_OFF] += (uint32_t)w.size();  // relative -> absolute
+      for (int p = 0; p < GH_PARTS - 1; ++p) j.hdr[GH_PART_O1 + p] += (uint32_t)w.size();
       std::memcpy(&w[gh + (size_t)i * GH_DWORDS], j.hdr, sizeof(j.hdr));
       if (w.size() + j.words.size() >= (size_t)1 << 32)
         throw std::invalid_argument("program larger than 16 GiB");

@@ -95,11 +95,21 @@ void validate_words(const uint32_t *w, size_t n) {
       const uint32_t *gh = w + ch[CH_GRAPH_OFF] + (size_t)g * GH_DWORDS;
       const uint32_t W32 = gh[GH_W32];
       if (W32 < 1 || W32 > cw) bad("graph mask words");
+      const uint32_t nt = gh[GH_NTERMS];
+      if (gh[GH_FLAGS] & ~(GF_MONO | GF_NARROW)) bad("graph flags");
+      if ((gh[GH_FLAGS] & GF_MONO) && nt != 1) bad("single-term flag on a graph of several terms");
+      const uint32_t pt[GH_PARTS + 1] = {0, gh[GH_PART_T1], gh[GH_PART_T2], gh[GH_PART_T3], nt};
+      for (int p = 0; p < GH_PARTS; ++p)
+        if (pt[p] > pt[p + 1]) bad("term parts out of order");
       uint64_t to = gh[GH_TERM_OFF];
-      for (uint32_t t = 0; t < gh[GH_NTERMS]; ++t) {
+      for (uint32_t t = 0; t <= nt; ++t) {
+        for (int p = 1; p < GH_PARTS; ++p)
+          if (pt[p] == t && gh[GH_PART_O1 + p - 1] != to) bad("term part offset");
+        if (t == nt) break;
         need(to + TH_DWORDS, "term header");
         const uint32_t *th = w + to;
         if (th[TH_FLAGS] & ~(TF_RESIDUAL | TF_PI)) bad("term flags");
+        if ((gh[GH_FLAGS] & GF_MONO) && th[TH_FLAGS]) bad("single-term graph with pi or residual");
         uint64_t sz = (uint64_t)th[TH_NPROJ] * (W32 + 1) + (uint64_t)th[TH_NPPX] * (2 * W32 + 2);
         uint64_t rp = to + TH_DWORDS + sz;
         for (uint32_t k = 0; k < th[TH_NCHUNK]; ++k) {

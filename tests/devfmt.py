@@ -15,7 +15,7 @@ PK_OV, PK_WV, PK_ON, PK_WN, PK_OA, PK_WA, PK_OB, PK_WB, PK_OS = 0, 9, 9, 8, 17, 
 TH = dict(NPROJ=0, NPPX=1, NCHUNK=2, FLAGS=3, S=4, V=5, N=6, A=7, B=8, E=9, R0=10, SIZE=14)
 TH_DWORDS = 16
 CHUNK_HDR = 4
-GH_DWORDS = 8
+GH_DWORDS = 16
 CH = dict(NGRAPHS=0, F=1, NOUT=2, WMAX=3, COLBASE=4, FSEL=5, DEST=6, GRAPH=7)
 PH_DWORDS = 8
 TF_RESIDUAL, TF_PI = 1, 2

@@ -48,7 +48,7 @@ def test_blob_holds_the_packed_words(prog, blob):
     st = {}
     w = pack_words(prog, st)
     assert blob[:8] == b"KSIMPRG\0"
-    assert struct.unpack("<II", blob[8:16]) == (1, HDR)
+    assert struct.unpack("<II", blob[8:16]) == (2, HDR)
     assert np.array_equal(words_of(blob), w)
     assert struct.unpack("<Q", blob[24:32])[0] == fnv1a64(w.tobytes())
     got = blob_stats(blob)
@@ -69,7 +69,7 @@ def test_rejects_damaged_envelope(blob, case):
     if case == "magic":
         b[0] ^= 1
     elif case == "version":
-        b[8] = 2
+        b[8] ^= 3  # another version number
     elif case == "truncated":
         b = b[:-4]
     elif case == "extra":
@@ -84,7 +84,7 @@ def test_rejects_damaged_envelope(blob, case):
 
 def structural_cases(w):
     """(name, mutated words) pairs, each a corruption the validator must catch."""
-    PH, CH, GH, TH = 8, 16, 8, 16
+    PH, CH, GH, TH = 8, 16, 16, 16
     c0 = int(w[PH])                       # component 0 header offset
     fsel, dest, graphs = int(w[c0 + 5]), int(w[c0 + 6]), int(w[c0 + 7])
     g1 = graphs + GH                      # graph 1 header
@@ -107,13 +107,19 @@ def structural_cases(w):
     mut("term size", t0 + 14, int(w[t0 + 14]) + 1)
     mut("term flags", t0 + 3, 8)
     mut("chunk count", t0 + 2, int(w[t0 + 2]) + 1000)
+    mut("graph flags", g1 + 5, 8)
+    mut("term part order", g1 + 6, int(w[g1 + 0]) + 1)
+    mut("term part offset", g1 + 9, int(w[g1 + 9]) + 1)
+    ng = int(w[c0 + 0])
+    multi = [graphs + GH * i for i in range(ng) if int(w[graphs + GH * i]) >= 2]
+    mut("single-term flag", multi[0] + 5, int(w[multi[0] + 5]) | 1)
     return out
 
 
 def test_rejects_structural_corruption(blob):
     w = words_of(blob)
     cases = structural_cases(w)
-    assert len(cases) == 11
+    assert len(cases) == 15
     for name, v in cases:
         with pytest.raises(ValueError):
             blob_stats(with_words(blob, v))

@@ -294,6 +294,7 @@ def batched_result(steps, warmup, n_programs=64, tune=True):
                 plain_ms=wall_p / steps * 1e3, batched_ms=wall_b / steps * 1e3,
                 batched_equals_plain=bool(same), ops_total=batch.ops,
                 tuned=dict(wave_shots=batch.tune["wave_shots"],
+                           split_share=batch.tune.get("split_share", 8),
                            blocks_per_cu=batch.tune["blocks_per_cu"],
                            tune_s=batch.tune["tune_s"]),
                 roofline=valu_roofline(achieved, batch.bytes / kern_b))

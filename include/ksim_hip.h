@@ -266,12 +266,19 @@ typedef struct ksim_batch ksim_batch;
 
 /* A batch of n programs sampled by one persistent kernel.  Each program k
  * gets its own resident f matrix d_f[k] (B[k], f_cols[k]) u8 and output
- * d_out[k] (B[k], num_outputs_k) u8 and seed[k]. */
+ * d_out[k] (B[k], num_outputs_k) u8 and seed[k].  Work items are (program,
+ * component, shot range, term split): a persistent 256-thread block takes
+ * one item at a time; with term split L its four waves form 4/L groups
+ * whose L waves share each graph's terms on one 64-shot tile.  wave_shots
+ * (a multiple of 64) is the shots one group covers per item.  The split
+ * of each component is ksim_program_split_hint's for a tile budget of
+ * tile_ops algorithmic ops per wave (<= 0: 1/8 of one wave's share of the
+ * batch, with 5 blocks per CU; a huge budget turns the split off). */
 int ksim_batch_create(ksim_program *const *progs, int32_t n,
                       const uint8_t *const *d_f, const int32_t *f_cols,
                       const int64_t *B,
                       const uint64_t *seed, uint8_t *const *d_out,
-                      int32_t wave_shots, ksim_batch **out);
+                      int32_t wave_shots, double tile_ops, ksim_batch **out);
 
 /* Run one batched launch on `stream`.  use_graph != 0 captures the launch
  * (work-list reset + persistent kernel) into a hipGraph on first use and
@@ -280,6 +287,15 @@ int ksim_batch_run(ksim_batch *b, int32_t persistent_blocks, int32_t use_graph,
                    void *stream);
 
 int ksim_batch_check(ksim_batch *b);
+
+/* Term split for component `comp` of a program in the persistent kernels:
+ * the smallest L in {1, 2, 4} whose 64-shot tile costs each of its L waves
+ * at most `tile_ops` algorithmic ops while keeping at least 60% of their
+ * work useful (graphs with one term are evaluated by every wave of the
+ * group).  tile_cost_ops (optional) receives that per-wave tile cost.  No
+ * reference counterpart (the reference runs one thread per shot). */
+int ksim_program_split_hint(const ksim_program *p, int32_t comp, double tile_ops,
+                            int32_t *split, double *tile_cost_ops);
 void ksim_batch_destroy(ksim_batch *b);
 
 /* ---- scheduler: the device side of the serving loop ---------------------- */
@@ -299,8 +315,11 @@ int ksim_sched_set_slot(ksim_sched *q, int32_t slot, ksim_program *p,
                         uint64_t seed, uint8_t *d_out);
 int ksim_sched_clear_slot(ksim_sched *q, int32_t slot);
 
-/* Enqueue one persistent launch on `stream` over n items, each 4 int32:
- * (slot, component, first shot, shot count).  Returns after staging.
+/* Enqueue one persistent launch on `stream` over n items, each 5 int32:
+ * (slot, component, first shot, shot count, term split L in {1, 2, 4}).
+ * A persistent block takes one item: 4/L groups of L waves, group g
+ * sampling the 64-shot tiles g, g + 4/L, ... of the item (so an item of
+ * 256/L shots gives every wave one tile).  Returns after staging.
  * Launches rotate over four staging sets (own buffers and work cursor), so
  * up to four launches may be in flight on different streams; launch k+4
  * waits for launch k to finish before reusing its set. */

@@ -55,12 +55,15 @@ EXPORTS = {
     "ksim_program_sample_device": (C.c_int, [vp, vp, C.c_int64, C.c_int32, C.c_int64, C.c_uint64,
                                              vp, vp, vp]),
     "ksim_program_set_tiles": (C.c_int, [vp, C.c_int32, C.c_int32]),
+    "ksim_program_split_hint": (C.c_int, [vp, C.c_int32, C.c_double, i32p,
+                                          C.POINTER(C.c_double)]),
     "ksim_program_check": (C.c_int, [vp, vp]),
     "ksim_program_device_bytes": (C.c_int64, [vp]),
     "ksim_program_stats": (C.c_int, [vp, f64p]),
     "ksim_program_destroy": (None, [vp]),
     "ksim_batch_create": (C.c_int, [C.POINTER(vp), C.c_int32, C.POINTER(vp), i32p, i64p,
-                                    C.POINTER(C.c_uint64), C.POINTER(vp), C.c_int32, C.POINTER(vp)]),
+                                    C.POINTER(C.c_uint64), C.POINTER(vp), C.c_int32, C.c_double,
+                                    C.POINTER(vp)]),
     "ksim_batch_run": (C.c_int, [vp, C.c_int32, C.c_int32, vp]),
     "ksim_batch_check": (C.c_int, [vp]),
     "ksim_batch_destroy": (None, [vp]),

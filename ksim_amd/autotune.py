@@ -1,16 +1,21 @@
 """Autotuner for the batched persistent launch.
 
-The persistent kernel has two launch-shape knobs:
+The persistent kernel has three launch-shape knobs:
 
-* ``wave_shots`` — shots per work item.  A wave takes one item at a time from
-  the device-wide cursor and runs its shots 64 at a time (one lane per shot),
-  so larger items amortise the cursor atomic and keep a wave on one program
-  (its packed words stay hot in that CU's scalar cache) at the cost of a
-  coarser tail;
+* ``wave_shots`` — shots per wave group per work item.  A block takes one
+  item at a time from the device-wide cursor and each of its wave groups
+  runs its shots 64 at a time (one lane per shot), so larger items amortise
+  the cursor atomic and keep a block on one program (its packed words stay
+  hot in that CU's scalar cache) at the cost of a coarser tail;
+* ``split_share`` — how finely heavy components are split over waves: a
+  64-shot tile may take each of its waves at most 1/split_share of one
+  wave's part of the batch; the components above that budget have their
+  terms shared by 2 or 4 waves (``None``: no split).  Splitting shortens
+  the tail of heavy tiles at the cost of a partial-sum exchange per graph;
 * ``blocks_per_cu`` — 256-thread persistent blocks per CU.  Past the
   occupancy the kernel's registers allow, extra blocks only queue.
 
-Both depend on the mix (term counts, shot counts, parameter words), so the
+All depend on the mix (term counts, shot counts, parameter words), so the
 tuner times every candidate on the actual batch (one untimed warm-up and the
 median of ``reps`` hipGraph replays each) and keeps the fastest.  Results are
 cached on disk per (device, batch signature); ``KSIM_TUNE_CACHE`` names the
@@ -25,6 +30,9 @@ import time
 
 WAVE_SHOTS = (64, 128, 256)
 BLOCKS_PER_CU = (4, 5, 6, 8)
+# term-split budget: a 64-shot tile may take each wave at most 1/share of
+# its share of the batch (None: no split)
+SPLIT_SHARES = (8, 32, None)
 
 
 def cache_path() -> str:
@@ -78,10 +86,22 @@ def time_batch(batch, blocks: int, reps: int) -> float:
     return times[len(times) // 2]
 
 
+def split_budget(samplers, fs, cus: int, share) -> float:
+    """Per-wave tile budget (algorithmic ops) for a split share: 1/share of
+    one wave's part of the batch (5 blocks of 4 waves per CU); None turns
+    the term split off."""
+    if share is None:
+        return 1e300
+    total = sum(s.ops_per_shot * int(f.shape[0]) for s, f in zip(samplers, fs))
+    return total / (cus * 5 * 4) / float(share)
+
+
 def tune_batch(samplers, fs, seeds, reps: int = 3, use_cache: bool = True,
-               wave_shots=WAVE_SHOTS, blocks_per_cu=BLOCKS_PER_CU, log=None) -> dict:
-    """Pick (wave_shots, blocks_per_cu) for this batch; returns
-    ``{"wave_shots", "blocks_per_cu", "seconds", "trials", "cached"}``."""
+               wave_shots=WAVE_SHOTS, blocks_per_cu=BLOCKS_PER_CU, split_shares=SPLIT_SHARES,
+               log=None) -> dict:
+    """Pick (wave_shots, split share, blocks_per_cu) for this batch; returns
+    ``{"wave_shots", "split_share", "tile_ops", "blocks_per_cu", "seconds",
+    "trials", "cached"}``."""
     from .batch import BatchedSampler, device_info
     info = device_info()
     dev_key = f"cus{info['cus']}-clk{info['clock_khz']}"
@@ -93,15 +113,20 @@ def tune_batch(samplers, fs, seeds, reps: int = 3, use_cache: bool = True,
     trials = []
     t0 = time.perf_counter()
     for ws in wave_shots:
-        b = BatchedSampler(samplers, fs, seeds, wave_shots=ws)
-        for bpc in blocks_per_cu:
-            sec = time_batch(b, info["cus"] * bpc, reps)
-            trials.append(dict(wave_shots=ws, blocks_per_cu=bpc, seconds=sec))
-            if log:
-                log(f"[tune] wave_shots={ws} blocks/CU={bpc}: {sec * 1e3:.2f} ms")
-        del b
+        for share in split_shares:
+            tops = split_budget(samplers, fs, info["cus"], share)
+            b = BatchedSampler(samplers, fs, seeds, wave_shots=ws, tile_ops=tops)
+            for bpc in blocks_per_cu:
+                sec = time_batch(b, info["cus"] * bpc, reps)
+                trials.append(dict(wave_shots=ws, split_share=share, tile_ops=tops,
+                                   blocks_per_cu=bpc, seconds=sec))
+                if log:
+                    log(f"[tune] wave_shots={ws} split_share={share} blocks/CU={bpc}: "
+                        f"{sec * 1e3:.2f} ms")
+            del b
     best = min(trials, key=lambda r: r["seconds"])
-    res = dict(wave_shots=best["wave_shots"], blocks_per_cu=best["blocks_per_cu"],
+    res = dict(wave_shots=best["wave_shots"], split_share=best["split_share"],
+               tile_ops=best["tile_ops"], blocks_per_cu=best["blocks_per_cu"],
                seconds=best["seconds"], trials=trials,
                tune_s=time.perf_counter() - t0)
     if use_cache:

@@ -3,9 +3,12 @@
 The reference samples one component per kernel launch
 (src/python/ksim/sample/sampler.py:60-64, src/cpp/ksim/kokkos_sim.cpp:214);
 with many small programs the launches cannot fill the GPU.  Here every
-(program, component, 64-shot tile) of the batch is one work item; items are
-ordered heaviest first and a fixed grid of persistent waves pulls them from
-a device-side cursor until the list is drained.  The launch (cursor reset +
+(program, component, shot range) of the batch is one work item of one
+256-thread block; items are ordered heaviest first and a fixed grid of
+persistent blocks pulls them from a device-side cursor until the list is
+drained.  The components of heavy programs get a term split: 2 or 4 waves
+share each graph's terms on one 64-shot tile, so no single tile is long
+enough to leave the rest of the grid waiting at the end.  The launch (cursor reset +
 persistent kernel) can be captured once into a hipGraph and replayed.
 
 Each program keeps its own resident f matrix, output buffer and seed, and
@@ -38,7 +41,7 @@ class BatchedSampler:
     """
 
     def __init__(self, samplers: list[HipProgramSampler], f: list, seeds: list[int],
-                 wave_shots: int = 128, blocks_per_cu: int = 6):
+                 wave_shots: int = 128, blocks_per_cu: int = 6, tile_ops: float = 0.0):
         import torch
         torch.cuda.init()
         self.torch = torch
@@ -56,7 +59,7 @@ class BatchedSampler:
         h = C.c_void_p()
         _lib.check(_lib.load().ksim_batch_create(progs, n, dfs, _lib.ptr(fcols, C.c_int32),
                                                  _lib.ptr(Bs, C.c_int64), sd, douts,
-                                                 int(wave_shots), C.byref(h)))
+                                                 int(wave_shots), float(tile_ops), C.byref(h)))
         self._h = h
         self.total_shots = int(Bs.sum())
         self.ops = float(sum(s.ops_per_shot * int(b) for s, b in zip(samplers, Bs)))
@@ -67,6 +70,7 @@ class BatchedSampler:
         # persistent grid: enough 256-thread blocks for every CU to hold
         # as many as registers allow (5 waves per SIMD, 5 blocks per CU)
         self.wave_shots = int(wave_shots)
+        self.tile_ops = float(tile_ops)
         self.blocks = info["cus"] * int(blocks_per_cu)
 
     def run(self, use_graph: bool = True, blocks: int | None = None, stream=None):
@@ -83,7 +87,8 @@ class BatchedSampler:
         (ksim_amd.autotune; cached per device and batch signature)."""
         from .autotune import tune_batch
         t = tune_batch(samplers, f, seeds, **kw)
-        b = cls(samplers, f, seeds, wave_shots=t["wave_shots"], blocks_per_cu=t["blocks_per_cu"])
+        b = cls(samplers, f, seeds, wave_shots=t["wave_shots"], blocks_per_cu=t["blocks_per_cu"],
+                tile_ops=t.get("tile_ops", 0.0))
         b.tune = t
         return b
 

@@ -222,13 +222,18 @@ __global__ __launch_bounds__(kBlock) void k_eval(EvalArgs a, Tables tb) {
     a.exp2[s] = e0;
   } else {
     // the sampler's own probability path (table magnitudes, 64-bit sums)
-    a.absout[s] = graph_prob_any<WL>(as_const(a.prog), as_const(a.gh), pv, &lt, tb, err);
+    uint32_t xn = 0;
+    a.absout[s] = graph_prob_any<WL>(as_const(a.prog), as_const(a.gh), pv, &lt, tb, err,
+                                     Split{1, 0, 0, nullptr}, xn);
   }
   if (err) atomicOr(a.err, err);
 }
 
-// Batched persistent path: each wave pulls 64-shot work items from a
-// device-wide cursor until the list is drained.
+// Batched persistent path: each block pulls work items from a device-wide
+// cursor until the list is drained.  An item is shots [shot0, shot0 + count)
+// of one component of one program with a term split L: the block's four
+// waves form 4/L groups of L waves; group g takes the 64-shot tiles
+// g, g + 4/L, ... of the item and its L waves share each graph's terms.
 struct BatchProg {
   const uint32_t *prog;
   const uint8_t *f;
@@ -240,7 +245,8 @@ struct BatchProg {
 struct BatchItem {
   int32_t prog, comp;
   int64_t shot0;
-  int32_t count, pad;  // shots [shot0, shot0 + count) of program `prog`
+  int32_t count;  // shots [shot0, shot0 + count) of program `prog`
+  int32_t split;  // waves per group: 1, 2 or 4
 };
 
 // reset the work cursor and the word `clear` (the batch clears its error
@@ -256,14 +262,22 @@ __global__ void k_reset_cursor(uint32_t *cursor, uint32_t *clear) {
 template <int WL>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KSIM_WAVES_PER_EU))) void k_batch(
     const BatchProg *__restrict__ progs, const BatchItem *__restrict__ items, uint32_t n_items,
-    uint32_t *cursor, uint32_t *err, Tables tb) {
+    uint32_t *cursor, uint32_t *err, Tables tb, int4 *xch) {
   __shared__ LdsTables lt;
+  __shared__ uint32_t s_item;
   load_tables(&lt, tb);
+  // the wave index is the same on every lane of a wave: readfirstlane lets
+  // the compiler keep what derives from it (term ranges, program offsets)
+  // on the scalar path
   const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  int4 *my_xch = xch + (size_t)blockIdx.x * XCH_INT4_PER_BLOCK;
+  uint32_t xn = 0;  // exchanges so far (the same count in every wave of the block)
   for (;;) {
-    uint32_t idx = 0;
-    if (lane == 0) idx = atomicAdd(cursor, 1u);
-    idx = __builtin_amdgcn_readfirstlane(__shfl(idx, 0));
+    if (threadIdx.x == 0) s_item = atomicAdd(cursor, 1u);
+    __syncthreads();
+    const uint32_t idx = __builtin_amdgcn_readfirstlane(s_item);
+    __syncthreads();  // s_item is read before thread 0 takes the next item
     if (idx >= n_items) break;
     const BatchItem it = items[idx];
     const BatchProg bp = progs[it.prog];
@@ -273,9 +287,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KSIM_WAV
     a.num_f = bp.num_f; a.num_outputs = bp.num_outputs; a.tot_out = bp.tot_out; a.pad = 0;
     a.out_words = 0; a.out_atomic = 0;
     a.noise = NoiseDev{};
-    for (int sub = 0; sub < it.count; sub += 64) {
-      const int64_t s = it.shot0 + sub + lane;
-      if (sub + lane < it.count && s < bp.B) run_shot<WL>(a, it.comp, s, &lt, tb);
+    const int L = it.split, G = 4 / L;
+    const Split sp{L, wave % L, wave - wave % L, my_xch};
+    const int ntiles = (it.count + 63) >> 6;
+    const int rounds = (ntiles + G - 1) / G;
+    for (int r = 0; r < rounds; ++r) {
+      const int tile = r * G + wave / L;
+      // with L == 1 there are no barriers: a wave without a tile stops
+      if (L == 1 && tile >= ntiles) break;
+      const int off = tile * 64 + lane;
+      const bool valid = off < it.count;
+      const int64_t s = it.shot0 + min(off, it.count - 1);
+      run_shot<WL, false, true>(a, it.comp, s, &lt, tb, sp, valid, &xn);
     }
   }
 }
@@ -805,6 +828,65 @@ int ksim_device_info(int64_t *info4) {
 }  // extern "C"
 
 // ------------------------------------------------------------------ batch
+namespace {
+
+// Per-wave work of one shot of component c when L waves (1, 2, 4) share each
+// graph's terms, in record units (a term costs about 40 records of
+// materialisation, a graph about 60 of decision work; single-term graphs
+// are evaluated by every wave of the group, the others add an exchange).
+void comp_work(const ksim_program *p, int c, double w[3]) {
+  w[0] = w[1] = w[2] = 0.0;
+  const std::vector<uint32_t> &gh = p->graph_hdr[c];
+  for (int i = 0; i < p->n_graphs[c]; ++i) {
+    const uint32_t *h = &gh[(size_t)i * GH_DWORDS];
+    const double body = (double)h[GH_NREC] + 40.0 * h[GH_NTERMS];
+    w[0] += body + 60.0;
+    const bool mono = h[GH_FLAGS] & GF_MONO;
+    w[1] += mono ? body + 60.0 : body / 2 + 80.0;
+    w[2] += mono ? body + 60.0 : body / 4 + 80.0;
+  }
+}
+
+// algorithmic ops per shot of component c (the program's ops shared out by work)
+double comp_ops(const ksim_program *p, int c) {
+  double tot = 0.0, mine = 0.0, w[3];
+  for (int k = 0; k < p->n_comp; ++k) {
+    comp_work(p, k, w);
+    tot += w[0];
+    if (k == c) mine = w[0];
+  }
+  return tot > 0 ? p->st.ops * mine / tot : 0.0;
+}
+
+// Term split for component c: the smallest L whose 64-shot tile costs each
+// wave at most `tile_ops` algorithmic ops, among the splits that keep at
+// least 60% of the waves' work useful; the cheapest such tile otherwise.
+int split_for(const ksim_program *p, int c, double tile_ops) {
+  double w[3];
+  comp_work(p, c, w);
+  if (w[0] <= 0) return 1;
+  const double scale = comp_ops(p, c) / w[0];
+  const int Ls[3] = {1, 2, 4};
+  int best = 1;
+  for (int k = 0; k < 3; ++k) {
+    const double eff = w[0] / (Ls[k] * w[k]);
+    if (k > 0 && eff < 0.6) break;
+    best = Ls[k];
+    if (64.0 * w[k] * scale <= tile_ops) break;
+  }
+  return best;
+}
+
+// per-wave algorithmic ops of a 64-shot tile of component c at split L
+double tile_cost(const ksim_program *p, int c, int L) {
+  double w[3];
+  comp_work(p, c, w);
+  if (w[0] <= 0) return 0.0;
+  return 64.0 * w[L == 1 ? 0 : L == 2 ? 1 : 2] * comp_ops(p, c) / w[0];
+}
+
+}  // namespace
+
 struct ksim_batch {
   std::vector<ksim_program *> progs;
   BatchProg *d_progs = nullptr;
@@ -817,13 +899,27 @@ struct ksim_batch {
   hipGraphExec_t exec = nullptr;
   int graph_blocks = 0;
   hipStream_t graph_stream = nullptr;
+  int4 *d_xch = nullptr;  // per-block exchange areas of the term split
+  int xch_blocks = 0;
 };
 
 extern "C" {
 
+int ksim_program_split_hint(const ksim_program *p, int32_t comp, double tile_ops,
+                            int32_t *split, double *tile_cost_ops) {
+  return guarded([&] {
+    if (!p || !split) throw std::invalid_argument("NULL argument");
+    if (comp < 0 || comp >= p->n_comp) throw std::invalid_argument("component index out of range");
+    *split = split_for(p, comp, tile_ops);
+    if (tile_cost_ops) *tile_cost_ops = tile_cost(p, comp, *split);
+  });
+}
+
+
 int ksim_batch_create(ksim_program *const *progs, int32_t n, const uint8_t *const *d_f,
                       const int32_t *f_cols, const int64_t *B, const uint64_t *seed,
-                      uint8_t *const *d_out, int32_t wave_shots, ksim_batch **out) {
+                      uint8_t *const *d_out, int32_t wave_shots, double tile_ops,
+                      ksim_batch **out) {
   return guarded([&] {
     if (!out || (n > 0 && (!progs || !d_f || !f_cols || !B || !seed || !d_out)))
       throw std::invalid_argument("NULL argument");
@@ -833,23 +929,38 @@ int ksim_batch_create(ksim_program *const *progs, int32_t n, const uint8_t *cons
     std::vector<BatchProg> hp(n);
     struct Cost { double c; BatchItem it; };
     std::vector<Cost> items;
+    double total_ops = 0.0;
     for (int k = 0; k < n; ++k) {
       ksim_program *p = progs[k];
       if (!p) throw std::invalid_argument("NULL program in batch");
       if (B[k] < 0) throw std::invalid_argument("B < 0");
       if (B[k] > 0 && ((!d_f[k] && f_cols[k] > 0) || !d_out[k])) throw std::invalid_argument("NULL buffer");
       if (f_cols[k] < p->num_f) throw std::invalid_argument("f matrix has fewer columns than f_selection needs");
+      total_ops += p->st.ops * (double)B[k];
+    }
+    // a tile should take each wave at most 1/8 of its share of the batch
+    // (waves: 5 resident blocks of 4 waves per CU), so the heaviest-first
+    // order leaves a short tail
+    int dev = 0, cus = 256;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "CU count");
+    const double tile_target =
+        tile_ops > 0 ? tile_ops : total_ops / (double)(cus * 5 * 4) / 8.0;
+    for (int k = 0; k < n; ++k) {
+      ksim_program *p = progs[k];
       b->progs.push_back(p);
       b->wl = std::max(b->wl, p->wl);
       hp[k] = BatchProg{p->d_prog, d_f[k], d_out[k], seed[k], B[k], f_cols[k], p->num_outputs,
                         p->tot_out, p->wl};
       for (int c = 0; c < p->n_comp; ++c) {
-        // cost of one shot of this component ~ records + terms over its graphs
-        double cost = 1.0;
-        for (int i = 0; i < p->n_graphs[c]; ++i) cost += p->graph_hdr[c][i * GH_DWORDS + GH_NREC];
-        for (int64_t s0 = 0; s0 < B[k]; s0 += wave_shots) {
-          const int64_t ns = std::min<int64_t>(wave_shots, B[k] - s0);
-          items.push_back(Cost{cost * (double)ns, BatchItem{k, c, s0, (int32_t)ns, 0}});
+        const int L = split_for(p, c, tile_target);
+        const double tc = tile_cost(p, c, L);
+        // one item: wave_shots shots per group of L waves, 4 / L groups
+        const int64_t span = (int64_t)wave_shots * (4 / L);
+        for (int64_t s0 = 0; s0 < B[k]; s0 += span) {
+          const int64_t ns = std::min<int64_t>(span, B[k] - s0);
+          const int64_t rounds = ((ns + 63) / 64 + (4 / L) - 1) / (4 / L);
+          items.push_back(Cost{tc * (double)rounds, BatchItem{k, c, s0, (int32_t)ns, L}});
         }
       }
     }
@@ -877,13 +988,25 @@ int ksim_batch_create(ksim_program *const *progs, int32_t n, const uint8_t *cons
   });
 }
 
+// exchange areas for `blocks` persistent blocks (allocated outside any capture)
+static void batch_xch(ksim_batch *b, int blocks) {
+  if (blocks <= b->xch_blocks) return;
+  if (b->d_xch) hip_check(hipFree(b->d_xch), "hipFree");
+  b->d_xch = nullptr;
+  b->xch_blocks = 0;
+  hip_check(hipMalloc(&b->d_xch, sizeof(int4) * XCH_INT4_PER_BLOCK * (size_t)blocks),
+            "hipMalloc(exchange)");
+  b->xch_blocks = blocks;
+}
+
 static void batch_enqueue(ksim_batch *b, int blocks, hipStream_t st) {
   DevTables t = tables_for_current_device();
+  if (blocks > b->xch_blocks) throw std::logic_error("exchange area smaller than the grid");
   hipLaunchKernelGGL(k_reset_cursor, dim3(1), dim3(64), 0, st, b->d_cursor, b->d_cursor + 1);
   hip_check(hipGetLastError(), "k_reset_cursor");
 #define KSIM_BATCH_LAUNCH(W)                                                                  \
   hipLaunchKernelGGL(k_batch<W>, dim3(blocks), dim3(kBlock), 0, st, b->d_progs, b->d_items,    \
-                     b->n_items, b->d_cursor, b->d_cursor + 1, t.view())
+                     b->n_items, b->d_cursor, b->d_cursor + 1, t.view(), b->d_xch)
   switch (b->wl) {
     case 1: KSIM_BATCH_LAUNCH(1); break;
     case 2: KSIM_BATCH_LAUNCH(2); break;
@@ -899,6 +1022,11 @@ int ksim_batch_run(ksim_batch *b, int32_t persistent_blocks, int32_t use_graph, 
     if (!b) throw std::invalid_argument("batch is NULL");
     if (persistent_blocks <= 0) throw std::invalid_argument("persistent_blocks must be > 0");
     hipStream_t st = (hipStream_t)stream;
+    if (persistent_blocks > b->xch_blocks) {
+      // the new area replaces one a running launch may still use
+      hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+      batch_xch(b, persistent_blocks);
+    }
     if (!use_graph) {
       batch_enqueue(b, persistent_blocks, st);
       return;
@@ -936,6 +1064,7 @@ void ksim_batch_destroy(ksim_batch *b) {
   if (b->d_progs) (void)hipFree(b->d_progs);
   if (b->d_items) (void)hipFree(b->d_items);
   if (b->d_cursor) (void)hipFree(b->d_cursor);
+  if (b->d_xch) (void)hipFree(b->d_xch);
   delete b;
 }
 
@@ -962,6 +1091,8 @@ struct ksim_sched {
   hipEvent_t done[KSIM_SCHED_BUFS] = {};  // recorded after the launch's kernel
   bool done_valid[KSIM_SCHED_BUFS] = {};
   uint32_t *d_cursor = nullptr;  // [2k] cursor of set k, [2k+1] scratch; [64] error word
+  int4 *d_xch[KSIM_SCHED_BUFS] = {};  // exchange areas of set k's persistent blocks
+  int xch_blocks = 0;
   int next = 0;
 };
 
@@ -1021,7 +1152,8 @@ int ksim_sched_clear_slot(ksim_sched *q, int32_t slot) {
   });
 }
 
-// items: n rows of (slot, component, first shot, shot count), int32 each.
+// items: n rows of (slot, component, first shot, shot count, term split),
+// int32 each.
 int ksim_sched_launch(ksim_sched *q, const int32_t *items, int32_t n, int32_t blocks,
                       void *stream) {
   return guarded([&] {
@@ -1031,14 +1163,31 @@ int ksim_sched_launch(ksim_sched *q, const int32_t *items, int32_t n, int32_t bl
     if (n == 0) return;
     if (!items) throw std::invalid_argument("items is NULL");
     hipStream_t st = (hipStream_t)stream;
+    if (blocks > q->xch_blocks) {
+      // every set's area grows; launches in flight may still use the old ones
+      for (int j = 0; j < KSIM_SCHED_BUFS; ++j)
+        if (q->done_valid[j]) hip_check(hipEventSynchronize(q->done[j]), "hipEventSynchronize");
+      for (int j = 0; j < KSIM_SCHED_BUFS; ++j) {
+        if (q->d_xch[j]) hip_check(hipFree(q->d_xch[j]), "hipFree");
+        q->d_xch[j] = nullptr;
+      }
+      q->xch_blocks = 0;
+      for (int j = 0; j < KSIM_SCHED_BUFS; ++j)
+        hip_check(hipMalloc(&q->d_xch[j], sizeof(int4) * XCH_INT4_PER_BLOCK * (size_t)blocks),
+                  "hipMalloc(exchange)");
+      q->xch_blocks = blocks;
+    }
     const int k = q->next;
     q->next = (q->next + 1) % KSIM_SCHED_BUFS;
     // the set's previous launch (possibly on another stream) must be over
     if (q->done_valid[k]) hip_check(hipEventSynchronize(q->done[k]), "hipEventSynchronize");
     int wl = 1;
     for (int i = 0; i < n; ++i) {
-      const int32_t slot = items[4 * i], comp = items[4 * i + 1];
-      const int32_t shot0 = items[4 * i + 2], count = items[4 * i + 3];
+      const int32_t slot = items[5 * i], comp = items[5 * i + 1];
+      const int32_t shot0 = items[5 * i + 2], count = items[5 * i + 3];
+      const int32_t split = items[5 * i + 4];
+      if (split != 1 && split != 2 && split != 4)
+        throw std::invalid_argument("item term split must be 1, 2 or 4");
       if (slot < 0 || slot >= q->max_slots || !q->slots[slot].prog)
         throw std::invalid_argument("item names an empty slot");
       const BatchProg &bp = q->slots[slot];
@@ -1047,7 +1196,7 @@ int ksim_sched_launch(ksim_sched *q, const int32_t *items, int32_t n, int32_t bl
       if (shot0 < 0 || count <= 0 || (int64_t)shot0 + count > bp.B)
         throw std::invalid_argument("item shot range outside its request");
       wl = std::max(wl, q->slot_wl[slot]);
-      q->h_items[k][i] = BatchItem{slot, comp, shot0, count, 0};
+      q->h_items[k][i] = BatchItem{slot, comp, shot0, count, split};
     }
     std::memcpy(q->h_progs[k], q->slots.data(), sizeof(BatchProg) * q->max_slots);
     hip_check(hipMemcpyAsync(q->d_progs[k], q->h_progs[k], sizeof(BatchProg) * q->max_slots,
@@ -1060,7 +1209,7 @@ int ksim_sched_launch(ksim_sched *q, const int32_t *items, int32_t n, int32_t bl
     hip_check(hipGetLastError(), "k_reset_cursor");
 #define KSIM_SCHED_LAUNCH(W)                                                                   \
   hipLaunchKernelGGL(k_batch<W>, dim3(blocks), dim3(kBlock), 0, st, q->d_progs[k],              \
-                     q->d_items[k], (uint32_t)n, cursor, q->d_cursor + 64, t.view())
+                     q->d_items[k], (uint32_t)n, cursor, q->d_cursor + 64, t.view(), q->d_xch[k])
     switch (lane_words(wl)) {
       case 1: KSIM_SCHED_LAUNCH(1); break;
       case 2: KSIM_SCHED_LAUNCH(2); break;
@@ -1098,6 +1247,7 @@ void ksim_sched_destroy(ksim_sched *q) {
     if (q->h_items[k]) (void)hipHostFree(q->h_items[k]);
     if (q->d_progs[k]) (void)hipFree(q->d_progs[k]);
     if (q->d_items[k]) (void)hipFree(q->d_items[k]);
+    if (q->d_xch[k]) (void)hipFree(q->d_xch[k]);
   }
   if (q->d_cursor) (void)hipFree(q->d_cursor);
   delete q;

@@ -409,37 +409,112 @@ __device__ __forceinline__ Acc4 widen(const Acc64 &n) {
   return a;
 }
 
+// Waves that share the term sums of one 64-shot tile: L waves (1, 2 or 4)
+// of a block form a group; wave `part` of the group sums term parts
+// [part * GH_PARTS / L, (part + 1) * GH_PARTS / L) of each graph and the
+// partial sums meet in `xch`, the block's exchange area in global memory:
+// [2 buffers][4 waves][4 int4][64 lanes].  Every wave of the group then
+// holds the same exact sum, so all of them take the same decisions.
+struct Split {
+  int L;          // waves per group
+  int part;       // this wave's index in its group
+  int first;      // the group's first wave in the block
+  int4 *xch;      // the block's exchange area (nullptr when L == 1)
+};
+constexpr int XCH_INT4_PER_BLOCK = 2 * 4 * 4 * 64;
+
+// Sum the partial sums of the group's waves (exact integer sums, so every
+// wave ends with the same value).  Uniform across the block: every wave of
+// the block calls it for the same graph, so the barrier is reached by all.
+__device__ __forceinline__ void exchange_sums(GraphSum &g, bool narrow, const Split &sp,
+                                              uint32_t &xn) {
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  int4 *buf = sp.xch + (size_t)(xn & 1u) * (4 * 4 * 64);
+  ++xn;
+  int4 *mine = buf + wave * 256 + lane;
+  if (narrow) {
+    mine[0] = make_int4((int)(uint32_t)g.n.a, (int)(g.n.a >> 32), (int)(uint32_t)g.n.b, (int)(g.n.b >> 32));
+    mine[64] = make_int4((int)(uint32_t)g.n.c, (int)(g.n.c >> 32), (int)(uint32_t)g.n.d, (int)(g.n.d >> 32));
+  } else {
+    const i128_t v[4] = {g.acc.a, g.acc.b, g.acc.c, g.acc.d};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const u128_t u = (u128_t)v[k];
+      mine[64 * k] = make_int4((int)(uint32_t)u, (int)(uint32_t)(u >> 32), (int)(uint32_t)(u >> 64),
+                               (int)(uint32_t)(u >> 96));
+    }
+  }
+  __syncthreads();
+  for (int j = 0; j < sp.L; ++j) {
+    const int w = sp.first + j;
+    if (w == wave) continue;
+    const int4 *o = buf + w * 256 + lane;
+    if (narrow) {
+      const int4 x = o[0], y = o[64];
+      g.n.a += (int64_t)(((uint64_t)(uint32_t)x.y << 32) | (uint32_t)x.x);
+      g.n.b += (int64_t)(((uint64_t)(uint32_t)x.w << 32) | (uint32_t)x.z);
+      g.n.c += (int64_t)(((uint64_t)(uint32_t)y.y << 32) | (uint32_t)y.x);
+      g.n.d += (int64_t)(((uint64_t)(uint32_t)y.w << 32) | (uint32_t)y.z);
+    } else {
+      i128_t v[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int4 x = o[64 * k];
+        v[k] = (i128_t)(((u128_t)(uint32_t)x.w << 96) | ((u128_t)(uint32_t)x.z << 64) |
+                        ((u128_t)(uint32_t)x.y << 32) | (u128_t)(uint32_t)x.x);
+      }
+      g.acc.a += v[0]; g.acc.b += v[1]; g.acc.c += v[2]; g.acc.d += v[3];
+    }
+  }
+}
+
 // |amplitude| of one graph (float64) for this lane's parameter words: the
-// value magnitude() gives on the exact term sum.
-template <int W32, int WL>
+// value magnitude() gives on the exact term sum.  SPLIT: the graph's terms
+// are shared by the waves of sp's group (single-term graphs are evaluated
+// by every wave; they need no exchange).
+template <int W32, int WL, bool SPLIT>
 __device__ __forceinline__ double graph_prob(const cword *__restrict__ P,
                                              const cword *__restrict__ gh,
                                              const uint32_t (&pv)[WL], const LdsTables *lt,
-                                             const Tables &tb, uint32_t &err) {
+                                             const Tables &tb, uint32_t &err, const Split &sp,
+                                             uint32_t &xn) {
   const uint32_t gflags = gh[GH_FLAGS];
   const int e0 = (int)gh[GH_E0];
+  const uint32_t nt = gh[GH_NTERMS];
+  uint32_t t0 = 0, t1 = nt, off = gh[GH_TERM_OFF];
+  const bool share = SPLIT && sp.L > 1 && !(gflags & GF_MONO);
+  if (share) {
+    const int per = GH_PARTS / sp.L;
+    const int p0 = __builtin_amdgcn_readfirstlane(sp.part * per), p1 = p0 + per;
+    if (p0 > 0) {
+      t0 = gh[GH_PART_T1 + p0 - 1];
+      off = gh[GH_PART_O1 + p0 - 1];
+    }
+    if (p1 < GH_PARTS) t1 = gh[GH_PART_T1 + p1 - 1];
+  }
   GraphSum g;
   graph_sum_clear(g);
-  sum_terms<W32, WL>(P + gh[GH_TERM_OFF], gh[GH_NTERMS], gflags, e0, pv, lt, tb, g, err);
+  sum_terms<W32, WL>(P + off, t1 - t0, gflags, e0, pv, lt, tb, g, err);
   if (gflags & GF_MONO) return g.pm;
+  if (share) exchange_sums(g, (gflags & GF_NARROW) != 0, sp, xn);
   if (gflags & GF_NARROW) return magnitude(widen(g.n), e0);
   return magnitude(g.acc, e0);
 }
 
-template <int WL>
+template <int WL, bool SPLIT = false>
 __device__ __forceinline__ double graph_prob_any(const cword *__restrict__ P,
                                                  const cword *__restrict__ gh,
                                                  const uint32_t (&pv)[WL], const LdsTables *lt,
-                                                 const Tables &tb, uint32_t &err) {
+                                                 const Tables &tb, uint32_t &err, const Split &sp,
+                                                 uint32_t &xn) {
   switch (gh[GH_W32]) {
-    case 1: return graph_prob<1, WL>(P, gh, pv, lt, tb, err);
-    case 2: if constexpr (WL >= 2) return graph_prob<2, WL>(P, gh, pv, lt, tb, err); break;
-    case 3: if constexpr (WL >= 3) return graph_prob<3, WL>(P, gh, pv, lt, tb, err); break;
-    case 4: if constexpr (WL >= 4) return graph_prob<4, WL>(P, gh, pv, lt, tb, err); break;
-    case 5: if constexpr (WL >= 5) return graph_prob<5, WL>(P, gh, pv, lt, tb, err); break;
-    case 6: if constexpr (WL >= 6) return graph_prob<6, WL>(P, gh, pv, lt, tb, err); break;
-    case 7: if constexpr (WL >= 7) return graph_prob<7, WL>(P, gh, pv, lt, tb, err); break;
-    case 8: if constexpr (WL >= 8) return graph_prob<8, WL>(P, gh, pv, lt, tb, err); break;
+#define KSIM_GP(w)                                                                       \
+  case w:                                                                                \
+    if constexpr (WL >= w) return graph_prob<w, WL, SPLIT>(P, gh, pv, lt, tb, err, sp, xn); \
+    break;
+    KSIM_GP(1) KSIM_GP(2) KSIM_GP(3) KSIM_GP(4) KSIM_GP(5) KSIM_GP(6) KSIM_GP(7) KSIM_GP(8)
+#undef KSIM_GP
     default: break;
   }
   return 0.0;
@@ -639,9 +714,15 @@ __device__ __forceinline__ void write_packed_row(const ShotArgs &a, const uint32
 
 // The whole autoregressive loop of one shot of component `comp`
 // (src/cpp/ksim/kokkos_sim.cpp:214-230, src/python/ksim/sample/evaluate.py:76-85).
-template <int WL, bool NOISE = false>
+// SPLIT: the waves of sp's group share each graph's terms; every lane runs
+// the loop (lanes without a shot of their own run a copy of a valid shot,
+// so the group's barriers are reached by whole waves) and only `valid`
+// lanes of the group's first wave write.
+template <int WL, bool NOISE = false, bool SPLIT = false>
 __device__ __forceinline__ void run_shot(const ShotArgs &a, int comp, int64_t s,
-                                         const LdsTables *lt, const Tables &tb) {
+                                         const LdsTables *lt, const Tables &tb,
+                                         const Split &sp = Split{1, 0, 0, nullptr},
+                                         bool valid = true, uint32_t *xn_io = nullptr) {
   const cword *__restrict__ P = as_const(a.prog);
   const cword *__restrict__ ch = P + P[PH_DWORDS + comp];
   const int ng = (int)ch[CH_NGRAPHS], F = (int)ch[CH_F];
@@ -725,6 +806,7 @@ __device__ __forceinline__ void run_shot(const ShotArgs &a, int comp, int64_t s,
   // per-component stream: seed + 0x9E37*k (src/python/ksim/sample/sampler.py:63)
   uint64_t rs = rng_seed(a.seed + 0x9E37ull * (uint64_t)comp, (uint64_t)(a.shot_base + s));
   uint32_t err = 0;
+  uint32_t xn = xn_io ? *xn_io : 0u;
   double prev = 0.0;
   // graph 0 gives the normaliser, graph i+1 the probability of output i = 1
   // (one evaluation site, so the graph code is inlined once); the decided
@@ -732,7 +814,7 @@ __device__ __forceinline__ void run_shot(const ShotArgs &a, int comp, int64_t s,
   for (int i = 0; i < ng; ++i) {
     const cword *gh = ghs + (size_t)i * GH_DWORDS;
     if (i > 0) set_param_bit<WL>(pv, F + i - 1, 1u);  // trying bit
-    const double p1 = graph_prob_any<WL>(P, gh, pv, lt, tb, err);
+    const double p1 = graph_prob_any<WL, SPLIT>(P, gh, pv, lt, tb, err, sp, xn);
     if (i == 0) {
       prev = p1;
       continue;
@@ -742,6 +824,11 @@ __device__ __forceinline__ void run_shot(const ShotArgs &a, int comp, int64_t s,
     const uint32_t bit = (u < __ddiv_rn(p1, prev)) ? 1u : 0u;
     set_param_bit<WL>(pv, F + i - 1, bit);
     prev = bit ? p1 : __dsub_rn(prev, p1);
+  }
+  if (xn_io) *xn_io = xn;
+  if (SPLIT && (!valid || sp.part != 0)) {
+    if (err) atomicOr(a.err, err);
+    return;
   }
   // the output row in one burst of stores, so each cache line of the rows is
   // completed while it is resident

@@ -439,6 +439,15 @@ class HipProgramSampler:
     def handle(self):
         return self._native._h
 
+    def split_hint(self, comp: int, tile_ops: float) -> tuple[int, float]:
+        """(term split, per-wave tile cost in algorithmic ops) the persistent
+        kernels use for component ``comp`` under a per-wave tile budget."""
+        L = C.c_int32(0)
+        cost = C.c_double(0.0)
+        _lib.check(_lib.load().ksim_program_split_hint(self.handle, int(comp), float(tile_ops),
+                                                       C.byref(L), C.byref(cost)))
+        return int(L.value), float(cost.value)
+
     def stats(self) -> dict:
         st = np.zeros(8, dtype=np.float64)
         _lib.check(_lib.load().ksim_program_stats(self._native._h, _lib.ptr(st, C.c_double)))

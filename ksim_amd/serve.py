@@ -65,7 +65,7 @@ class _Active:
     def __init__(self, req, sampler, slot, d_f, d_out, tiles):
         self.req, self.sampler, self.slot = req, sampler, slot
         self.d_f, self.d_out = d_f, d_out
-        self.tiles = tiles            # deque of (comp, shot0, count)
+        self.tiles = tiles            # deque of (comp, shot0, count, split)
         self.outstanding = 0          # tiles launched, not yet retired
         self.bytes = d_f.numel() + d_out.numel()
 
@@ -77,7 +77,8 @@ class ServingLoop:
                     matrices and outputs)
     max_slots       requests admitted at once
     step_ms         target duration of one persistent launch
-    tile_shots      shots per work item (a multiple of 64)
+    tile_shots      shots per wave group per work item (a multiple of 64);
+                    an item of term split L covers tile_shots * 4 / L shots
     blocks          persistent blocks per launch (default: 6 per CU)
     """
 
@@ -97,7 +98,7 @@ class ServingLoop:
         self.tile_shots = int(tile_shots)
         self.max_items = int(max_items)
         self.blocks = int(blocks or device_info()["cus"] * 6)
-        self.min_items = min(self.blocks * 4, max_items)  # 4 waves per 256-thread block
+        self.min_items = min(self.blocks, max_items)  # one item per persistent block
         h = C.c_void_p()
         _lib.check(self.lib.ksim_sched_create(self.max_slots, self.max_items, C.byref(h)))
         self._h = h
@@ -113,6 +114,15 @@ class ServingLoop:
         # algorithmic int64 ops per second of the whole device (sets the work
         # put into one step)
         self.rate = float(rate)
+
+    def _split(self, sampler, comp: int) -> int:
+        """Term split of a request's tiles: a 64-shot tile should take each
+        wave at most half of its share of one step's work."""
+        target = self.rate * self.step_s / (self.blocks * 4) / 2
+        L = C.c_int32(1)
+        _lib.check(self.lib.ksim_program_split_hint(sampler.handle, int(comp), float(target),
+                                                    C.byref(L), None))
+        return int(L.value)
 
     # ------------------------------------------------------------ programs
     @staticmethod
@@ -237,9 +247,12 @@ class ServingLoop:
                     _lib.check(lib.ksim_sched_set_slot(
                         self._h, slot, sampler.handle, d_f.data_ptr() if d_f.numel() else None,
                         int(r.f.shape[1]), r.shots, int(r.seed), d_out.data_ptr()))
-                    tiles = deque((c, s0, min(self.tile_shots, r.shots - s0))
-                                  for c in range(sampler.n_comp)
-                                  for s0 in range(0, r.shots, self.tile_shots))
+                    tiles = deque()
+                    for c in range(sampler.n_comp):
+                        L = self._split(sampler, c)
+                        span = self.tile_shots * (4 // L)
+                        tiles.extend((c, s0, min(span, r.shots - s0), L)
+                                     for s0 in range(0, r.shots, span))
                     a = _Active(r, sampler, slot, d_f, d_out, tiles)
                     r.admitted = now()
                     if not tiles:  # zero shots
@@ -264,10 +277,10 @@ class ServingLoop:
                 if len(inflight) < len(self.streams) and any(a.tiles for a in active):
                     budget = self.rate * self.step_s
                     items, taken, est = [], [], 0.0
-                    # round-robin over active requests, one tile each per pass,
+                    # round-robin over active requests, one item each per pass,
                     # until the step holds the estimated work of step_ms and at
-                    # least one tile per wave of the persistent grid (a tile
-                    # is one wave's work: fewer tiles leave SIMDs idle)
+                    # least one item per block of the persistent grid (an item
+                    # is one block's work: fewer items leave SIMDs idle)
                     def short():
                         return est < budget or len(items) < self.min_items
                     while short() and len(items) < self.max_items:
@@ -275,8 +288,8 @@ class ServingLoop:
                         for a in active:
                             if not a.tiles:
                                 continue
-                            c, s0, cnt = a.tiles.popleft()
-                            items.append((a.slot, c, s0, cnt))
+                            c, s0, cnt, L = a.tiles.popleft()
+                            items.append((a.slot, c, s0, cnt, L))
                             taken.append(a)
                             a.outstanding += 1
                             est += a.sampler.ops_per_shot * cnt / max(a.sampler.n_comp, 1)
@@ -285,7 +298,7 @@ class ServingLoop:
                                 break
                         if not progressed:
                             break
-                    arr = np.asarray(items, dtype=np.int32).reshape(-1, 4)
+                    arr = np.asarray(items, dtype=np.int32).reshape(-1, 5)
                     sst = self.streams[n_steps % len(self.streams)]
                     _lib.check(lib.ksim_sched_launch(self._h, _lib.ptr(arr, C.c_int32), len(items),
                                                      self.blocks, sst.cuda_stream))

@@ -60,8 +60,8 @@ def test_autotuned_batch_equals_alone(tmp_path, monkeypatch):
     monkeypatch.setenv("KSIM_TUNE_CACHE", str(tmp_path / "tune.json"))
     samplers, fs, seeds, alone = build(SPECS)
     b = BatchedSampler.tuned(samplers, fs, seeds, reps=1, wave_shots=(64, 128),
-                             blocks_per_cu=(2, 5))
-    assert not b.tune["cached"] and len(b.tune["trials"]) == 4
+                             blocks_per_cu=(2, 5), split_shares=(8, None))
+    assert not b.tune["cached"] and len(b.tune["trials"]) == 8
     from ksim_amd.batch import device_info
     cus = device_info()["cus"]
     assert b.wave_shots in (64, 128) and b.blocks in (2 * cus, 5 * cus)
@@ -71,5 +71,34 @@ def test_autotuned_batch_equals_alone(tmp_path, monkeypatch):
     check_same(b, samplers, alone)
     # second call: served from the cache file
     again = autotune.tune_batch(samplers, fs, seeds, reps=1, wave_shots=(64, 128),
-                                blocks_per_cu=(2, 5))
+                                blocks_per_cu=(2, 5), split_shares=(8, None))
     assert again["cached"] and again["wave_shots"] == b.wave_shots
+
+
+# terms shared by 1, 2 or 4 waves: heavier programs (many terms per graph,
+# several components, a partial last tile) under a tiny tile budget (every
+# component split as far as its graphs allow) and with the split off
+SPLIT_SPECS = [dict(n_q=10, t=14, F=12, n_out=10, seed=11, B=300),
+               dict(n_q=8, t=10, F=6, n_out=7, seed=12, B=129),
+               dict(n_q=6, t=0, F=20, n_out=8, seed=13, B=200),
+               dict(n_q=12, t=16, F=30, n_out=6, seed=14, B=64)]
+
+
+@pytest.mark.parametrize("tile_ops", [1.0, 1e300])
+def test_term_split_batch_equals_alone(tile_ops):
+    import torch
+    samplers, fs, seeds, alone = build(SPLIT_SPECS)
+    splits = {s.split_hint(c, tile_ops)[0] for s in samplers for c in range(s.n_comp)}
+    if tile_ops < 1e100:
+        assert {2, 4} & splits, splits  # the split paths are exercised
+    else:
+        assert splits == {1}
+    for wave_shots in (64, 128):
+        b = BatchedSampler(samplers, fs, seeds, wave_shots=wave_shots, tile_ops=tile_ops)
+        for use_graph in (False, True, True):
+            for o in b.out:
+                o.fill_(7)
+            b.run(use_graph=use_graph)
+            torch.cuda.synchronize()
+            b.check()
+            check_same(b, samplers, alone)

@@ -28,10 +28,16 @@ def make_requests(n, seed):
     return reqs, samplers
 
 
-@pytest.mark.parametrize("tile,step_ms", [(64, 0.05), (256, 1.0)])
-def test_results_same_alone_and_in_mix(tile, step_ms):
+@pytest.mark.parametrize("tile,step_ms,rate", [(64, 0.05, 2.5e13), (256, 1.0, 2.5e13),
+                                               (64, 0.05, 1e6)])
+def test_results_same_alone_and_in_mix(tile, step_ms, rate):
+    # rate 1e6: a tiny per-step budget, so the heavier requests' tiles have
+    # their terms split over 2 or 4 waves
     reqs, samplers = make_requests(24, 3)
-    loop = ServingLoop(max_slots=8, step_ms=step_ms, tile_shots=tile, hbm_budget=1 << 30)
+    loop = ServingLoop(max_slots=8, step_ms=step_ms, tile_shots=tile, hbm_budget=1 << 30,
+                       rate=rate)
+    if rate < 1e10:
+        assert {loop._split(s, c) for s in samplers for c in range(s.n_comp)} & {2, 4}
     stats = loop.run(reqs, realtime=False)
     assert stats["requests"] == len(reqs)
     for r in reqs:

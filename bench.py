@@ -48,11 +48,38 @@ INT64_PEAK = VALU32_PEAK / 2
 HBM_PEAK = 8.0e12
 
 
-def valu_roofline(ops_per_s, bytes_per_s, traffic=None):
+# HBM bytes per dispatch of the dominant kernel, measured by two rocprofv3
+# --pmc passes (FETCH_SIZE, WRITE_SIZE, each scaled by the same counter on a
+# copy kernel of known size) and recorded in these summaries
+PROFILE_OF = {"config2": "k_sample_config2_plain_1M.md",
+              "config3": "k_sample_config3_plain_1M.md",
+              "config4": "k_batch_config4_64prog.md"}
+
+
+def profiled_traffic(workload):
+    """(bytes per dispatch, summary file) from the committed counter summary
+    of this workload's dominant kernel, or (None, None)."""
+    name = PROFILE_OF.get(workload)
+    if not name:
+        return None, None
+    path = os.path.join(HERE, "profiles", name)
+    try:
+        with open(path) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_dispatch:"):
+                    return float(line.split(":")[1]), "profiles/" + name
+    except OSError:
+        pass
+    return None, None
+
+
+def valu_roofline(ops_per_s, bytes_per_s, workload=None):
     """Roofline object of a VALU-bound launch: algorithmic ops/s against the
-    VALU lane-op peak, with the int64-op peak and HBM rate beside it."""
+    VALU lane-op peak, with the int64-op peak and HBM rate beside it;
+    ``traffic`` is the profiled HBM bytes per dispatch."""
+    traffic, src = profiled_traffic(workload)
     return dict(bound="valu", achieved=ops_per_s, peak=VALU32_PEAK, unit="op/s",
-                frac=ops_per_s / VALU32_PEAK, traffic=traffic,
+                frac=ops_per_s / VALU32_PEAK, traffic=traffic, traffic_source=src,
                 frac_int64_peak=ops_per_s / INT64_PEAK,
                 hbm_GBps=bytes_per_s / 1e9, hbm_frac=bytes_per_s / HBM_PEAK)
 
@@ -171,7 +198,7 @@ def plain_result(name, steps, warmup, with_cpu=True):
     res = dict(workload=name, desc=w["desc"], shots=w["B"], ms_per_step=ms,
                shots_per_s=shots_s, us_per_shot=1e6 / shots_s,
                kernel_ms=kern * 1e3, ops_per_shot=pl.ops, stats=pl.s.stats(),
-               roofline=valu_roofline(achieved, pl.bytes_per_shot * w["B"] / kern))
+               roofline=valu_roofline(achieved, pl.bytes_per_shot * w["B"] / kern, name))
     if with_cpu:
         # about 10-20 s of CPU work
         probe = cpu_baseline_port(w["prog"], w["F"], 200)
@@ -243,7 +270,7 @@ def noise_result(prog, B, steps, warmup, with_cpu=True):
     return res
 
 
-def batched_result(steps, warmup, n_programs=64, tune=True):
+def batched_result(steps, warmup, n_programs=64, tune=True, shape=None):
     """config4: the same 64 programs sampled (a) by one plain launch each,
     back to back on one stream, and (b) by one persistent batched launch
     replayed from a hipGraph; both with resident inputs."""
@@ -270,7 +297,15 @@ def batched_result(steps, warmup, n_programs=64, tune=True):
                                                       f.shape[0], f.shape[1], 0, sd, None,
                                                       o.data_ptr(), st))
     # launch shape from the autotuner (timed on this mix, outside the timed region)
-    if tune:
+    if shape is not None:
+        # a given launch shape (wave_shots, split share or None, blocks per CU)
+        from ksim_amd.autotune import split_budget
+        ws, share, bpc = shape
+        tops = split_budget(samplers, fs, device_info()["cus"], share)
+        batch = BatchedSampler(samplers, fs, seeds, wave_shots=ws, blocks_per_cu=bpc,
+                               tile_ops=tops)
+        batch.tune = dict(wave_shots=ws, split_share=share, blocks_per_cu=bpc, tune_s=0.0)
+    elif tune:
         batch = BatchedSampler.tuned(samplers, fs, seeds, use_cache=False, log=log)
     else:
         batch = BatchedSampler(samplers, fs, seeds)
@@ -297,7 +332,7 @@ def batched_result(steps, warmup, n_programs=64, tune=True):
                            split_share=batch.tune.get("split_share", 8),
                            blocks_per_cu=batch.tune["blocks_per_cu"],
                            tune_s=batch.tune["tune_s"]),
-                roofline=valu_roofline(achieved, batch.bytes / kern_b))
+                roofline=valu_roofline(achieved, batch.bytes / kern_b, "config4"))
 
 
 def tcount_sweep(ts=(0, 4, 8, 12, 16, 20, 24, 32, 40), shots=100_000, n_q=30):
@@ -380,6 +415,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true",
                     help="config4: default launch shape instead of the autotuner's")
+    ap.add_argument("--batch-shape", default=None,
+                    help="config4: launch shape WAVE_SHOTS,SPLIT_SHARE|none,BLOCKS_PER_CU "
+                         "instead of the autotuner's (e.g. 64,none,5)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="launch one elementwise int32 copy kernel of this size first, so "
                          "counter runs can scale FETCH_SIZE / WRITE_SIZE against known bytes")
@@ -399,7 +437,11 @@ def main():
         log(f"[bench] calibration copy: {n * 4} bytes read, {n * 4} bytes written")
         del src, dst
     if args.workload == "config4":
-        r4 = batched_result(args.steps, args.warmup, tune=not args.no_tune)
+        shape = None
+        if args.batch_shape:
+            ws, share, bpc = args.batch_shape.split(",")
+            shape = (int(ws), None if share.lower() == "none" else float(share), int(bpc))
+        r4 = batched_result(args.steps, args.warmup, tune=not args.no_tune, shape=shape)
         print(json.dumps({"metric": "config4 batched", **r4}), flush=True)
         return
     if args.workload == "config5":

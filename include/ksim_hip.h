@@ -159,9 +159,15 @@ int ksim_program_sample_device_packed(ksim_program *p, const uint8_t *d_f, int64
                                       int32_t f_cols, int64_t shot_base, uint64_t seed,
                                       const double *d_U, uint32_t *d_out, void *stream);
 
-/* Launch shape of the plain kernel: threads per block and lanes per shot
- * (1 = each lane evaluates every term of its own shot).  Only (256, 1) is
- * implemented; other shapes return KSIM_E_INVALID. */
+/* Launch shape of the plain kernel: threads per block (256) and term
+ * split, the waves that share one 64-shot tile's term sums: 1 (each lane
+ * evaluates every term of its own shot), 2 or 4, or 0 (default: chosen per
+ * launch, splitting only launches whose tiles cannot fill the GPU's
+ * resident wave slots).  A split launch uses an exchange area of up to
+ * 64 MiB that the program allocates on its first split launch (so that
+ * first launch is not graph-capturable) and reuses: split launches of one
+ * program must not overlap on different streams.  No reference
+ * counterpart (the reference runs one thread per shot). */
 int ksim_program_set_tiles(ksim_program *p, int32_t block_threads,
                            int32_t term_split);
 

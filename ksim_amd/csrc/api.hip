@@ -146,6 +146,33 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KSIM_WAV
   run_shot<WL, NOISE>(a, (int)blockIdx.y, s, &lt, tb);
 }
 
+// Plain path with a term split: L waves (2 or 4) share each graph's terms
+// on one 64-shot tile, for launches with too few shots to fill the GPU one
+// lane per shot.  Block b takes the tile groups b, b + gridDim.x, ... (4/L
+// tiles per group); its exchange area is xch + b * XCH_INT4_PER_BLOCK.
+template <int WL, bool NOISE = false>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KSIM_WAVES_PER_EU))) void k_sample_split(
+    ShotArgs a, Tables tb, int L, int4 *xch) {
+  __shared__ LdsTables lt;
+  load_tables(&lt, tb);
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int G = 4 / L;
+  const Split sp{L, wave % L, wave - wave % L, xch + (size_t)blockIdx.x * XCH_INT4_PER_BLOCK};
+  const int64_t ntiles = (a.B + 63) >> 6;
+  const int64_t ngroups = (ntiles + G - 1) / G;
+  uint32_t xn = 0;
+  for (int64_t gi = blockIdx.x; gi < ngroups; gi += gridDim.x) {
+    // every wave of the block runs this iteration (the loop bound is the
+    // block's), so the group's exchanges see all of its waves
+    const int64_t tile = gi * G + wave / L;
+    const int64_t off = tile * 64 + lane;
+    const bool valid = off < a.B;
+    const int64_t s = min(off, a.B - 1);
+    run_shot<WL, NOISE, true>(a, (int)blockIdx.y, s, &lt, tb, sp, valid, &xn);
+  }
+}
+
 // Noise rows: lane per shot writes its f row (B, f_cols) u8, columns
 // [0, num_f), 128 f bits per pass over the shot's noise stream.
 __global__ __launch_bounds__(kBlock) void k_noise_rows(NoiseDev nz, int64_t B, int64_t shot_base,
@@ -318,7 +345,7 @@ struct ksim_program {
   int block_threads = kBlock;
   int out_words = 0;              // packed output row words: ceil(num_outputs / 32)
   bool packed_full_rows = false;  // one component writes whole packed rows in order
-  int term_split = 1;
+  int term_split = 0;  // waves per 64-shot tile in plain launches; 0: chosen per launch
   // staging for the host-pointer API
   uint8_t *d_f = nullptr;
   size_t f_cap = 0;
@@ -330,7 +357,68 @@ struct ksim_program {
   size_t params_cap = 0;
   void *d_res = nullptr;
   size_t res_cap = 0;
+  int4 *d_xch = nullptr;  // exchange areas of term-split plain launches
+  int xch_blocks = 0;
 };
+
+namespace {
+
+// Per-wave work of one shot of component c when L waves (1, 2, 4) share each
+// graph's terms, in record units (a term costs about 40 records of
+// materialisation, a graph about 60 of decision work; single-term graphs
+// are evaluated by every wave of the group, the others add an exchange).
+void comp_work(const ksim_program *p, int c, double w[3]) {
+  w[0] = w[1] = w[2] = 0.0;
+  const std::vector<uint32_t> &gh = p->graph_hdr[c];
+  for (int i = 0; i < p->n_graphs[c]; ++i) {
+    const uint32_t *h = &gh[(size_t)i * GH_DWORDS];
+    const double body = (double)h[GH_NREC] + 40.0 * h[GH_NTERMS];
+    w[0] += body + 60.0;
+    const bool mono = h[GH_FLAGS] & GF_MONO;
+    w[1] += mono ? body + 60.0 : body / 2 + 80.0;
+    w[2] += mono ? body + 60.0 : body / 4 + 80.0;
+  }
+}
+
+// algorithmic ops per shot of component c (the program's ops shared out by work)
+double comp_ops(const ksim_program *p, int c) {
+  double tot = 0.0, mine = 0.0, w[3];
+  for (int k = 0; k < p->n_comp; ++k) {
+    comp_work(p, k, w);
+    tot += w[0];
+    if (k == c) mine = w[0];
+  }
+  return tot > 0 ? p->st.ops * mine / tot : 0.0;
+}
+
+// Term split for component c: the smallest L whose 64-shot tile costs each
+// wave at most `tile_ops` algorithmic ops, among the splits that keep at
+// least 60% of the waves' work useful; the cheapest such tile otherwise.
+int split_for(const ksim_program *p, int c, double tile_ops) {
+  double w[3];
+  comp_work(p, c, w);
+  if (w[0] <= 0) return 1;
+  const double scale = comp_ops(p, c) / w[0];
+  const int Ls[3] = {1, 2, 4};
+  int best = 1;
+  for (int k = 0; k < 3; ++k) {
+    const double eff = w[0] / (Ls[k] * w[k]);
+    if (k > 0 && eff < 0.6) break;
+    best = Ls[k];
+    if (64.0 * w[k] * scale <= tile_ops) break;
+  }
+  return best;
+}
+
+// per-wave algorithmic ops of a 64-shot tile of component c at split L
+double tile_cost(const ksim_program *p, int c, int L) {
+  double w[3];
+  comp_work(p, c, w);
+  if (w[0] <= 0) return 0.0;
+  return 64.0 * w[L == 1 ? 0 : L == 2 ? 1 : 2] * comp_ops(p, c) / w[0];
+}
+
+}  // namespace
 
 namespace {
 
@@ -364,6 +452,43 @@ void check_err_sync(ksim_program *p, hipStream_t st) {
   if (e) throw RangeError(err_text(e));
 }
 
+// most blocks of a term-split plain launch (they loop over tile groups)
+constexpr int kSplitBlocks = 2048;
+
+// Waves per 64-shot tile for a plain launch of B shots: p->term_split when
+// set, else the L in {1, 2, 4} with the most useful wave slots,
+// min(L * tiles, resident slots) * efficiency(L), over every component (a
+// launch with at least as many tiles as resident wave slots gets L = 1).
+int plain_split(const ksim_program *p, int64_t B) {
+  if (p->term_split > 0) return p->term_split;
+  static int slots = 0;
+  if (!slots) {
+    int dev = 0, cus = 256;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "CU count");
+    slots = cus * 4 * KSIM_WAVES_PER_EU;
+  }
+  const double tiles = (double)((B + 63) / 64);
+  if (tiles >= slots) return 1;
+  int best = 1;
+  double best_v = tiles;
+  const int Ls[2] = {2, 4};
+  for (int k = 0; k < 2; ++k) {
+    double eff = 1.0;
+    for (int c = 0; c < p->n_comp; ++c) {
+      double w[3];
+      comp_work(p, c, w);
+      if (w[0] > 0) eff = std::min(eff, w[0] / (Ls[k] * w[k + 1]));
+    }
+    const double v = std::min(Ls[k] * tiles, (double)slots) * eff;
+    if (v > best_v * 1.05) {
+      best = Ls[k];
+      best_v = v;
+    }
+  }
+  return best;
+}
+
 void launch_sample(ksim_program *p, const uint8_t *d_f, int64_t B, int32_t f_cols,
                    int64_t shot_base, uint64_t seed, const double *d_U, uint8_t *d_out,
                    hipStream_t st, int out_words = 0, const NoiseDev *noise = nullptr) {
@@ -387,8 +512,41 @@ void launch_sample(ksim_program *p, const uint8_t *d_f, int64_t B, int32_t f_col
   }
   const int64_t nb = (B + kBlock - 1) / kBlock;
   if (nb > 0x7fffffff) throw std::invalid_argument("too many shots for one launch");
-  dim3 grid((unsigned)nb, (unsigned)p->n_comp);
   const bool nz = a.noise.C > 0;
+  const int L = plain_split(p, B);
+  if (L > 1) {
+    // blocks: one per group of 4/L tiles, at most kSplitBlocks (the
+    // exchange area is 32 KiB per block); blocks loop over the groups
+    const int64_t groups = ((B + 63) / 64 + (4 / L) - 1) / (4 / L);
+    const int blocks = (int)std::min<int64_t>(groups, kSplitBlocks);
+    if (blocks > p->xch_blocks) {
+      // the program's previous split launches must be over before the area goes
+      hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+      if (p->d_xch) hip_check(hipFree(p->d_xch), "hipFree");
+      p->d_xch = nullptr;
+      p->xch_blocks = 0;
+      hip_check(hipMalloc(&p->d_xch, sizeof(int4) * XCH_INT4_PER_BLOCK * (size_t)blocks),
+                "hipMalloc(exchange)");
+      p->xch_blocks = blocks;
+    }
+    dim3 g2((unsigned)blocks, (unsigned)p->n_comp);
+#define KSIM_SPLIT_LAUNCH(W, N) \
+  hipLaunchKernelGGL((k_sample_split<W, N>), g2, dim3(kBlock), 0, st, a, t.view(), L, p->d_xch)
+    switch (p->wl * 2 + (nz ? 1 : 0)) {
+      case 2: KSIM_SPLIT_LAUNCH(1, false); break;
+      case 3: KSIM_SPLIT_LAUNCH(1, true); break;
+      case 4: KSIM_SPLIT_LAUNCH(2, false); break;
+      case 5: KSIM_SPLIT_LAUNCH(2, true); break;
+      case 8: KSIM_SPLIT_LAUNCH(4, false); break;
+      case 9: KSIM_SPLIT_LAUNCH(4, true); break;
+      case 16: KSIM_SPLIT_LAUNCH(8, false); break;
+      default: KSIM_SPLIT_LAUNCH(8, true); break;
+    }
+#undef KSIM_SPLIT_LAUNCH
+    hip_check(hipGetLastError(), "k_sample_split launch");
+    return;
+  }
+  dim3 grid((unsigned)nb, (unsigned)p->n_comp);
   switch (p->wl * 2 + (nz ? 1 : 0)) {
     case 2: hipLaunchKernelGGL((k_sample<1, false>), grid, dim3(kBlock), 0, st, a, t.view()); break;
     case 3: hipLaunchKernelGGL((k_sample<1, true>), grid, dim3(kBlock), 0, st, a, t.view()); break;
@@ -688,7 +846,8 @@ int ksim_program_set_tiles(ksim_program *p, int32_t block_threads, int32_t term_
   return guarded([&] {
     if (!p) throw std::invalid_argument("program is NULL");
     if (block_threads != kBlock) throw std::invalid_argument("block_threads must be 256");
-    if (term_split != 1) throw std::invalid_argument("term_split must be 1");
+    if (term_split != 0 && term_split != 1 && term_split != 2 && term_split != 4)
+      throw std::invalid_argument("term_split must be 0 (automatic), 1, 2 or 4");
     p->block_threads = block_threads;
     p->term_split = term_split;
   });
@@ -721,6 +880,7 @@ void ksim_program_destroy(ksim_program *p) {
   if (p->d_U) (void)hipFree(p->d_U);
   if (p->d_params) (void)hipFree(p->d_params);
   if (p->d_res) (void)hipFree(p->d_res);
+  if (p->d_xch) (void)hipFree(p->d_xch);
   delete p;
 }
 
@@ -828,64 +988,6 @@ int ksim_device_info(int64_t *info4) {
 }  // extern "C"
 
 // ------------------------------------------------------------------ batch
-namespace {
-
-// Per-wave work of one shot of component c when L waves (1, 2, 4) share each
-// graph's terms, in record units (a term costs about 40 records of
-// materialisation, a graph about 60 of decision work; single-term graphs
-// are evaluated by every wave of the group, the others add an exchange).
-void comp_work(const ksim_program *p, int c, double w[3]) {
-  w[0] = w[1] = w[2] = 0.0;
-  const std::vector<uint32_t> &gh = p->graph_hdr[c];
-  for (int i = 0; i < p->n_graphs[c]; ++i) {
-    const uint32_t *h = &gh[(size_t)i * GH_DWORDS];
-    const double body = (double)h[GH_NREC] + 40.0 * h[GH_NTERMS];
-    w[0] += body + 60.0;
-    const bool mono = h[GH_FLAGS] & GF_MONO;
-    w[1] += mono ? body + 60.0 : body / 2 + 80.0;
-    w[2] += mono ? body + 60.0 : body / 4 + 80.0;
-  }
-}
-
-// algorithmic ops per shot of component c (the program's ops shared out by work)
-double comp_ops(const ksim_program *p, int c) {
-  double tot = 0.0, mine = 0.0, w[3];
-  for (int k = 0; k < p->n_comp; ++k) {
-    comp_work(p, k, w);
-    tot += w[0];
-    if (k == c) mine = w[0];
-  }
-  return tot > 0 ? p->st.ops * mine / tot : 0.0;
-}
-
-// Term split for component c: the smallest L whose 64-shot tile costs each
-// wave at most `tile_ops` algorithmic ops, among the splits that keep at
-// least 60% of the waves' work useful; the cheapest such tile otherwise.
-int split_for(const ksim_program *p, int c, double tile_ops) {
-  double w[3];
-  comp_work(p, c, w);
-  if (w[0] <= 0) return 1;
-  const double scale = comp_ops(p, c) / w[0];
-  const int Ls[3] = {1, 2, 4};
-  int best = 1;
-  for (int k = 0; k < 3; ++k) {
-    const double eff = w[0] / (Ls[k] * w[k]);
-    if (k > 0 && eff < 0.6) break;
-    best = Ls[k];
-    if (64.0 * w[k] * scale <= tile_ops) break;
-  }
-  return best;
-}
-
-// per-wave algorithmic ops of a 64-shot tile of component c at split L
-double tile_cost(const ksim_program *p, int c, int L) {
-  double w[3];
-  comp_work(p, c, w);
-  if (w[0] <= 0) return 0.0;
-  return 64.0 * w[L == 1 ? 0 : L == 2 ? 1 : 2] * comp_ops(p, c) / w[0];
-}
-
-}  // namespace
 
 struct ksim_batch {
   std::vector<ksim_program *> progs;

@@ -439,6 +439,12 @@ class HipProgramSampler:
     def handle(self):
         return self._native._h
 
+    def set_term_split(self, term_split: int) -> None:
+        """Waves that share one 64-shot tile's terms in plain launches: 1, 2
+        or 4, or 0 (default) to choose per launch (split only launches too
+        small to fill the GPU).  Results do not depend on it."""
+        _lib.check(_lib.load().ksim_program_set_tiles(self.handle, 256, int(term_split)))
+
     def split_hint(self, comp: int, tile_ops: float) -> tuple[int, float]:
         """(term split, per-wave tile cost in algorithmic ops) the persistent
         kernels use for component ``comp`` under a per-wave tile budget."""

@@ -232,3 +232,30 @@ def test_packed_rows_equal_byte_rows(which):
     packed = s.sample_packed(f, 99)
     assert packed.shape == (B, (s.num_outputs + 31) // 32)
     assert np.array_equal(unpack_outputs(packed, s.num_outputs), want)
+
+
+@pytest.mark.parametrize("n_components", [1, 2])
+def test_plain_term_split_is_identical(n_components):
+    # 1, 2 or 4 waves per 64-shot tile (and the automatic choice): the same
+    # bits through every entry point, byte rows, packed rows, fed uniforms
+    # and device-drawn noise, on a shot count that ends in a partial tile
+    from ksim_amd.noise import NoiseModel
+    from tests.test_gpu_noise import model_b
+    prog = synth.shape_program(n_q=10, t=14, F=20, n_out=9, seed=41, n_components=n_components)
+    s = HipProgramSampler(prog)
+    B = 1000
+    f = synth.random_f(B, s.min_num_f, 0.4, 3)
+    U = np.random.default_rng(5).random((B, s.tot_out))
+    probs, T = model_b(num_f=s.min_num_f, n_ch=60, p=0.02, seed=2)
+    nm = NoiseModel(probs, T, seed=9)
+    got = {}
+    for L in (1, 2, 4, 0):
+        s.set_term_split(L)
+        got[L] = (s.sample(f, 77), s.sample_packed(f, 77), s.sample_uniforms(f, U),
+                  s.sample_noisy(nm, B, 78))
+    for L in (2, 4, 0):
+        for k in range(4):
+            assert np.array_equal(got[L][k], got[1][k]), (L, k)
+    # and the fed-uniform stream is the oracle's (first 300 shots: the
+    # big-integer oracle is slow)
+    assert np.array_equal(got[4][2][:300], exact.sample_program(prog, f[:300], U[:300]))

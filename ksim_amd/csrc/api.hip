@@ -149,19 +149,19 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KSIM_WAV
 // Plain path with a term split: L waves (2 or 4) share each graph's terms
 // on one 64-shot tile, for launches with too few shots to fill the GPU one
 // lane per shot.  Block b takes the tile groups b, b + gridDim.x, ... (4/L
-// tiles per group); its exchange area is xch + b * XCH_INT4_PER_BLOCK.
+// tiles per group).
 template <int WL, bool NOISE = false>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KSIM_WAVES_PER_EU))) void k_sample_split(
-    ShotArgs a, Tables tb, int L, int4 *xch) {
+    ShotArgs a, Tables tb, int L) {
   __shared__ LdsTables lt;
+  __shared__ int4 xch[XCH_INT4];
   load_tables(&lt, tb);
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int G = 4 / L;
-  const Split sp{L, wave % L, wave - wave % L, xch + (size_t)blockIdx.x * XCH_INT4_PER_BLOCK};
+  const Split sp{L, wave % L, wave - wave % L, xch};
   const int64_t ntiles = (a.B + 63) >> 6;
   const int64_t ngroups = (ntiles + G - 1) / G;
-  uint32_t xn = 0;
   for (int64_t gi = blockIdx.x; gi < ngroups; gi += gridDim.x) {
     // every wave of the block runs this iteration (the loop bound is the
     // block's), so the group's exchanges see all of its waves
@@ -169,7 +169,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KSIM_WAV
     const int64_t off = tile * 64 + lane;
     const bool valid = off < a.B;
     const int64_t s = min(off, a.B - 1);
-    run_shot<WL, NOISE, true>(a, (int)blockIdx.y, s, &lt, tb, sp, valid, &xn);
+    run_shot<WL, NOISE, true>(a, (int)blockIdx.y, s, &lt, tb, sp, valid);
   }
 }
 
@@ -249,9 +249,8 @@ __global__ __launch_bounds__(kBlock) void k_eval(EvalArgs a, Tables tb) {
     a.exp2[s] = e0;
   } else {
     // the sampler's own probability path (table magnitudes, 64-bit sums)
-    uint32_t xn = 0;
     a.absout[s] = graph_prob_any<WL>(as_const(a.prog), as_const(a.gh), pv, &lt, tb, err,
-                                     Split{1, 0, 0, nullptr}, xn);
+                                     Split{1, 0, 0, nullptr});
   }
   if (err) atomicOr(a.err, err);
 }
@@ -289,17 +288,16 @@ __global__ void k_reset_cursor(uint32_t *cursor, uint32_t *clear) {
 template <int WL>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KSIM_WAVES_PER_EU))) void k_batch(
     const BatchProg *__restrict__ progs, const BatchItem *__restrict__ items, uint32_t n_items,
-    uint32_t *cursor, uint32_t *err, Tables tb, int4 *xch) {
+    uint32_t *cursor, uint32_t *err, Tables tb) {
   __shared__ LdsTables lt;
   __shared__ uint32_t s_item;
+  __shared__ int4 xch[XCH_INT4];
   load_tables(&lt, tb);
   // the wave index is the same on every lane of a wave: readfirstlane lets
   // the compiler keep what derives from it (term ranges, program offsets)
   // on the scalar path
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  int4 *my_xch = xch + (size_t)blockIdx.x * XCH_INT4_PER_BLOCK;
-  uint32_t xn = 0;  // exchanges so far (the same count in every wave of the block)
   for (;;) {
     if (threadIdx.x == 0) s_item = atomicAdd(cursor, 1u);
     __syncthreads();
@@ -315,7 +313,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KSIM_WAV
     a.out_words = 0; a.out_atomic = 0;
     a.noise = NoiseDev{};
     const int L = it.split, G = 4 / L;
-    const Split sp{L, wave % L, wave - wave % L, my_xch};
+    const Split sp{L, wave % L, wave - wave % L, xch};
     const int ntiles = (it.count + 63) >> 6;
     const int rounds = (ntiles + G - 1) / G;
     for (int r = 0; r < rounds; ++r) {
@@ -325,7 +323,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KSIM_WAV
       const int off = tile * 64 + lane;
       const bool valid = off < it.count;
       const int64_t s = it.shot0 + min(off, it.count - 1);
-      run_shot<WL, false, true>(a, it.comp, s, &lt, tb, sp, valid, &xn);
+      run_shot<WL, false, true>(a, it.comp, s, &lt, tb, sp, valid);
     }
   }
 }
@@ -357,8 +355,6 @@ struct ksim_program {
   size_t params_cap = 0;
   void *d_res = nullptr;
   size_t res_cap = 0;
-  int4 *d_xch = nullptr;  // exchange areas of term-split plain launches
-  int xch_blocks = 0;
 };
 
 namespace {
@@ -456,9 +452,11 @@ void check_err_sync(ksim_program *p, hipStream_t st) {
 constexpr int kSplitBlocks = 2048;
 
 // Waves per 64-shot tile for a plain launch of B shots: p->term_split when
-// set, else the L in {1, 2, 4} with the most useful wave slots,
-// min(L * tiles, resident slots) * efficiency(L), over every component (a
-// launch with at least as many tiles as resident wave slots gets L = 1).
+// set; else 1 when the tiles fill the resident wave slots, otherwise the
+// smallest L in {2, 4} that fills them among the splits whose estimated
+// useful share of the waves' work is at least 3/4 (every component), or
+// the largest such L (measured on 30-qubit programs at 100k shots: t = 8
+// is fastest at L = 2, t = 12..32 at L = 4).
 int plain_split(const ksim_program *p, int64_t B) {
   if (p->term_split > 0) return p->term_split;
   static int slots = 0;
@@ -468,10 +466,9 @@ int plain_split(const ksim_program *p, int64_t B) {
     hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "CU count");
     slots = cus * 4 * KSIM_WAVES_PER_EU;
   }
-  const double tiles = (double)((B + 63) / 64);
+  const int64_t tiles = (B + 63) / 64;
   if (tiles >= slots) return 1;
   int best = 1;
-  double best_v = tiles;
   const int Ls[2] = {2, 4};
   for (int k = 0; k < 2; ++k) {
     double eff = 1.0;
@@ -480,11 +477,9 @@ int plain_split(const ksim_program *p, int64_t B) {
       comp_work(p, c, w);
       if (w[0] > 0) eff = std::min(eff, w[0] / (Ls[k] * w[k + 1]));
     }
-    const double v = std::min(Ls[k] * tiles, (double)slots) * eff;
-    if (v > best_v * 1.05) {
-      best = Ls[k];
-      best_v = v;
-    }
+    if (eff < 0.75) break;
+    best = Ls[k];
+    if (Ls[k] * tiles >= slots) break;
   }
   return best;
 }
@@ -515,23 +510,13 @@ void launch_sample(ksim_program *p, const uint8_t *d_f, int64_t B, int32_t f_col
   const bool nz = a.noise.C > 0;
   const int L = plain_split(p, B);
   if (L > 1) {
-    // blocks: one per group of 4/L tiles, at most kSplitBlocks (the
-    // exchange area is 32 KiB per block); blocks loop over the groups
+    // blocks: one per group of 4/L tiles, at most kSplitBlocks; blocks
+    // loop over the groups
     const int64_t groups = ((B + 63) / 64 + (4 / L) - 1) / (4 / L);
     const int blocks = (int)std::min<int64_t>(groups, kSplitBlocks);
-    if (blocks > p->xch_blocks) {
-      // the program's previous split launches must be over before the area goes
-      hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
-      if (p->d_xch) hip_check(hipFree(p->d_xch), "hipFree");
-      p->d_xch = nullptr;
-      p->xch_blocks = 0;
-      hip_check(hipMalloc(&p->d_xch, sizeof(int4) * XCH_INT4_PER_BLOCK * (size_t)blocks),
-                "hipMalloc(exchange)");
-      p->xch_blocks = blocks;
-    }
     dim3 g2((unsigned)blocks, (unsigned)p->n_comp);
 #define KSIM_SPLIT_LAUNCH(W, N) \
-  hipLaunchKernelGGL((k_sample_split<W, N>), g2, dim3(kBlock), 0, st, a, t.view(), L, p->d_xch)
+  hipLaunchKernelGGL((k_sample_split<W, N>), g2, dim3(kBlock), 0, st, a, t.view(), L)
     switch (p->wl * 2 + (nz ? 1 : 0)) {
       case 2: KSIM_SPLIT_LAUNCH(1, false); break;
       case 3: KSIM_SPLIT_LAUNCH(1, true); break;
@@ -880,7 +865,6 @@ void ksim_program_destroy(ksim_program *p) {
   if (p->d_U) (void)hipFree(p->d_U);
   if (p->d_params) (void)hipFree(p->d_params);
   if (p->d_res) (void)hipFree(p->d_res);
-  if (p->d_xch) (void)hipFree(p->d_xch);
   delete p;
 }
 
@@ -1001,8 +985,6 @@ struct ksim_batch {
   hipGraphExec_t exec = nullptr;
   int graph_blocks = 0;
   hipStream_t graph_stream = nullptr;
-  int4 *d_xch = nullptr;  // per-block exchange areas of the term split
-  int xch_blocks = 0;
 };
 
 extern "C" {
@@ -1090,25 +1072,13 @@ int ksim_batch_create(ksim_program *const *progs, int32_t n, const uint8_t *cons
   });
 }
 
-// exchange areas for `blocks` persistent blocks (allocated outside any capture)
-static void batch_xch(ksim_batch *b, int blocks) {
-  if (blocks <= b->xch_blocks) return;
-  if (b->d_xch) hip_check(hipFree(b->d_xch), "hipFree");
-  b->d_xch = nullptr;
-  b->xch_blocks = 0;
-  hip_check(hipMalloc(&b->d_xch, sizeof(int4) * XCH_INT4_PER_BLOCK * (size_t)blocks),
-            "hipMalloc(exchange)");
-  b->xch_blocks = blocks;
-}
-
 static void batch_enqueue(ksim_batch *b, int blocks, hipStream_t st) {
   DevTables t = tables_for_current_device();
-  if (blocks > b->xch_blocks) throw std::logic_error("exchange area smaller than the grid");
   hipLaunchKernelGGL(k_reset_cursor, dim3(1), dim3(64), 0, st, b->d_cursor, b->d_cursor + 1);
   hip_check(hipGetLastError(), "k_reset_cursor");
 #define KSIM_BATCH_LAUNCH(W)                                                                  \
   hipLaunchKernelGGL(k_batch<W>, dim3(blocks), dim3(kBlock), 0, st, b->d_progs, b->d_items,    \
-                     b->n_items, b->d_cursor, b->d_cursor + 1, t.view(), b->d_xch)
+                     b->n_items, b->d_cursor, b->d_cursor + 1, t.view())
   switch (b->wl) {
     case 1: KSIM_BATCH_LAUNCH(1); break;
     case 2: KSIM_BATCH_LAUNCH(2); break;
@@ -1124,11 +1094,6 @@ int ksim_batch_run(ksim_batch *b, int32_t persistent_blocks, int32_t use_graph, 
     if (!b) throw std::invalid_argument("batch is NULL");
     if (persistent_blocks <= 0) throw std::invalid_argument("persistent_blocks must be > 0");
     hipStream_t st = (hipStream_t)stream;
-    if (persistent_blocks > b->xch_blocks) {
-      // the new area replaces one a running launch may still use
-      hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
-      batch_xch(b, persistent_blocks);
-    }
     if (!use_graph) {
       batch_enqueue(b, persistent_blocks, st);
       return;
@@ -1166,7 +1131,6 @@ void ksim_batch_destroy(ksim_batch *b) {
   if (b->d_progs) (void)hipFree(b->d_progs);
   if (b->d_items) (void)hipFree(b->d_items);
   if (b->d_cursor) (void)hipFree(b->d_cursor);
-  if (b->d_xch) (void)hipFree(b->d_xch);
   delete b;
 }
 
@@ -1193,8 +1157,6 @@ struct ksim_sched {
   hipEvent_t done[KSIM_SCHED_BUFS] = {};  // recorded after the launch's kernel
   bool done_valid[KSIM_SCHED_BUFS] = {};
   uint32_t *d_cursor = nullptr;  // [2k] cursor of set k, [2k+1] scratch; [64] error word
-  int4 *d_xch[KSIM_SCHED_BUFS] = {};  // exchange areas of set k's persistent blocks
-  int xch_blocks = 0;
   int next = 0;
 };
 
@@ -1265,20 +1227,6 @@ int ksim_sched_launch(ksim_sched *q, const int32_t *items, int32_t n, int32_t bl
     if (n == 0) return;
     if (!items) throw std::invalid_argument("items is NULL");
     hipStream_t st = (hipStream_t)stream;
-    if (blocks > q->xch_blocks) {
-      // every set's area grows; launches in flight may still use the old ones
-      for (int j = 0; j < KSIM_SCHED_BUFS; ++j)
-        if (q->done_valid[j]) hip_check(hipEventSynchronize(q->done[j]), "hipEventSynchronize");
-      for (int j = 0; j < KSIM_SCHED_BUFS; ++j) {
-        if (q->d_xch[j]) hip_check(hipFree(q->d_xch[j]), "hipFree");
-        q->d_xch[j] = nullptr;
-      }
-      q->xch_blocks = 0;
-      for (int j = 0; j < KSIM_SCHED_BUFS; ++j)
-        hip_check(hipMalloc(&q->d_xch[j], sizeof(int4) * XCH_INT4_PER_BLOCK * (size_t)blocks),
-                  "hipMalloc(exchange)");
-      q->xch_blocks = blocks;
-    }
     const int k = q->next;
     q->next = (q->next + 1) % KSIM_SCHED_BUFS;
     // the set's previous launch (possibly on another stream) must be over
@@ -1311,7 +1259,7 @@ int ksim_sched_launch(ksim_sched *q, const int32_t *items, int32_t n, int32_t bl
     hip_check(hipGetLastError(), "k_reset_cursor");
 #define KSIM_SCHED_LAUNCH(W)                                                                   \
   hipLaunchKernelGGL(k_batch<W>, dim3(blocks), dim3(kBlock), 0, st, q->d_progs[k],              \
-                     q->d_items[k], (uint32_t)n, cursor, q->d_cursor + 64, t.view(), q->d_xch[k])
+                     q->d_items[k], (uint32_t)n, cursor, q->d_cursor + 64, t.view())
     switch (lane_words(wl)) {
       case 1: KSIM_SCHED_LAUNCH(1); break;
       case 2: KSIM_SCHED_LAUNCH(2); break;
@@ -1349,7 +1297,6 @@ void ksim_sched_destroy(ksim_sched *q) {
     if (q->h_items[k]) (void)hipHostFree(q->h_items[k]);
     if (q->d_progs[k]) (void)hipFree(q->d_progs[k]);
     if (q->d_items[k]) (void)hipFree(q->d_items[k]);
-    if (q->d_xch[k]) (void)hipFree(q->d_xch[k]);
   }
   if (q->d_cursor) (void)hipFree(q->d_cursor);
   delete q;

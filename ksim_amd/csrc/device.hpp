@@ -140,7 +140,6 @@ __device__ __forceinline__ Acc4 add_residual_term(Acc4 acc, int64_t x0, int64_t 
 
 // LDS-resident lookup tables of one block
 struct LdsTables {
-  int4 tab[TAB_LDS_N];          // w^s (1+w)^r (1+sqrt2)^n, |n| <= TAB_LDS_R
   int4 pi[PI_MAX * PI_MAX];     // pi^a pibar^b
   double mag[MAG_LDS_N];        // magnitude() of the entries with |n| <= MAG_LDS_R
 };
@@ -235,10 +234,10 @@ __device__ __forceinline__ TermIdx term_index(int S, int V, int N) {
   return t;
 }
 
+// the first digit plane is read from global memory: its central rows stay
+// resident in the CU's vector L1 cache (an LDS copy measured no faster)
 __device__ __forceinline__ int4 table_digit0(const TermIdx &ti, bool vanished, const LdsTables *lt,
                                              const Tables &tb, uint32_t &err) {
-  if ((unsigned)(ti.n2 + TAB_LDS_R) <= 2u * TAB_LDS_R)
-    return lt->tab[((ti.n2 + TAB_LDS_R) * 4 + ti.r) * 8 + ti.s2];
   if (ti.in_tab) return tb.tab[ti.gi];
   if (!vanished) err |= ERR_TAB_RANGE;
   return make_int4(0, 0, 0, 0);
@@ -412,8 +411,8 @@ __device__ __forceinline__ Acc4 widen(const Acc64 &n) {
 // Waves that share the term sums of one 64-shot tile: L waves (1, 2 or 4)
 // of a block form a group; wave `part` of the group sums term parts
 // [part * GH_PARTS / L, (part + 1) * GH_PARTS / L) of each graph and the
-// partial sums meet in `xch`, the block's exchange area in global memory:
-// [2 buffers][4 waves][4 int4][64 lanes].  Every wave of the group then
+// partial sums meet in `xch`, the block's exchange area in LDS:
+// [4 waves][4 int4][64 lanes] (16 KiB).  Every wave of the group then
 // holds the same exact sum, so all of them take the same decisions.
 struct Split {
   int L;          // waves per group
@@ -421,17 +420,16 @@ struct Split {
   int first;      // the group's first wave in the block
   int4 *xch;      // the block's exchange area (nullptr when L == 1)
 };
-constexpr int XCH_INT4_PER_BLOCK = 2 * 4 * 4 * 64;
+constexpr int XCH_INT4 = 4 * 4 * 64;
 
 // Sum the partial sums of the group's waves (exact integer sums, so every
 // wave ends with the same value).  Uniform across the block: every wave of
-// the block calls it for the same graph, so the barrier is reached by all.
-__device__ __forceinline__ void exchange_sums(GraphSum &g, bool narrow, const Split &sp,
-                                              uint32_t &xn) {
+// the block calls it for the same graph, so both barriers are reached by
+// all (the second keeps the area until every wave has read it).
+__device__ __forceinline__ void exchange_sums(GraphSum &g, bool narrow, const Split &sp) {
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  int4 *buf = sp.xch + (size_t)(xn & 1u) * (4 * 4 * 64);
-  ++xn;
+  int4 *buf = sp.xch;
   int4 *mine = buf + wave * 256 + lane;
   if (narrow) {
     mine[0] = make_int4((int)(uint32_t)g.n.a, (int)(g.n.a >> 32), (int)(uint32_t)g.n.b, (int)(g.n.b >> 32));
@@ -467,6 +465,7 @@ __device__ __forceinline__ void exchange_sums(GraphSum &g, bool narrow, const Sp
       g.acc.a += v[0]; g.acc.b += v[1]; g.acc.c += v[2]; g.acc.d += v[3];
     }
   }
+  __syncthreads();
 }
 
 // |amplitude| of one graph (float64) for this lane's parameter words: the
@@ -477,8 +476,7 @@ template <int W32, int WL, bool SPLIT>
 __device__ __forceinline__ double graph_prob(const cword *__restrict__ P,
                                              const cword *__restrict__ gh,
                                              const uint32_t (&pv)[WL], const LdsTables *lt,
-                                             const Tables &tb, uint32_t &err, const Split &sp,
-                                             uint32_t &xn) {
+                                             const Tables &tb, uint32_t &err, const Split &sp) {
   const uint32_t gflags = gh[GH_FLAGS];
   const int e0 = (int)gh[GH_E0];
   const uint32_t nt = gh[GH_NTERMS];
@@ -497,7 +495,7 @@ __device__ __forceinline__ double graph_prob(const cword *__restrict__ P,
   graph_sum_clear(g);
   sum_terms<W32, WL>(P + off, t1 - t0, gflags, e0, pv, lt, tb, g, err);
   if (gflags & GF_MONO) return g.pm;
-  if (share) exchange_sums(g, (gflags & GF_NARROW) != 0, sp, xn);
+  if (share) exchange_sums(g, (gflags & GF_NARROW) != 0, sp);
   if (gflags & GF_NARROW) return magnitude(widen(g.n), e0);
   return magnitude(g.acc, e0);
 }
@@ -506,12 +504,11 @@ template <int WL, bool SPLIT = false>
 __device__ __forceinline__ double graph_prob_any(const cword *__restrict__ P,
                                                  const cword *__restrict__ gh,
                                                  const uint32_t (&pv)[WL], const LdsTables *lt,
-                                                 const Tables &tb, uint32_t &err, const Split &sp,
-                                                 uint32_t &xn) {
+                                                 const Tables &tb, uint32_t &err, const Split &sp) {
   switch (gh[GH_W32]) {
 #define KSIM_GP(w)                                                                       \
   case w:                                                                                \
-    if constexpr (WL >= w) return graph_prob<w, WL, SPLIT>(P, gh, pv, lt, tb, err, sp, xn); \
+    if constexpr (WL >= w) return graph_prob<w, WL, SPLIT>(P, gh, pv, lt, tb, err, sp); \
     break;
     KSIM_GP(1) KSIM_GP(2) KSIM_GP(3) KSIM_GP(4) KSIM_GP(5) KSIM_GP(6) KSIM_GP(7) KSIM_GP(8)
 #undef KSIM_GP
@@ -722,7 +719,7 @@ template <int WL, bool NOISE = false, bool SPLIT = false>
 __device__ __forceinline__ void run_shot(const ShotArgs &a, int comp, int64_t s,
                                          const LdsTables *lt, const Tables &tb,
                                          const Split &sp = Split{1, 0, 0, nullptr},
-                                         bool valid = true, uint32_t *xn_io = nullptr) {
+                                         bool valid = true) {
   const cword *__restrict__ P = as_const(a.prog);
   const cword *__restrict__ ch = P + P[PH_DWORDS + comp];
   const int ng = (int)ch[CH_NGRAPHS], F = (int)ch[CH_F];
@@ -806,7 +803,6 @@ __device__ __forceinline__ void run_shot(const ShotArgs &a, int comp, int64_t s,
   // per-component stream: seed + 0x9E37*k (src/python/ksim/sample/sampler.py:63)
   uint64_t rs = rng_seed(a.seed + 0x9E37ull * (uint64_t)comp, (uint64_t)(a.shot_base + s));
   uint32_t err = 0;
-  uint32_t xn = xn_io ? *xn_io : 0u;
   double prev = 0.0;
   // graph 0 gives the normaliser, graph i+1 the probability of output i = 1
   // (one evaluation site, so the graph code is inlined once); the decided
@@ -814,7 +810,7 @@ __device__ __forceinline__ void run_shot(const ShotArgs &a, int comp, int64_t s,
   for (int i = 0; i < ng; ++i) {
     const cword *gh = ghs + (size_t)i * GH_DWORDS;
     if (i > 0) set_param_bit<WL>(pv, F + i - 1, 1u);  // trying bit
-    const double p1 = graph_prob_any<WL, SPLIT>(P, gh, pv, lt, tb, err, sp, xn);
+    const double p1 = graph_prob_any<WL, SPLIT>(P, gh, pv, lt, tb, err, sp);
     if (i == 0) {
       prev = p1;
       continue;
@@ -825,7 +821,6 @@ __device__ __forceinline__ void run_shot(const ShotArgs &a, int comp, int64_t s,
     set_param_bit<WL>(pv, F + i - 1, bit);
     prev = bit ? p1 : __dsub_rn(prev, p1);
   }
-  if (xn_io) *xn_io = xn;
   if (SPLIT && (!valid || sp.part != 0)) {
     if (err) atomicOr(a.err, err);
     return;
@@ -841,11 +836,8 @@ __device__ __forceinline__ void run_shot(const ShotArgs &a, int comp, int64_t s,
   if (err) atomicOr(a.err, err);
 }
 
-// stage the central rows of the materialisation and magnitude tables and
-// the pi powers
+// stage the central rows of the magnitude table and the pi powers
 __device__ inline void load_tables(LdsTables *t, const Tables &tb) {
-  const int4 *src = tb.tab + (TAB_W - TAB_LDS_R) * 32;
-  for (int i = threadIdx.x; i < TAB_LDS_N; i += blockDim.x) t->tab[i] = src[i];
   for (int i = threadIdx.x; i < PI_MAX * PI_MAX; i += blockDim.x) t->pi[i] = tb.pi[i];
   const double *msrc = tb.mag + (TAB_W - MAG_LDS_R) * 32;
   for (int i = threadIdx.x; i < MAG_LDS_N; i += blockDim.x) t->mag[i] = msrc[i];

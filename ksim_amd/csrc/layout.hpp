@@ -42,10 +42,6 @@ constexpr int TAB_DIGITS = 3;
 constexpr int TAB_DIGIT_BITS = 31;
 constexpr int TAB_N = (2 * TAB_W + 1) * 4 * 8;  // entries per digit plane
 inline int tab_index(int n, int r, int s) { return ((n + TAB_W) * 4 + r) * 8 + s; }
-// the rows |n| <= TAB_LDS_R are staged in LDS per block; the outer rows are
-// read from global memory (rare)
-constexpr int TAB_LDS_R = 17;
-constexpr int TAB_LDS_N = (2 * TAB_LDS_R + 1) * 4 * 8;
 // float64 magnitudes |w^s (1+w)^r (1+sqrt2)^n| of the table entries (by the
 // sampler's magnitude sequence), same indexing; rows |n| <= MAG_LDS_R in LDS
 constexpr int MAG_LDS_R = 12;

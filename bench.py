@@ -205,6 +205,12 @@ def plain_result(name, steps, warmup, with_cpu=True):
         n = scale_cpu_shots(probe["us_per_shot"], 12.0)
         log(f"[bench] {name}: CPU port baseline on {n} shots")
         res["cpu_baseline"] = cpu_baseline_port(w["prog"], w["F"], n)
+        # the reference's other CPU path, numpy sample_flat (one core), on
+        # about 6 s of work
+        probe = cpu_baseline_numpy(w["prog"], w["F"], 100)
+        n = scale_cpu_shots(probe["us_per_shot"], 6.0, lo=100, hi=100_000)
+        log(f"[bench] {name}: numpy sample_flat baseline on {n} shots")
+        res["cpu_baseline_numpy"] = cpu_baseline_numpy(w["prog"], w["F"], n)
     return res
 
 
@@ -471,6 +477,7 @@ def main():
                    "program": res["stats"]},
         "roofline": res["roofline"],
         "cpu_baseline": res.get("cpu_baseline"),
+        "cpu_baseline_numpy": res.get("cpu_baseline_numpy"),
     }
     if "noise" in res:
         line["noise"] = res["noise"]
@@ -479,7 +486,8 @@ def main():
         for name in ("config1", "config3"):
             r = plain_result(name, max(2, args.steps // 2), 1, with_cpu=not args.no_cpu)
             line["suite"][name] = {k: r[k] for k in ("shots_per_s", "us_per_shot", "kernel_ms",
-                                                     "roofline", "cpu_baseline") if k in r}
+                                                     "roofline", "cpu_baseline",
+                                                     "cpu_baseline_numpy") if k in r}
         line["suite"]["config4"] = batched_result(max(2, args.steps // 2), 1)
         line["suite"]["tcount_sweep"] = tcount_sweep()
         line["suite"]["config5"] = serving_result()

@@ -318,21 +318,32 @@ def batched_result(steps, warmup, n_programs=64, tune=True, shape=None):
         batch.tune = dict(wave_shots=batch.wave_shots,
                           blocks_per_cu=batch.blocks // device_info()["cus"], tune_s=0.0)
     total = batch.total_shots
+    # the plain path: one launch per program, one lane per shot (the
+    # reference's launch shape), then the same launches with the automatic
+    # term split of small launches
+    for s in samplers:
+        s.set_term_split(1)
     wall_p, dev_p = time_steps(plain_step, steps, warmup)
     for s in samplers:
-        s_check = lib.ksim_program_check(s.handle, torch.cuda.current_stream().cuda_stream)
-        _lib.check(s_check)
+        _lib.check(lib.ksim_program_check(s.handle, torch.cuda.current_stream().cuda_stream))
+        s.set_term_split(0)
+    wall_a, dev_a = time_steps(plain_step, steps, warmup)
+    for s in samplers:
+        _lib.check(lib.ksim_program_check(s.handle, torch.cuda.current_stream().cuda_stream))
     wall_b, dev_b = time_steps(lambda: batch.run(use_graph=True), steps, warmup)
     batch.check()
     # same bits either way
     same = all(torch.equal(o, bo[:, :o.shape[1]]) for o, bo in zip(outs, batch.out))
     plain_sps = total * steps / wall_p
+    auto_sps = total * steps / wall_a
     batch_sps = total * steps / wall_b
     kern_b = dev_b / steps
     achieved = batch.ops / kern_b
     return dict(programs=len(specs), shots=total, plain_shots_per_s=plain_sps,
                 batched_shots_per_s=batch_sps, speedup=batch_sps / plain_sps,
                 plain_ms=wall_p / steps * 1e3, batched_ms=wall_b / steps * 1e3,
+                plain_split_ms=wall_a / steps * 1e3, plain_split_shots_per_s=auto_sps,
+                speedup_vs_plain_split=batch_sps / auto_sps,
                 batched_equals_plain=bool(same), ops_total=batch.ops,
                 tuned=dict(wave_shots=batch.tune["wave_shots"],
                            split_share=batch.tune.get("split_share", 8),

@@ -102,6 +102,7 @@ std::string err_text(uint32_t e) {
   if (e & ERR_TAB_RANGE) s += " (1+sqrt2) exponent beyond the table";
   if (e & ERR_PI_RANGE) s += " pi exponent beyond the table";
   if (e & ERR_SHIFT) s += " alignment shift beyond the 128-bit sum";
+  if (e & ERR_WIDE) s += " a graph needing 128-bit sums in a 64-bit kernel";
   return s;
 }
 
@@ -136,22 +137,28 @@ __global__ void k_init_mag(const int4 *tab, double *mag) {
 }
 
 // Plain path: one lane per shot, grid (ceil(B/256), n_comp).  NOISE: the
-// f bits are drawn in the prologue instead of read.
-template <int WL, bool NOISE = false>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KSIM_WAVES_PER_EU))) void k_sample(ShotArgs a, Tables tb) {
+// f bits are drawn in the prologue instead of read.  WIDE = false: a
+// program whose graphs all have a single term or 64-bit sums (GF_MONO /
+// GF_NARROW) runs without the 128-bit path, in fewer registers, so more
+// waves per SIMD hide the scalar-load latency of its short graphs.
+#ifndef KSIM_NARROW_WAVES_PER_EU
+#define KSIM_NARROW_WAVES_PER_EU 8
+#endif
+template <int WL, bool NOISE = false, bool WIDE = true>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WIDE ? KSIM_WAVES_PER_EU : KSIM_NARROW_WAVES_PER_EU))) void k_sample(ShotArgs a, Tables tb) {
   __shared__ LdsTables lt;
   load_tables(&lt, tb);
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= a.B) return;
-  run_shot<WL, NOISE>(a, (int)blockIdx.y, s, &lt, tb);
+  run_shot<WL, NOISE, false, WIDE>(a, (int)blockIdx.y, s, &lt, tb);
 }
 
 // Plain path with a term split: L waves (2 or 4) share each graph's terms
 // on one 64-shot tile, for launches with too few shots to fill the GPU one
 // lane per shot.  Block b takes the tile groups b, b + gridDim.x, ... (4/L
 // tiles per group).
-template <int WL, bool NOISE = false>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KSIM_WAVES_PER_EU))) void k_sample_split(
+template <int WL, bool NOISE = false, bool WIDE = true>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WIDE ? KSIM_WAVES_PER_EU : KSIM_NARROW_WAVES_PER_EU))) void k_sample_split(
     ShotArgs a, Tables tb, int L) {
   __shared__ LdsTables lt;
   __shared__ int4 xch[XCH_INT4];
@@ -169,7 +176,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KSIM_WAV
     const int64_t off = tile * 64 + lane;
     const bool valid = off < a.B;
     const int64_t s = min(off, a.B - 1);
-    run_shot<WL, NOISE, true>(a, (int)blockIdx.y, s, &lt, tb, sp, valid);
+    run_shot<WL, NOISE, true, WIDE>(a, (int)blockIdx.y, s, &lt, tb, sp, valid);
   }
 }
 
@@ -285,8 +292,8 @@ __global__ void k_reset_cursor(uint32_t *cursor, uint32_t *clear) {
   }
 }
 
-template <int WL>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KSIM_WAVES_PER_EU))) void k_batch(
+template <int WL, bool WIDE = true>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WIDE ? KSIM_WAVES_PER_EU : KSIM_NARROW_WAVES_PER_EU))) void k_batch(
     const BatchProg *__restrict__ progs, const BatchItem *__restrict__ items, uint32_t n_items,
     uint32_t *cursor, uint32_t *err, Tables tb) {
   __shared__ LdsTables lt;
@@ -323,7 +330,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KSIM_WAV
       const int off = tile * 64 + lane;
       const bool valid = off < it.count;
       const int64_t s = it.shot0 + min(off, it.count - 1);
-      run_shot<WL, false, true>(a, it.comp, s, &lt, tb, sp, valid);
+      run_shot<WL, false, true, WIDE>(a, it.comp, s, &lt, tb, sp, valid);
     }
   }
 }
@@ -344,6 +351,7 @@ struct ksim_program {
   int out_words = 0;              // packed output row words: ceil(num_outputs / 32)
   bool packed_full_rows = false;  // one component writes whole packed rows in order
   int term_split = 0;  // waves per 64-shot tile in plain launches; 0: chosen per launch
+  bool all_narrow = false;  // every graph GF_MONO or GF_NARROW: kernels without 128-bit sums
   // staging for the host-pointer API
   uint8_t *d_f = nullptr;
   size_t f_cap = 0;
@@ -515,33 +523,46 @@ void launch_sample(ksim_program *p, const uint8_t *d_f, int64_t B, int32_t f_col
     const int64_t groups = ((B + 63) / 64 + (4 / L) - 1) / (4 / L);
     const int blocks = (int)std::min<int64_t>(groups, kSplitBlocks);
     dim3 g2((unsigned)blocks, (unsigned)p->n_comp);
-#define KSIM_SPLIT_LAUNCH(W, N) \
-  hipLaunchKernelGGL((k_sample_split<W, N>), g2, dim3(kBlock), 0, st, a, t.view(), L)
-    switch (p->wl * 2 + (nz ? 1 : 0)) {
-      case 2: KSIM_SPLIT_LAUNCH(1, false); break;
-      case 3: KSIM_SPLIT_LAUNCH(1, true); break;
-      case 4: KSIM_SPLIT_LAUNCH(2, false); break;
-      case 5: KSIM_SPLIT_LAUNCH(2, true); break;
-      case 8: KSIM_SPLIT_LAUNCH(4, false); break;
-      case 9: KSIM_SPLIT_LAUNCH(4, true); break;
-      case 16: KSIM_SPLIT_LAUNCH(8, false); break;
-      default: KSIM_SPLIT_LAUNCH(8, true); break;
+#define KSIM_SPLIT_LAUNCH(W, N, WD) \
+  hipLaunchKernelGGL((k_sample_split<W, N, WD>), g2, dim3(kBlock), 0, st, a, t.view(), L)
+#define KSIM_SPLIT_CASES(W)                                   \
+  if (p->all_narrow) {                                        \
+    if (nz) KSIM_SPLIT_LAUNCH(W, true, false);                \
+    else KSIM_SPLIT_LAUNCH(W, false, false);                  \
+  } else {                                                    \
+    if (nz) KSIM_SPLIT_LAUNCH(W, true, true);                 \
+    else KSIM_SPLIT_LAUNCH(W, false, true);                   \
+  }
+    switch (p->wl) {
+      case 1: KSIM_SPLIT_CASES(1); break;
+      case 2: KSIM_SPLIT_CASES(2); break;
+      case 4: KSIM_SPLIT_CASES(4); break;
+      default: KSIM_SPLIT_CASES(8); break;
     }
+#undef KSIM_SPLIT_CASES
 #undef KSIM_SPLIT_LAUNCH
     hip_check(hipGetLastError(), "k_sample_split launch");
     return;
   }
   dim3 grid((unsigned)nb, (unsigned)p->n_comp);
-  switch (p->wl * 2 + (nz ? 1 : 0)) {
-    case 2: hipLaunchKernelGGL((k_sample<1, false>), grid, dim3(kBlock), 0, st, a, t.view()); break;
-    case 3: hipLaunchKernelGGL((k_sample<1, true>), grid, dim3(kBlock), 0, st, a, t.view()); break;
-    case 4: hipLaunchKernelGGL((k_sample<2, false>), grid, dim3(kBlock), 0, st, a, t.view()); break;
-    case 5: hipLaunchKernelGGL((k_sample<2, true>), grid, dim3(kBlock), 0, st, a, t.view()); break;
-    case 8: hipLaunchKernelGGL((k_sample<4, false>), grid, dim3(kBlock), 0, st, a, t.view()); break;
-    case 9: hipLaunchKernelGGL((k_sample<4, true>), grid, dim3(kBlock), 0, st, a, t.view()); break;
-    case 16: hipLaunchKernelGGL((k_sample<8, false>), grid, dim3(kBlock), 0, st, a, t.view()); break;
-    default: hipLaunchKernelGGL((k_sample<8, true>), grid, dim3(kBlock), 0, st, a, t.view()); break;
+#define KSIM_SAMPLE_LAUNCH(W, N, WD) \
+  hipLaunchKernelGGL((k_sample<W, N, WD>), grid, dim3(kBlock), 0, st, a, t.view())
+#define KSIM_SAMPLE_CASES(W)                                  \
+  if (p->all_narrow) {                                        \
+    if (nz) KSIM_SAMPLE_LAUNCH(W, true, false);               \
+    else KSIM_SAMPLE_LAUNCH(W, false, false);                 \
+  } else {                                                    \
+    if (nz) KSIM_SAMPLE_LAUNCH(W, true, true);                \
+    else KSIM_SAMPLE_LAUNCH(W, false, true);                  \
   }
+  switch (p->wl) {
+    case 1: KSIM_SAMPLE_CASES(1); break;
+    case 2: KSIM_SAMPLE_CASES(2); break;
+    case 4: KSIM_SAMPLE_CASES(4); break;
+    default: KSIM_SAMPLE_CASES(8); break;
+  }
+#undef KSIM_SAMPLE_CASES
+#undef KSIM_SAMPLE_LAUNCH
   hip_check(hipGetLastError(), "k_sample launch");
 }
 
@@ -580,6 +601,10 @@ ksim_program *upload_program(const std::vector<uint32_t> &words, const PackStats
                                &words[ch[CH_GRAPH_OFF]] + (size_t)ch[CH_NGRAPHS] * GH_DWORDS);
       p->graph_hdr.push_back(std::move(gh));
     }
+    p->all_narrow = true;
+    for (const auto &gh : p->graph_hdr)
+      for (size_t i = 0; i + GH_DWORDS <= gh.size(); i += GH_DWORDS)
+        if (!(gh[i + GH_FLAGS] & (GF_MONO | GF_NARROW))) p->all_narrow = false;
     tables_for_current_device();
   } catch (...) {
     ksim_program_destroy(p);
@@ -981,6 +1006,7 @@ struct ksim_batch {
   uint32_t *d_cursor = nullptr;  // [0] cursor, [1] err
   int wave_shots = 64;
   int wl = 1;  // lane parameter words: the widest program of the batch
+  bool all_narrow = true;  // every program without 128-bit graphs
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int graph_blocks = 0;
@@ -1034,6 +1060,7 @@ int ksim_batch_create(ksim_program *const *progs, int32_t n, const uint8_t *cons
       ksim_program *p = progs[k];
       b->progs.push_back(p);
       b->wl = std::max(b->wl, p->wl);
+      b->all_narrow = b->all_narrow && p->all_narrow;
       hp[k] = BatchProg{p->d_prog, d_f[k], d_out[k], seed[k], B[k], f_cols[k], p->num_outputs,
                         p->tot_out, p->wl};
       for (int c = 0; c < p->n_comp; ++c) {
@@ -1076,15 +1103,19 @@ static void batch_enqueue(ksim_batch *b, int blocks, hipStream_t st) {
   DevTables t = tables_for_current_device();
   hipLaunchKernelGGL(k_reset_cursor, dim3(1), dim3(64), 0, st, b->d_cursor, b->d_cursor + 1);
   hip_check(hipGetLastError(), "k_reset_cursor");
-#define KSIM_BATCH_LAUNCH(W)                                                                  \
-  hipLaunchKernelGGL(k_batch<W>, dim3(blocks), dim3(kBlock), 0, st, b->d_progs, b->d_items,    \
+#define KSIM_BATCH_LAUNCH(W, WD)                                                               \
+  hipLaunchKernelGGL((k_batch<W, WD>), dim3(blocks), dim3(kBlock), 0, st, b->d_progs, b->d_items, \
                      b->n_items, b->d_cursor, b->d_cursor + 1, t.view())
+#define KSIM_BATCH_CASES(W)                               \
+  if (b->all_narrow) KSIM_BATCH_LAUNCH(W, false);         \
+  else KSIM_BATCH_LAUNCH(W, true);
   switch (b->wl) {
-    case 1: KSIM_BATCH_LAUNCH(1); break;
-    case 2: KSIM_BATCH_LAUNCH(2); break;
-    case 4: KSIM_BATCH_LAUNCH(4); break;
-    default: KSIM_BATCH_LAUNCH(8); break;
+    case 1: KSIM_BATCH_CASES(1); break;
+    case 2: KSIM_BATCH_CASES(2); break;
+    case 4: KSIM_BATCH_CASES(4); break;
+    default: KSIM_BATCH_CASES(8); break;
   }
+#undef KSIM_BATCH_CASES
 #undef KSIM_BATCH_LAUNCH
   hip_check(hipGetLastError(), "k_batch");
 }
@@ -1150,6 +1181,7 @@ struct ksim_sched {
   int max_slots = 0, max_items = 0;
   std::vector<BatchProg> slots;
   std::vector<int> slot_wl, slot_ncomp;
+  std::vector<char> slot_narrow;
   BatchProg *h_progs[KSIM_SCHED_BUFS] = {};
   BatchItem *h_items[KSIM_SCHED_BUFS] = {};
   BatchProg *d_progs[KSIM_SCHED_BUFS] = {};
@@ -1172,6 +1204,7 @@ int ksim_sched_create(int32_t max_slots, int32_t max_items, ksim_sched **out) {
     q->slots.assign(max_slots, BatchProg{nullptr, nullptr, nullptr, 0, 0, 0, 0, 0, 1});
     q->slot_wl.assign(max_slots, 0);
     q->slot_ncomp.assign(max_slots, 0);
+    q->slot_narrow.assign(max_slots, 1);
     try {
       for (int k = 0; k < KSIM_SCHED_BUFS; ++k) {
         hip_check(hipHostMalloc((void **)&q->h_progs[k], sizeof(BatchProg) * max_slots), "hipHostMalloc");
@@ -1203,6 +1236,7 @@ int ksim_sched_set_slot(ksim_sched *q, int32_t slot, ksim_program *p, const uint
                                p->tot_out, p->wl};
     q->slot_wl[slot] = p->wl;
     q->slot_ncomp[slot] = p->n_comp;
+    q->slot_narrow[slot] = p->all_narrow ? 1 : 0;
   });
 }
 
@@ -1232,6 +1266,7 @@ int ksim_sched_launch(ksim_sched *q, const int32_t *items, int32_t n, int32_t bl
     // the set's previous launch (possibly on another stream) must be over
     if (q->done_valid[k]) hip_check(hipEventSynchronize(q->done[k]), "hipEventSynchronize");
     int wl = 1;
+    bool narrow = true;
     for (int i = 0; i < n; ++i) {
       const int32_t slot = items[5 * i], comp = items[5 * i + 1];
       const int32_t shot0 = items[5 * i + 2], count = items[5 * i + 3];
@@ -1246,6 +1281,7 @@ int ksim_sched_launch(ksim_sched *q, const int32_t *items, int32_t n, int32_t bl
       if (shot0 < 0 || count <= 0 || (int64_t)shot0 + count > bp.B)
         throw std::invalid_argument("item shot range outside its request");
       wl = std::max(wl, q->slot_wl[slot]);
+      narrow = narrow && q->slot_narrow[slot];
       q->h_items[k][i] = BatchItem{slot, comp, shot0, count, split};
     }
     std::memcpy(q->h_progs[k], q->slots.data(), sizeof(BatchProg) * q->max_slots);
@@ -1257,15 +1293,19 @@ int ksim_sched_launch(ksim_sched *q, const int32_t *items, int32_t n, int32_t bl
     uint32_t *cursor = q->d_cursor + 2 * k;
     hipLaunchKernelGGL(k_reset_cursor, dim3(1), dim3(64), 0, st, cursor, cursor + 1);
     hip_check(hipGetLastError(), "k_reset_cursor");
-#define KSIM_SCHED_LAUNCH(W)                                                                   \
-  hipLaunchKernelGGL(k_batch<W>, dim3(blocks), dim3(kBlock), 0, st, q->d_progs[k],              \
+#define KSIM_SCHED_LAUNCH(W, WD)                                                               \
+  hipLaunchKernelGGL((k_batch<W, WD>), dim3(blocks), dim3(kBlock), 0, st, q->d_progs[k],        \
                      q->d_items[k], (uint32_t)n, cursor, q->d_cursor + 64, t.view())
+#define KSIM_SCHED_CASES(W)                        \
+  if (narrow) KSIM_SCHED_LAUNCH(W, false);         \
+  else KSIM_SCHED_LAUNCH(W, true);
     switch (lane_words(wl)) {
-      case 1: KSIM_SCHED_LAUNCH(1); break;
-      case 2: KSIM_SCHED_LAUNCH(2); break;
-      case 4: KSIM_SCHED_LAUNCH(4); break;
-      default: KSIM_SCHED_LAUNCH(8); break;
+      case 1: KSIM_SCHED_CASES(1); break;
+      case 2: KSIM_SCHED_CASES(2); break;
+      case 4: KSIM_SCHED_CASES(4); break;
+      default: KSIM_SCHED_CASES(8); break;
     }
+#undef KSIM_SCHED_CASES
 #undef KSIM_SCHED_LAUNCH
     hip_check(hipGetLastError(), "k_batch");
     hip_check(hipEventRecord(q->done[k], st), "hipEventRecord");

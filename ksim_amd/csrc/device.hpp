@@ -152,13 +152,15 @@ struct Tables {
 };
 
 // XOR over the lane's parameter words of (word & mask word): its popcount
-// has the parity of <params, mask> (one AND and W32-1 AND-XORs, one popcount)
+// has the parity of <params, mask>.  One AND, then one v_bitop3 per further
+// word computing (mask & word) ^ t (truth table 0x6c for operands
+// (mask, t, word)), so a W32-word mask costs W32 VALU ops.
 template <int W32, int WL>
 __device__ __forceinline__ uint32_t masked_xor(const uint32_t (&pv)[WL],
                                                const cword *__restrict__ m) {
   uint32_t t = pv[0] & m[0];
 #pragma unroll
-  for (int w = 1; w < W32; ++w) t ^= pv[w] & m[w];
+  for (int w = 1; w < W32; ++w) t = __builtin_amdgcn_bitop3_b32(m[w], t, pv[w], 0x6c);
   return t;
 }
 
@@ -343,16 +345,23 @@ struct GraphSum {
   double pm;    // the float64 magnitude (GF_MONO)
 };
 
-__device__ __forceinline__ void graph_sum_clear(GraphSum &g) {
-  g.acc.a = g.acc.b = g.acc.c = g.acc.d = 0;
-  g.n.a = g.n.b = g.n.c = g.n.d = 0;
-  g.pm = 0.0;
+// clear the sums the graph's flags select (the others stay unset: the
+// evaluation never reads them)
+__device__ __forceinline__ void graph_sum_clear(GraphSum &g, uint32_t gflags) {
+  if (gflags & GF_MONO) {
+    g.pm = 0.0;
+  } else if (gflags & GF_NARROW) {
+    g.n.a = g.n.b = g.n.c = g.n.d = 0;
+  } else {
+    g.n.a = g.n.b = g.n.c = g.n.d = 0;  // kernels without the 128-bit path
+    g.acc.a = g.acc.b = g.acc.c = g.acc.d = 0;
+  }
 }
 
 // Add terms [0, nt) starting at program dword `th` into g, by the graph's
 // flags: a single table magnitude (GF_MONO), 64-bit sums (GF_NARROW) or
 // 128-bit sums.
-template <int W32, int WL>
+template <int W32, int WL, bool WIDE = true>
 __device__ __forceinline__ void sum_terms(const cword *__restrict__ th, uint32_t nt,
                                           uint32_t gflags, int e0, const uint32_t (&pv)[WL],
                                           const LdsTables *lt, const Tables &tb, GraphSum &g,
@@ -382,9 +391,10 @@ __device__ __forceinline__ void sum_terms(const cword *__restrict__ th, uint32_t
       g.pm = vanished ? 0.0 : __builtin_ldexp(m, sh + e0);
     } else {
       const int4 e = table_digit0(ti, vanished, lt, tb, err);
-      if (gflags & GF_NARROW) {
+      if (!WIDE || (gflags & GF_NARROW)) {
+        if (!WIDE && !(gflags & GF_NARROW)) err |= ERR_WIDE;
         add_term_narrow(g.n, e, vanished, flags, A, Bq, lt, th, sh, err);
-      } else {
+      } else if constexpr (WIDE) {
         g.acc = add_term_digit<true>(g.acc, e, vanished, flags, A, Bq, lt, th, sh, err);
         const bool wide = ti.in_tab && !vanished && (unsigned)(ti.n2 + TAB_R) > 2u * TAB_R;
         if (__builtin_amdgcn_ballot_w64(wide)) {
@@ -472,7 +482,7 @@ __device__ __forceinline__ void exchange_sums(GraphSum &g, bool narrow, const Sp
 // value magnitude() gives on the exact term sum.  SPLIT: the graph's terms
 // are shared by the waves of sp's group (single-term graphs are evaluated
 // by every wave; they need no exchange).
-template <int W32, int WL, bool SPLIT>
+template <int W32, int WL, bool SPLIT, bool WIDE = true>
 __device__ __forceinline__ double graph_prob(const cword *__restrict__ P,
                                              const cword *__restrict__ gh,
                                              const uint32_t (&pv)[WL], const LdsTables *lt,
@@ -492,15 +502,16 @@ __device__ __forceinline__ double graph_prob(const cword *__restrict__ P,
     if (p1 < GH_PARTS) t1 = gh[GH_PART_T1 + p1 - 1];
   }
   GraphSum g;
-  graph_sum_clear(g);
-  sum_terms<W32, WL>(P + off, t1 - t0, gflags, e0, pv, lt, tb, g, err);
+  graph_sum_clear(g, gflags);
+  sum_terms<W32, WL, WIDE>(P + off, t1 - t0, gflags, e0, pv, lt, tb, g, err);
   if (gflags & GF_MONO) return g.pm;
-  if (share) exchange_sums(g, (gflags & GF_NARROW) != 0, sp);
-  if (gflags & GF_NARROW) return magnitude(widen(g.n), e0);
+  const bool narrow = !WIDE || (gflags & GF_NARROW);
+  if (share) exchange_sums(g, narrow, sp);
+  if (narrow) return magnitude(widen(g.n), e0);
   return magnitude(g.acc, e0);
 }
 
-template <int WL, bool SPLIT = false>
+template <int WL, bool SPLIT = false, bool WIDE = true>
 __device__ __forceinline__ double graph_prob_any(const cword *__restrict__ P,
                                                  const cword *__restrict__ gh,
                                                  const uint32_t (&pv)[WL], const LdsTables *lt,
@@ -508,7 +519,7 @@ __device__ __forceinline__ double graph_prob_any(const cword *__restrict__ P,
   switch (gh[GH_W32]) {
 #define KSIM_GP(w)                                                                       \
   case w:                                                                                \
-    if constexpr (WL >= w) return graph_prob<w, WL, SPLIT>(P, gh, pv, lt, tb, err, sp); \
+    if constexpr (WL >= w) return graph_prob<w, WL, SPLIT, WIDE>(P, gh, pv, lt, tb, err, sp); \
     break;
     KSIM_GP(1) KSIM_GP(2) KSIM_GP(3) KSIM_GP(4) KSIM_GP(5) KSIM_GP(6) KSIM_GP(7) KSIM_GP(8)
 #undef KSIM_GP
@@ -526,7 +537,7 @@ __device__ __forceinline__ void eval_graph(const cword *__restrict__ P,
                                            const uint32_t (&pv)[WL], const LdsTables *lt,
                                            const Tables &tb, Acc4 &acc, uint32_t &err) {
   GraphSum g;
-  graph_sum_clear(g);
+  graph_sum_clear(g, 0u);
   sum_terms<W32, WL>(P + gh[GH_TERM_OFF], gh[GH_NTERMS], 0u, (int)gh[GH_E0], pv, lt, tb, g, err);
   acc = g.acc;
 }
@@ -715,7 +726,7 @@ __device__ __forceinline__ void write_packed_row(const ShotArgs &a, const uint32
 // the loop (lanes without a shot of their own run a copy of a valid shot,
 // so the group's barriers are reached by whole waves) and only `valid`
 // lanes of the group's first wave write.
-template <int WL, bool NOISE = false, bool SPLIT = false>
+template <int WL, bool NOISE = false, bool SPLIT = false, bool WIDE = true>
 __device__ __forceinline__ void run_shot(const ShotArgs &a, int comp, int64_t s,
                                          const LdsTables *lt, const Tables &tb,
                                          const Split &sp = Split{1, 0, 0, nullptr},
@@ -810,7 +821,7 @@ __device__ __forceinline__ void run_shot(const ShotArgs &a, int comp, int64_t s,
   for (int i = 0; i < ng; ++i) {
     const cword *gh = ghs + (size_t)i * GH_DWORDS;
     if (i > 0) set_param_bit<WL>(pv, F + i - 1, 1u);  // trying bit
-    const double p1 = graph_prob_any<WL, SPLIT>(P, gh, pv, lt, tb, err, sp);
+    const double p1 = graph_prob_any<WL, SPLIT, WIDE>(P, gh, pv, lt, tb, err, sp);
     if (i == 0) {
       prev = p1;
       continue;

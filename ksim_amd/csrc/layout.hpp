@@ -104,6 +104,7 @@ enum ProgHdr : int {
 constexpr uint32_t ERR_TAB_RANGE = 1u;   // |n'| beyond the table
 constexpr uint32_t ERR_PI_RANGE = 2u;    // pi exponents beyond PI_MAX
 constexpr uint32_t ERR_SHIFT = 4u;       // alignment shift outside [0, max]
+constexpr uint32_t ERR_WIDE = 8u;        // a 128-bit graph in a kernel built without that path
 
 // Largest alignment shift the 128-bit sum accepts; the packer proves
 // bits(term) + shift + log2(G) <= 126 before upload.

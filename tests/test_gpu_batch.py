@@ -102,3 +102,30 @@ def test_term_split_batch_equals_alone(tile_ops):
             torch.cuda.synchronize()
             b.check()
             check_same(b, samplers, alone)
+
+
+@pytest.mark.parametrize("tile_ops", [1.0, 1e300])
+def test_batch_with_128bit_graphs_equals_alone(tile_ops):
+    # a program whose graphs need 128-bit sums in the batch selects the
+    # kernel with that path for the whole batch; results stay the same
+    import torch
+    progs = [synth.wide_exponent_program(seed=5), synth.wide_exponent_program(n_out=2, seed=7),
+             synth.shape_program(n_q=8, t=10, F=6, n_out=7, seed=12)]
+    samplers, fs, seeds, alone = [], [], [], []
+    for k, prog in enumerate(progs):
+        s = HipProgramSampler(prog)
+        B = (300, 129, 200)[k]
+        f = synth.random_f(B, s.min_num_f, 0.3, 60 + k)
+        samplers.append(s)
+        fs.append(torch.from_numpy(f).cuda())
+        seeds.append(3000 + k)
+        alone.append(s.sample(f, 3000 + k))
+    assert samplers[0].stats()["wide_terms"] > 0
+    b = BatchedSampler(samplers, fs, seeds, wave_shots=64, tile_ops=tile_ops)
+    for use_graph in (False, True):
+        for o in b.out:
+            o.fill_(7)
+        b.run(use_graph=use_graph)
+        torch.cuda.synchronize()
+        b.check()
+        check_same(b, samplers, alone)

@@ -67,7 +67,29 @@ def test_wide_table_rows():
     B = 256
     f = synth.random_f(B, 6, 0.5, 9)
     U = np.random.default_rng(9).random((B, prog.num_outputs))
-    assert np.array_equal(s.sample_uniforms(f, U), exact.sample_program(prog, f, U))
+    want = exact.sample_program(prog, f, U)
+    # 128-bit sums through the plain kernel and the split kernel
+    for L in (1, 2, 4, 0):
+        s.set_term_split(L)
+        assert np.array_equal(s.sample_uniforms(f, U), want), L
+
+
+def test_mixed_widths_program():
+    # one launch holding a component with 128-bit graphs and components
+    # whose graphs are single-term or 64-bit: every graph takes its own path
+    log("mixed widths")
+    prog = synth.product_program([synth.wide_exponent_program(n_out=2, seed=3),
+                                  synth.shape_program(n_q=6, t=6, F=4, n_out=3, seed=8),
+                                  synth.htkh_program(3)])
+    s = HipProgramSampler(prog)
+    assert s.stats()["wide_terms"] > 0
+    B = 200
+    f = synth.random_f(B, s.min_num_f, 0.5, 4)
+    U = np.random.default_rng(4).random((B, s.tot_out))
+    want = exact.sample_program(prog, f, U)
+    for L in (1, 4):
+        s.set_term_split(L)
+        assert np.array_equal(s.sample_uniforms(f, U), want), L
 
 
 def test_eval_abs_matches_oracle_and_reference(golden_files):

@@ -561,7 +561,9 @@ __device__ __forceinline__ void eval_graph_any(const cword *__restrict__ P,
 }
 
 // set parameter bit `pos` of the lane's words to `bit` (pos is wave-uniform:
-// a scalar branch picks the one word, so pv stays in registers)
+// a scalar branch picks the one word, so pv stays in registers; measured
+// faster than rewriting every word with a uniform word mask, which costs
+// one VALU op per word)
 template <int WL>
 __device__ __forceinline__ void set_param_bit(uint32_t (&pv)[WL], int pos, uint32_t bit) {
   const int word = __builtin_amdgcn_readfirstlane(pos >> 5);

@@ -174,3 +174,84 @@ def test_kokkos_stream_properties():
     assert np.array_equal(U, np.stack(cols, axis=1))
     assert abs(U.mean() - 0.5) < 0.02
     assert np.all((U >= 0) & (U < 1))
+
+
+def _magnitude_hp(c, e):
+    """|a + b w + c i + d conj(w)| * 2^e to 80 significant digits (decimal):
+    re = a + (b + d)/sqrt2, im = c + (b - d)/sqrt2 (the basis (1, w, i, conj w))."""
+    from decimal import Decimal, localcontext
+    with localcontext() as ctx:
+        ctx.prec = 80
+        r2 = Decimal(2).sqrt()
+        a, b, cc, d = (Decimal(int(x)) for x in c)
+        re = a + (b + d) / r2
+        im = cc + (b - d) / r2
+        return (re * re + im * im).sqrt() * (Decimal(2) ** int(e))
+
+
+def _graph_values(prog, rng, max_rows=64):
+    """(coeffs, exp) of every graph of prog at up to max_rows parameter rows
+    (all rows when there are at most max_rows)."""
+    out = []
+    for comp in prog.components:
+        F = len(comp.f_selection)
+        for i, g in enumerate(comp.graphs):
+            d = graph_to_dict(g)
+            P = F + i
+            if 2 ** P <= max_rows:
+                rows = [np.array([(r >> j) & 1 for j in range(P)]) for r in range(2 ** P)]
+            else:
+                rows = [rng.integers(0, 2, size=P) for _ in range(max_rows)]
+            out += [exact.graph_value(d, pv) for pv in rows]
+    return out
+
+
+def test_magnitude_deviation_from_the_exact_value(golden_files):
+    # The float64 magnitude of the exact sum against the exact value (80
+    # digits).  (1) Programs of real circuits (GHZ-5, H T^3 H, its noisy
+    # variant, a product of both): every amplitude <= 1 and the deviation
+    # stays below 1.1e-16 absolute -- the figure the reference publishes for
+    # its float64 evaluation (docs/benchmarks.md:354,379).  (2) Every golden
+    # and synthetic program: the fixed sequence (two conversions, a product
+    # by sqrt(1/2) and a sum per part, then squares, a sum and sqrt) is
+    # within 4 ulp of the scale of its inputs, |a| + |s| / sqrt2 and
+    # |c| + |t| / sqrt2 -- tight where they do not cancel, a condition-number
+    # bound where they do (the synthetic shape programs' terms cancel by
+    # many digits; the reference's term-by-term float64 sum loses those too).
+    from decimal import Decimal
+    rng = np.random.default_rng(12)
+    physical = ("golden_ghz5", "golden_htkh3", "golden_htkh_noisy", "golden_split_product")
+    progs = []
+    for fn in golden_files:
+        prog = load_program(np.load(fn))
+        progs.append(prog)
+        if any(p in fn for p in physical):
+            worst = 0.0
+            for c, e in _graph_values(prog, rng):
+                h = _magnitude_hp(c, e)
+                assert h <= 1 + Decimal("1e-15")
+                worst = max(worst, float(abs(Decimal(exact.magnitude(c, e)) - h)))
+            assert worst <= 1.1e-16, (fn, worst)
+    progs += [synth.shape_program(n_q=8, t=8, F=5, n_out=4, seed=2), synth.config1_program()]
+    ulp4 = Decimal(4) * Decimal(2) ** -52
+    for prog in progs:
+        for c, e in _graph_values(prog, rng, 16):
+            a, b, cc, d = c
+            s, t = b + d, b - d
+            scale = (abs(Decimal(a)) + abs(Decimal(s)) / Decimal(2).sqrt()
+                     + abs(Decimal(cc)) + abs(Decimal(t)) / Decimal(2).sqrt()) * Decimal(2) ** e
+            dev = abs(Decimal(exact.magnitude(c, e)) - _magnitude_hp(c, e))
+            assert dev <= ulp4 * scale + Decimal(2) ** -1074, (c, e, dev, scale)
+
+
+def test_magnitude_bound_catches_a_planted_error():
+    # the deviation test above fails for a plausible mistake in the
+    # sequence: b - d where b + d belongs (the real part's sqrt(1/2) share)
+    c, e = (3, 5, 0, 1), 0
+    good = exact.magnitude(c, e)
+    a, b, cc, d = c
+    wrong = math.sqrt((a + (b - d) * exact.SQ) ** 2 + (cc + (b + d) * exact.SQ) ** 2)
+    from decimal import Decimal
+    h = _magnitude_hp(c, e)
+    assert abs(Decimal(good) - h) < Decimal("1e-14")
+    assert abs(Decimal(wrong) - h) > Decimal("1e-3")

@@ -208,10 +208,18 @@ __device__ __forceinline__ void term_exps(const cword *__restrict__ th,
     uint32_t x = rp[2];
     rp += CHUNK_HDR;
     // LIN records, unrolled so each scalar load fetches several records
+    if constexpr (LONG) {
+#pragma unroll 8
+      for (uint32_t k = 0; k < nl; ++k, rp += W32 + 1) {
+        const uint32_t c = __popc(masked_xor<W32>(pv, rp));
+        x += (0u - (c & 1u)) & rp[W32];
+      }
+    } else {
 #pragma unroll 4
-    for (uint32_t k = 0; k < nl; ++k, rp += W32 + 1) {
-      const uint32_t c = __popc(masked_xor<W32>(pv, rp));
-      x += (0u - (c & 1u)) & rp[W32];
+      for (uint32_t k = 0; k < nl; ++k, rp += W32 + 1) {
+        const uint32_t c = __popc(masked_xor<W32>(pv, rp));
+        x += (0u - (c & 1u)) & rp[W32];
+      }
     }
     if constexpr (LONG) {
 #pragma unroll 2

@@ -169,6 +169,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WIDE ? K
   const Split sp{L, wave % L, wave - wave % L, xch};
   const int64_t ntiles = (a.B + 63) >> 6;
   const int64_t ngroups = (ntiles + G - 1) / G;
+  uint32_t xs = 0;  // exchange state, the same in every wave of the block
   for (int64_t gi = blockIdx.x; gi < ngroups; gi += gridDim.x) {
     // every wave of the block runs this iteration (the loop bound is the
     // block's), so the group's exchanges see all of its waves
@@ -176,7 +177,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WIDE ? K
     const int64_t off = tile * 64 + lane;
     const bool valid = off < a.B;
     const int64_t s = min(off, a.B - 1);
-    run_shot<WL, NOISE, true, WIDE>(a, (int)blockIdx.y, s, &lt, tb, sp, valid);
+    run_shot<WL, NOISE, true, WIDE>(a, (int)blockIdx.y, s, &lt, tb, sp, valid, &xs);
   }
 }
 
@@ -256,8 +257,9 @@ __global__ __launch_bounds__(kBlock) void k_eval(EvalArgs a, Tables tb) {
     a.exp2[s] = e0;
   } else {
     // the sampler's own probability path (table magnitudes, 64-bit sums)
+    uint32_t xs = 0;
     a.absout[s] = graph_prob_any<WL>(as_const(a.prog), as_const(a.gh), pv, &lt, tb, err,
-                                     Split{1, 0, 0, nullptr});
+                                     Split{1, 0, 0, nullptr}, xs);
   }
   if (err) atomicOr(a.err, err);
 }
@@ -300,6 +302,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WIDE ? K
   __shared__ uint32_t s_item;
   __shared__ int4 xch[XCH_INT4];
   load_tables(&lt, tb);
+  uint32_t xs = 0;  // exchange state, the same in every wave of the block
   // the wave index is the same on every lane of a wave: readfirstlane lets
   // the compiler keep what derives from it (term ranges, program offsets)
   // on the scalar path
@@ -330,7 +333,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WIDE ? K
       const int off = tile * 64 + lane;
       const bool valid = off < it.count;
       const int64_t s = it.shot0 + min(off, it.count - 1);
-      run_shot<WL, false, true, WIDE>(a, it.comp, s, &lt, tb, sp, valid);
+      run_shot<WL, false, true, WIDE>(a, it.comp, s, &lt, tb, sp, valid, &xs);
     }
   }
 }

@@ -464,46 +464,59 @@ constexpr int XCH_INT4 = 4 * 4 * 64;
 
 // Sum the partial sums of the group's waves (exact integer sums, so every
 // wave ends with the same value).  Uniform across the block: every wave of
-// the block calls it for the same graph, so both barriers are reached by
-// all (the second keeps the area until every wave has read it).
-__device__ __forceinline__ void exchange_sums(GraphSum &g, bool narrow, const Split &sp) {
+// the block calls it for the same graph, so every barrier is reached by
+// all.  64-bit sums use half of a wave's area at a time, alternating
+// (xs bit 0), with one barrier: a half is written again only two
+// exchanges later, after a barrier that every wave reached when done
+// reading it.  128-bit sums use the whole area between two barriers (one
+// more first if the previous exchange left a half being read, xs bit 1).
+__device__ __forceinline__ void exchange_sums(GraphSum &g, bool narrow, const Split &sp,
+                                              uint32_t &xs) {
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   int4 *buf = sp.xch;
-  int4 *mine = buf + wave * 256 + lane;
   if (narrow) {
+    const int half = (int)(xs & 1u) * 128;
+    xs = (xs ^ 1u) | 2u;
+    int4 *mine = buf + wave * 256 + half + lane;
     mine[0] = make_int4((int)(uint32_t)g.n.a, (int)(g.n.a >> 32), (int)(uint32_t)g.n.b, (int)(g.n.b >> 32));
     mine[64] = make_int4((int)(uint32_t)g.n.c, (int)(g.n.c >> 32), (int)(uint32_t)g.n.d, (int)(g.n.d >> 32));
-  } else {
-    const i128_t v[4] = {g.acc.a, g.acc.b, g.acc.c, g.acc.d};
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const u128_t u = (u128_t)v[k];
-      mine[64 * k] = make_int4((int)(uint32_t)u, (int)(uint32_t)(u >> 32), (int)(uint32_t)(u >> 64),
-                               (int)(uint32_t)(u >> 96));
+    __syncthreads();
+    for (int j = 0; j < sp.L; ++j) {
+      const int w = sp.first + j;
+      if (w == wave) continue;
+      const int4 *o = buf + w * 256 + half + lane;
+      const int4 x = o[0], y = o[64];
+      g.n.a += (int64_t)(((uint64_t)(uint32_t)x.y << 32) | (uint32_t)x.x);
+      g.n.b += (int64_t)(((uint64_t)(uint32_t)x.w << 32) | (uint32_t)x.z);
+      g.n.c += (int64_t)(((uint64_t)(uint32_t)y.y << 32) | (uint32_t)y.x);
+      g.n.d += (int64_t)(((uint64_t)(uint32_t)y.w << 32) | (uint32_t)y.z);
     }
+    return;
+  }
+  if (xs & 2u) __syncthreads();  // the last 64-bit exchange's reads are over
+  xs &= 1u;
+  int4 *mine = buf + wave * 256 + lane;
+  const i128_t v[4] = {g.acc.a, g.acc.b, g.acc.c, g.acc.d};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const u128_t u = (u128_t)v[k];
+    mine[64 * k] = make_int4((int)(uint32_t)u, (int)(uint32_t)(u >> 32), (int)(uint32_t)(u >> 64),
+                             (int)(uint32_t)(u >> 96));
   }
   __syncthreads();
   for (int j = 0; j < sp.L; ++j) {
     const int w = sp.first + j;
     if (w == wave) continue;
     const int4 *o = buf + w * 256 + lane;
-    if (narrow) {
-      const int4 x = o[0], y = o[64];
-      g.n.a += (int64_t)(((uint64_t)(uint32_t)x.y << 32) | (uint32_t)x.x);
-      g.n.b += (int64_t)(((uint64_t)(uint32_t)x.w << 32) | (uint32_t)x.z);
-      g.n.c += (int64_t)(((uint64_t)(uint32_t)y.y << 32) | (uint32_t)y.x);
-      g.n.d += (int64_t)(((uint64_t)(uint32_t)y.w << 32) | (uint32_t)y.z);
-    } else {
-      i128_t v[4];
+    i128_t r[4];
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const int4 x = o[64 * k];
-        v[k] = (i128_t)(((u128_t)(uint32_t)x.w << 96) | ((u128_t)(uint32_t)x.z << 64) |
-                        ((u128_t)(uint32_t)x.y << 32) | (u128_t)(uint32_t)x.x);
-      }
-      g.acc.a += v[0]; g.acc.b += v[1]; g.acc.c += v[2]; g.acc.d += v[3];
+    for (int k = 0; k < 4; ++k) {
+      const int4 x = o[64 * k];
+      r[k] = (i128_t)(((u128_t)(uint32_t)x.w << 96) | ((u128_t)(uint32_t)x.z << 64) |
+                      ((u128_t)(uint32_t)x.y << 32) | (u128_t)(uint32_t)x.x);
     }
+    g.acc.a += r[0]; g.acc.b += r[1]; g.acc.c += r[2]; g.acc.d += r[3];
   }
   __syncthreads();
 }
@@ -516,7 +529,8 @@ template <int W32, int WL, bool SPLIT, bool WIDE = true>
 __device__ __forceinline__ double graph_prob(const cword *__restrict__ P,
                                              const cword *__restrict__ gh,
                                              const uint32_t (&pv)[WL], const LdsTables *lt,
-                                             const Tables &tb, uint32_t &err, const Split &sp) {
+                                             const Tables &tb, uint32_t &err, const Split &sp,
+                                             uint32_t &xs) {
   const uint32_t gflags = gh[GH_FLAGS];
   const int e0 = (int)gh[GH_E0];
   const uint32_t nt = gh[GH_NTERMS];
@@ -536,7 +550,7 @@ __device__ __forceinline__ double graph_prob(const cword *__restrict__ P,
   sum_terms<W32, WL, WIDE>(P + off, t1 - t0, gflags, e0, pv, lt, tb, g, err);
   if (gflags & GF_MONO) return g.pm;
   const bool narrow = !WIDE || (gflags & GF_NARROW);
-  if (share) exchange_sums(g, narrow, sp);
+  if (share) exchange_sums(g, narrow, sp, xs);
   if (narrow) return magnitude(widen(g.n), e0);
   return magnitude(g.acc, e0);
 }
@@ -545,11 +559,12 @@ template <int WL, bool SPLIT = false, bool WIDE = true>
 __device__ __forceinline__ double graph_prob_any(const cword *__restrict__ P,
                                                  const cword *__restrict__ gh,
                                                  const uint32_t (&pv)[WL], const LdsTables *lt,
-                                                 const Tables &tb, uint32_t &err, const Split &sp) {
+                                                 const Tables &tb, uint32_t &err, const Split &sp,
+                                                 uint32_t &xs) {
   switch (gh[GH_W32]) {
 #define KSIM_GP(w)                                                                       \
   case w:                                                                                \
-    if constexpr (WL >= w) return graph_prob<w, WL, SPLIT, WIDE>(P, gh, pv, lt, tb, err, sp); \
+    if constexpr (WL >= w) return graph_prob<w, WL, SPLIT, WIDE>(P, gh, pv, lt, tb, err, sp, xs); \
     break;
     KSIM_GP(1) KSIM_GP(2) KSIM_GP(3) KSIM_GP(4) KSIM_GP(5) KSIM_GP(6) KSIM_GP(7) KSIM_GP(8)
 #undef KSIM_GP
@@ -762,7 +777,7 @@ template <int WL, bool NOISE = false, bool SPLIT = false, bool WIDE = true>
 __device__ __forceinline__ void run_shot(const ShotArgs &a, int comp, int64_t s,
                                          const LdsTables *lt, const Tables &tb,
                                          const Split &sp = Split{1, 0, 0, nullptr},
-                                         bool valid = true) {
+                                         bool valid = true, uint32_t *xs_io = nullptr) {
   const cword *__restrict__ P = as_const(a.prog);
   const cword *__restrict__ ch = P + P[PH_DWORDS + comp];
   const int ng = (int)ch[CH_NGRAPHS], F = (int)ch[CH_F];
@@ -846,6 +861,7 @@ __device__ __forceinline__ void run_shot(const ShotArgs &a, int comp, int64_t s,
   // per-component stream: seed + 0x9E37*k (src/python/ksim/sample/sampler.py:63)
   uint64_t rs = rng_seed(a.seed + 0x9E37ull * (uint64_t)comp, (uint64_t)(a.shot_base + s));
   uint32_t err = 0;
+  uint32_t xs = xs_io ? *xs_io : 0u;  // exchange state of the block (split launches)
   double prev = 0.0;
   // graph 0 gives the normaliser, graph i+1 the probability of output i = 1
   // (one evaluation site, so the graph code is inlined once); the decided
@@ -853,7 +869,7 @@ __device__ __forceinline__ void run_shot(const ShotArgs &a, int comp, int64_t s,
   for (int i = 0; i < ng; ++i) {
     const cword *gh = ghs + (size_t)i * GH_DWORDS;
     if (i > 0) set_param_bit<WL>(pv, F + i - 1, 1u);  // trying bit
-    const double p1 = graph_prob_any<WL, SPLIT, WIDE>(P, gh, pv, lt, tb, err, sp);
+    const double p1 = graph_prob_any<WL, SPLIT, WIDE>(P, gh, pv, lt, tb, err, sp, xs);
     if (i == 0) {
       prev = p1;
       continue;
@@ -864,6 +880,7 @@ __device__ __forceinline__ void run_shot(const ShotArgs &a, int comp, int64_t s,
     set_param_bit<WL>(pv, F + i - 1, bit);
     prev = bit ? p1 : __dsub_rn(prev, p1);
   }
+  if (xs_io) *xs_io = xs;
   if (SPLIT && (!valid || sp.part != 0)) {
     if (err) atomicOr(a.err, err);
     return;

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -1185,6 +1186,8 @@ struct ksim_sched {
   std::vector<BatchProg> slots;
   std::vector<int> slot_wl, slot_ncomp;
   std::vector<char> slot_narrow;
+  // per slot and component: per-wave cost of a 64-shot tile at split 1, 2, 4
+  std::vector<std::vector<std::array<double, 3>>> slot_cost;
   BatchProg *h_progs[KSIM_SCHED_BUFS] = {};
   BatchItem *h_items[KSIM_SCHED_BUFS] = {};
   BatchProg *d_progs[KSIM_SCHED_BUFS] = {};
@@ -1208,6 +1211,7 @@ int ksim_sched_create(int32_t max_slots, int32_t max_items, ksim_sched **out) {
     q->slot_wl.assign(max_slots, 0);
     q->slot_ncomp.assign(max_slots, 0);
     q->slot_narrow.assign(max_slots, 1);
+    q->slot_cost.assign(max_slots, {});
     try {
       for (int k = 0; k < KSIM_SCHED_BUFS; ++k) {
         hip_check(hipHostMalloc((void **)&q->h_progs[k], sizeof(BatchProg) * max_slots), "hipHostMalloc");
@@ -1240,6 +1244,9 @@ int ksim_sched_set_slot(ksim_sched *q, int32_t slot, ksim_program *p, const uint
     q->slot_wl[slot] = p->wl;
     q->slot_ncomp[slot] = p->n_comp;
     q->slot_narrow[slot] = p->all_narrow ? 1 : 0;
+    q->slot_cost[slot].assign(p->n_comp, {0.0, 0.0, 0.0});
+    for (int c = 0; c < p->n_comp; ++c)
+      for (int j = 0; j < 3; ++j) q->slot_cost[slot][c][j] = tile_cost(p, c, 1 << j);
   });
 }
 
@@ -1250,6 +1257,8 @@ int ksim_sched_clear_slot(ksim_sched *q, int32_t slot) {
     q->slots[slot] = BatchProg{nullptr, nullptr, nullptr, 0, 0, 0, 0, 0, 1};
     q->slot_wl[slot] = 0;
     q->slot_ncomp[slot] = 0;
+    q->slot_narrow[slot] = 1;
+    q->slot_cost[slot].clear();
   });
 }
 
@@ -1286,6 +1295,24 @@ int ksim_sched_launch(ksim_sched *q, const int32_t *items, int32_t n, int32_t bl
       wl = std::max(wl, q->slot_wl[slot]);
       narrow = narrow && q->slot_narrow[slot];
       q->h_items[k][i] = BatchItem{slot, comp, shot0, count, split};
+    }
+    // heaviest items first (per-wave tile cost x rounds), as in a batch: the
+    // cursor then hands the long tiles out while the short ones fill the end
+    {
+      std::vector<std::pair<double, int>> order(n);
+      for (int i = 0; i < n; ++i) {
+        const BatchItem &it = q->h_items[k][i];
+        const int G = 4 / it.split;
+        const double rounds = (double)(((it.count + 63) / 64 + G - 1) / G);
+        const int j = it.split == 1 ? 0 : it.split == 2 ? 1 : 2;
+        order[i] = {q->slot_cost[it.prog][it.comp][j] * rounds, i};
+      }
+      std::stable_sort(order.begin(), order.end(),
+                       [](const std::pair<double, int> &x, const std::pair<double, int> &y) {
+                         return x.first > y.first;
+                       });
+      std::vector<BatchItem> tmp(q->h_items[k], q->h_items[k] + n);
+      for (int i = 0; i < n; ++i) q->h_items[k][i] = tmp[order[i].second];
     }
     std::memcpy(q->h_progs[k], q->slots.data(), sizeof(BatchProg) * q->max_slots);
     hip_check(hipMemcpyAsync(q->d_progs[k], q->h_progs[k], sizeof(BatchProg) * q->max_slots,

@@ -148,10 +148,17 @@ __global__ void k_init_mag(const int4 *tab, double *mag) {
 template <int WL, bool NOISE = false, bool WIDE = true>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WIDE ? KSIM_WAVES_PER_EU : KSIM_NARROW_WAVES_PER_EU))) void k_sample(ShotArgs a, Tables tb) {
   __shared__ LdsTables lt;
+  __shared__ uint32_t stage[4][NOISE ? 1 : stage_words(WL)];
   load_tables(&lt, tb);
-  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= a.B) return;
-  run_shot<WL, NOISE, false, WIDE>(a, (int)blockIdx.y, s, &lt, tb);
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int64_t r0 = (int64_t)blockIdx.x * blockDim.x + 64 * wave;
+  if (r0 >= a.B) return;  // the whole wave is past the last shot
+  const int nrows = (int)min((int64_t)64, a.B - r0);
+  const int64_t s = r0 + min(lane, nrows - 1);
+  const FStage fs{NOISE ? nullptr : stage[wave], r0, nrows};
+  run_shot<WL, NOISE, false, WIDE>(a, (int)blockIdx.y, s, &lt, tb, Split{1, 0, 0, nullptr},
+                                   lane < nrows, nullptr, fs);
 }
 
 // Plain path with a term split: L waves (2 or 4) share each graph's terms
@@ -295,6 +302,14 @@ __global__ void k_reset_cursor(uint32_t *cursor, uint32_t *clear) {
   }
 }
 
+// The batch kernel stages f rows only where the extra LDS keeps the
+// resident block count: the 128-bit kernels (5 blocks per CU) with at most
+// 4 parameter words per lane.  The plain split kernel reads f lane by lane:
+// its launches are too small for the f traffic to matter, and the staged
+// read measured 3-21% slower there at L = 4 (more spilled registers).
+template <int WL, bool WIDE>
+constexpr bool kStageBatch = WIDE && WL <= 4;
+
 template <int WL, bool WIDE = true>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WIDE ? KSIM_WAVES_PER_EU : KSIM_NARROW_WAVES_PER_EU))) void k_batch(
     const BatchProg *__restrict__ progs, const BatchItem *__restrict__ items, uint32_t n_items,
@@ -302,6 +317,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WIDE ? K
   __shared__ LdsTables lt;
   __shared__ uint32_t s_item;
   __shared__ int4 xch[XCH_INT4];
+  __shared__ uint32_t stage[4][kStageBatch<WL, WIDE> ? stage_words(WL) : 1];
   load_tables(&lt, tb);
   uint32_t xs = 0;  // exchange state, the same in every wave of the block
   // the wave index is the same on every lane of a wave: readfirstlane lets
@@ -334,7 +350,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WIDE ? K
       const int off = tile * 64 + lane;
       const bool valid = off < it.count;
       const int64_t s = it.shot0 + min(off, it.count - 1);
-      run_shot<WL, false, true, WIDE>(a, it.comp, s, &lt, tb, sp, valid, &xs);
+      // a tile past the item's end (split groups) repeats its last shot
+      const int t0 = min(tile * 64, it.count - 1);
+      const FStage fs{kStageBatch<WL, WIDE> ? stage[wave] : nullptr, it.shot0 + t0,
+                      min(64, it.count - t0)};
+      run_shot<WL, false, true, WIDE>(a, it.comp, s, &lt, tb, sp, valid, &xs, fs);
     }
   }
 }

@@ -5,9 +5,12 @@
 // time), so records are read through scalar loads into SGPRs and each record
 // costs a few VALU ops per lane: AND + popcount per 32-bit mask word, then a
 // masked add of a packed exponent delta (see layout.hpp).  The exact term
-// value is built from an LDS table, shifted to a per-graph exponent and added
-// into four 128-bit accumulators, so the sum over terms is exact (the
-// reference sums in float64, src/cpp/ksim/zx_eval.hpp:168-178).
+// value is built from a table (first digit plane read from global memory and
+// held in the vector L1; pi powers and float64 magnitudes staged in LDS),
+// shifted to a per-graph exponent and added into four integer accumulators:
+// int64 when the packer proved the graph's sums fit, 128-bit otherwise, so
+// the sum over terms is exact (the reference sums in float64,
+// src/cpp/ksim/zx_eval.hpp:168-178).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -767,17 +770,110 @@ __device__ __forceinline__ void write_packed_row(const ShotArgs &a, const uint32
   }
 }
 
+// ------------------------------------------------------- f rows of a wave
+// A wave's 64 shots are 64 consecutive f rows, i.e. one contiguous byte
+// range.  Read lane by lane (each lane its own row, e.g. 100 bytes apart),
+// every load instruction touches all of the range's cache lines and the
+// same lines are fetched again by the next instruction after the L2 has
+// turned over (about 3x the f bytes fetched from HBM for 100-byte rows at
+// full occupancy).  Staged, the range is read once in aligned 16-byte
+// pieces by consecutive lanes, and bit 0 of every byte goes into a per-wave
+// LDS bit string; each lane then takes its parameter words out of it with
+// funnel shifts.  Used when a row is at most 32 * WL bytes and f is 16-byte
+// aligned (torch allocations are); the string then needs stage_words(WL)
+// dwords.
+__host__ __device__ constexpr int stage_words(int WL) { return 65 * WL + 2; }
+
+struct FStage {
+  uint32_t *st;   // the wave's bit string (stage_words(WL) dwords), or nullptr
+  int64_t r0;     // first row of the wave's tile
+  int nrows;      // rows of the tile inside f (1..64)
+};
+
+// bit 0 of each of 16 bytes, byte k -> bit k
+__device__ __forceinline__ uint32_t low_bits16(const U32x4 &q) {
+  uint32_t b = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) b |= (((q.v[k] & 0x01010101u) * 0x01020408u) >> 24) << (4 * k);
+  return b;
+}
+
+// Fill the wave's bit string with rows [r0, r0 + nrows) of f: bit j of the
+// string is bit 0 of byte (start + j), start = the first row's address
+// rounded down to 16 bytes.  Every lane of the wave calls it with the same
+// arguments.  Returns the string offset of row r0's first byte.
+__device__ __forceinline__ int stage_f_rows(const ShotArgs &a, const FStage &fs) {
+  const int lane = threadIdx.x & 63;
+  const uint8_t *beg = a.f + fs.r0 * (int64_t)a.num_f;
+  const uint8_t *start = (const uint8_t *)((uintptr_t)beg & ~(uintptr_t)15);
+  const uint8_t *fend = a.f + a.B * (int64_t)a.num_f;
+  const int lead = (int)(beg - start);
+  const int n16 = (lead + fs.nrows * a.num_f + 15) >> 4;
+  uint16_t *st16 = (uint16_t *)fs.st;
+  // the string is rewritten for every tile: finish the previous tile's reads
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  // pieces entirely inside f, four per lane in flight before the first wait
+  const int nfull = (int)min((int64_t)n16, (int64_t)((fend - start) >> 4));
+  for (int i0 = 0; i0 < nfull; i0 += 4 * 64) {
+    U32x4 q[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int i = i0 + 64 * u + lane;
+      q[u] = i < nfull ? *(const U32x4 *)(start + 16 * (int64_t)i) : U32x4{{0u, 0u, 0u, 0u}};
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int i = i0 + 64 * u + lane;
+      if (i < nfull) st16[i] = (uint16_t)low_bits16(q[u]);
+    }
+  }
+  // the piece that runs past the end of f (the last row of f): byte by byte
+  if (nfull < n16 && lane == 0) {
+    const uint8_t *p = start + 16 * (int64_t)nfull;
+    uint32_t b = 0;
+    for (int k = 0; p + k < fend; ++k) b |= (uint32_t)(p[k] & 1u) << k;
+    st16[nfull] = (uint16_t)b;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  return lead;
+}
+
+// the lane's parameter words: F bits of the staged string from bit `off`
+template <int WL>
+__device__ __forceinline__ void staged_words(const uint32_t *st, int off, int F,
+                                             uint32_t (&pv)[WL]) {
+#pragma unroll
+  for (int w = 0; w < WL; ++w) {
+    uint32_t v = 0;
+    if (32 * w < F) {
+      const int j = off + 32 * w;
+      const uint32_t lo = st[j >> 5], hi = st[(j >> 5) + 1];
+      v = __builtin_amdgcn_alignbit(hi, lo, (uint32_t)(j & 31));
+      const int left = F - 32 * w;
+      if (left < 32) v &= (1u << left) - 1u;
+    }
+    pv[w] = v;
+  }
+}
+
 // The whole autoregressive loop of one shot of component `comp`
 // (src/cpp/ksim/kokkos_sim.cpp:214-230, src/python/ksim/sample/evaluate.py:76-85).
 // SPLIT: the waves of sp's group share each graph's terms; every lane runs
 // the loop (lanes without a shot of their own run a copy of a valid shot,
 // so the group's barriers are reached by whole waves) and only `valid`
-// lanes of the group's first wave write.
+// lanes of the group's first wave write.  Lanes that are not `valid` run a
+// copy of a valid shot of their wave in the plain path too, so that a
+// staged f read (fs.st set) has every lane of the wave; fs then names the
+// wave's tile, and s lies in it.
 template <int WL, bool NOISE = false, bool SPLIT = false, bool WIDE = true>
 __device__ __forceinline__ void run_shot(const ShotArgs &a, int comp, int64_t s,
                                          const LdsTables *lt, const Tables &tb,
                                          const Split &sp = Split{1, 0, 0, nullptr},
-                                         bool valid = true, uint32_t *xs_io = nullptr) {
+                                         bool valid = true, uint32_t *xs_io = nullptr,
+                                         const FStage &fs = FStage{nullptr, 0, 0}) {
   const cword *__restrict__ P = as_const(a.prog);
   const cword *__restrict__ ch = P + P[PH_DWORDS + comp];
   const int ng = (int)ch[CH_NGRAPHS], F = (int)ch[CH_F];
@@ -816,6 +912,11 @@ __device__ __forceinline__ void run_shot(const ShotArgs &a, int comp, int64_t s,
                    }
                  }
                });
+  } else if (fs.st != nullptr && fsel0 != 0xffffffffu && a.num_f <= 32 * WL &&
+             ((uintptr_t)a.f & 15u) == 0) {
+    // the wave's rows through its LDS bit string (one read of each line)
+    const int lead = stage_f_rows(a, fs);
+    staged_words<WL>(fs.st, lead + (int)(s - fs.r0) * a.num_f + (int)fsel0, F, pv);
   } else if (fsel0 != 0xffffffffu &&
              (((uint32_t)a.num_f | fsel0 | (uint32_t)(uintptr_t)a.f) & 3u) == 0) {
     // contiguous selection on a dword-aligned row: 16 f bytes per load,
@@ -881,7 +982,7 @@ __device__ __forceinline__ void run_shot(const ShotArgs &a, int comp, int64_t s,
     prev = bit ? p1 : __dsub_rn(prev, p1);
   }
   if (xs_io) *xs_io = xs;
-  if (SPLIT && (!valid || sp.part != 0)) {
+  if (!valid || (SPLIT && sp.part != 0)) {
     if (err) atomicOr(a.err, err);
     return;
   }

@@ -281,3 +281,38 @@ def test_plain_term_split_is_identical(n_components):
     # and the fed-uniform stream is the oracle's (first 300 shots: the
     # big-integer oracle is slow)
     assert np.array_equal(got[4][2][:300], exact.sample_program(prog, f[:300], U[:300]))
+
+
+@pytest.mark.parametrize("n_components,extra_cols,n_out", [(1, 0, 6), (2, 3, 40), (2, 300, 40)])
+def test_f_row_reads_agree(n_components, extra_cols, n_out):
+    # a 16-byte aligned f with rows of at most 32 * WL bytes is read by the
+    # whole wave through LDS; any other f lane by lane: the same bits, on
+    # row widths that are not multiples of 4 or 16 (extra_cols), a second
+    # component whose f bits start mid-row, shot counts that end inside a
+    # tile and inside a 16-byte piece, and (300 extra columns) rows too
+    # long to stage
+    import torch
+    from ksim_amd import _lib
+    prog = synth.shape_program(n_q=8, t=8, F=37, n_out=n_out, seed=51, n_components=n_components)
+    s = HipProgramSampler(prog)
+    num_f = s.min_num_f + extra_cols
+    lib = _lib.load()
+    for B in (1, 63, 65, 1000, 1031):
+        f = synth.random_f(B, num_f, 0.4, B)
+        rows = {}
+        for shift in (0, 4, 1):  # byte offset of f from a 256-byte aligned allocation
+            buf = torch.zeros(B * num_f + 16, dtype=torch.uint8, device="cuda")
+            buf[shift:shift + B * num_f] = torch.from_numpy(f.reshape(-1)).cuda()
+            out = torch.full((B, s.num_outputs), 7, dtype=torch.uint8, device="cuda")
+            _lib.check(lib.ksim_program_sample_device(s.handle, buf.data_ptr() + shift, B, num_f,
+                                                      0, 99, None, out.data_ptr(), None))
+            s.check()
+            rows[shift] = out.cpu().numpy()
+        assert np.array_equal(rows[4], rows[0]), B
+        assert np.array_equal(rows[1], rows[0]), B
+        assert np.array_equal(rows[0], s.sample(f, 99)), B
+    # fed uniforms against the oracle on a short run
+    B = 70
+    f = synth.random_f(B, num_f, 0.4, 5)
+    U = np.random.default_rng(6).random((B, s.tot_out))
+    assert np.array_equal(s.sample_uniforms(f, U), exact.sample_program(prog, f, U))

@@ -141,7 +141,9 @@ int ksim_program_eval_exact(ksim_program *p, int32_t comp, int32_t graph,
  * u8, d_U optional (NULL: seeded stream), d_out (B, num_outputs) u8.
  * shot_base offsets the shot index that keys the per-shot stream, so a range
  * split of one request samples exactly what a single call would.  Enqueued
- * on `stream` (hipStream_t, NULL = default); returns without waiting. */
+ * on `stream` (hipStream_t, NULL = default); returns without waiting.  A
+ * 16-byte aligned d_f is read once per 64-row tile, coalesced; any other
+ * alignment is accepted and read row by row. */
 int ksim_program_sample_device(ksim_program *p, const uint8_t *d_f, int64_t B,
                                int32_t f_cols, int64_t shot_base, uint64_t seed,
                                const double *d_U, uint8_t *d_out, void *stream);

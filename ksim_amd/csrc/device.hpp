@@ -629,6 +629,31 @@ __device__ __forceinline__ void set_param_bit(uint32_t (&pv)[WL], int pos, uint3
   }
 }
 
+// Decide parameter bit `pos`, which holds 1 (the trying bit), to `bit`, and
+// set bit pos + 1 (the next graph's trying bit) in the same word update when
+// `next` and it lies in the same word.  Returns whether the next trying bit
+// still has to be set (it starts a new word).
+template <int WL>
+__device__ __forceinline__ bool decide_param_bit(uint32_t (&pv)[WL], int pos, uint32_t bit,
+                                                 bool next) {
+  const int word = __builtin_amdgcn_readfirstlane(pos >> 5);
+  const uint32_t m = 1u << (pos & 31);
+  const bool same = next && (pos & 31) != 31;
+  const uint32_t mn = same ? m << 1 : 0u;
+  const uint32_t clr = m & ~(0u - (bit & 1u));  // the trying bit of lanes that decided 0
+  switch (word) {
+#define KSIM_DECIDE_WORD(w)                             \
+  case w:                                               \
+    if constexpr (w < WL) pv[w] = (pv[w] & ~clr) | mn;   \
+    break;
+    KSIM_DECIDE_WORD(0) KSIM_DECIDE_WORD(1) KSIM_DECIDE_WORD(2) KSIM_DECIDE_WORD(3)
+    KSIM_DECIDE_WORD(4) KSIM_DECIDE_WORD(5) KSIM_DECIDE_WORD(6) KSIM_DECIDE_WORD(7)
+#undef KSIM_DECIDE_WORD
+    default: break;
+  }
+  return next && !same;
+}
+
 // parameter bit `pos` (wave-uniform) of the lane's words
 template <int WL>
 __device__ __forceinline__ uint32_t get_param_bit(const uint32_t (&pv)[WL], int pos) {
@@ -967,9 +992,11 @@ __device__ __forceinline__ void run_shot(const ShotArgs &a, int comp, int64_t s,
   // graph 0 gives the normaliser, graph i+1 the probability of output i = 1
   // (one evaluation site, so the graph code is inlined once); the decided
   // bits stay in pv (parameter F + i) until the row is written at the end
+  // graph i > 0 runs with its trying bit (parameter F + i - 1) at 1; graph 0
+  // does not read parameter F, so graph 1's trying bit is set up front
+  if (ng > 1) set_param_bit<WL>(pv, F, 1u);
   for (int i = 0; i < ng; ++i) {
     const cword *gh = ghs + (size_t)i * GH_DWORDS;
-    if (i > 0) set_param_bit<WL>(pv, F + i - 1, 1u);  // trying bit
     const double p1 = graph_prob_any<WL, SPLIT, WIDE>(P, gh, pv, lt, tb, err, sp, xs);
     if (i == 0) {
       prev = p1;
@@ -978,7 +1005,8 @@ __device__ __forceinline__ void run_shot(const ShotArgs &a, int comp, int64_t s,
     const double u =
         a.U ? a.U[s * (int64_t)a.tot_out + colbase + i - 1] : rng_uniform(rs);
     const uint32_t bit = (u < __ddiv_rn(p1, prev)) ? 1u : 0u;
-    set_param_bit<WL>(pv, F + i - 1, bit);
+    if (decide_param_bit<WL>(pv, F + i - 1, bit, i + 1 < ng))
+      set_param_bit<WL>(pv, F + i, 1u);
     prev = bit ? p1 : __dsub_rn(prev, p1);
   }
   if (xs_io) *xs_io = xs;

@@ -884,6 +884,41 @@ __device__ __forceinline__ void staged_words(const uint32_t *st, int off, int F,
   }
 }
 
+// Graphs i, i + 1, ... of mask width W32 (up to the first of another
+// width) of one shot: graph 0 gives the normaliser, graph i > 0 the
+// probability that output i - 1 is 1, with its trying bit (parameter
+// F + i - 1) at 1, and the drawn bit then stays in that parameter.  Graph
+// i has P = F + i parameters and W32 = ceil((F + i) / 32) mask words
+// (checked at upload), so parameter F + i - 1 lies in word W32 - 1: the
+// bit is set and decided in a register named at compile time.  (A
+// switch over a runtime word index here made the compiler copy all WL
+// words at every graph.)  Returns the first graph not run.
+template <int W32, int WL, bool SPLIT, bool WIDE>
+__device__ __forceinline__ int run_graphs(const ShotArgs &a, const cword *__restrict__ P,
+                                          const cword *__restrict__ ghs, int i, int ng, int F,
+                                          int colbase, int64_t s, uint32_t (&pv)[WL],
+                                          const LdsTables *lt, const Tables &tb, const Split &sp,
+                                          double &prev, uint64_t &rs, uint32_t &err,
+                                          uint32_t &xs) {
+  static_assert(W32 <= WL, "graph wider than the lane's parameter words");
+  for (; i < ng; ++i) {
+    const cword *gh = ghs + (size_t)i * GH_DWORDS;
+    if ((int)gh[GH_W32] != W32) break;
+    const uint32_t m = 1u << ((F + i - 1) & 31);
+    if (i > 0) pv[W32 - 1] |= m;  // trying bit
+    const double p1 = graph_prob<W32, WL, SPLIT, WIDE>(P, gh, pv, lt, tb, err, sp, xs);
+    if (i == 0) {
+      prev = p1;
+      continue;
+    }
+    const double u = a.U ? a.U[s * (int64_t)a.tot_out + colbase + i - 1] : rng_uniform(rs);
+    const uint32_t bit = (u < __ddiv_rn(p1, prev)) ? 1u : 0u;
+    pv[W32 - 1] &= ~(m & (bit - 1u));  // lanes that drew 0 clear the trying bit
+    prev = bit ? p1 : __dsub_rn(prev, p1);
+  }
+  return i;
+}
+
 // The whole autoregressive loop of one shot of component `comp`
 // (src/cpp/ksim/kokkos_sim.cpp:214-230, src/python/ksim/sample/evaluate.py:76-85).
 // SPLIT: the waves of sp's group share each graph's terms; every lane runs
@@ -989,25 +1024,48 @@ __device__ __forceinline__ void run_shot(const ShotArgs &a, int comp, int64_t s,
   uint32_t err = 0;
   uint32_t xs = xs_io ? *xs_io : 0u;  // exchange state of the block (split launches)
   double prev = 0.0;
-  // graph 0 gives the normaliser, graph i+1 the probability of output i = 1
-  // (one evaluation site, so the graph code is inlined once); the decided
-  // bits stay in pv (parameter F + i) until the row is written at the end
-  // graph i > 0 runs with its trying bit (parameter F + i - 1) at 1; graph 0
-  // does not read parameter F, so graph 1's trying bit is set up front
-  if (ng > 1) set_param_bit<WL>(pv, F, 1u);
-  for (int i = 0; i < ng; ++i) {
-    const cword *gh = ghs + (size_t)i * GH_DWORDS;
-    const double p1 = graph_prob_any<WL, SPLIT, WIDE>(P, gh, pv, lt, tb, err, sp, xs);
-    if (i == 0) {
-      prev = p1;
-      continue;
+  // graph 0 gives the normaliser, graph i+1 the probability of output i = 1;
+  // the decided bits stay in pv (parameter F + i) until the row is written
+  // at the end.  The plain kernel runs the graphs a width at a time, in
+  // loops specialised on it (run_graphs); the kernels whose waves share
+  // terms keep one evaluation site and a runtime word index, because the
+  // specialised loops there cost registers (spills made the 128-bit split
+  // kernel up to 39 % slower)
+  if constexpr (SPLIT) {
+    // graph i > 0 runs with its trying bit (parameter F + i - 1) at 1; graph 0
+    // does not read parameter F, so graph 1's trying bit is set up front
+    if (ng > 1) set_param_bit<WL>(pv, F, 1u);
+    for (int i = 0; i < ng; ++i) {
+      const cword *gh = ghs + (size_t)i * GH_DWORDS;
+      const double p1 = graph_prob_any<WL, SPLIT, WIDE>(P, gh, pv, lt, tb, err, sp, xs);
+      if (i == 0) {
+        prev = p1;
+        continue;
+      }
+      const double u =
+          a.U ? a.U[s * (int64_t)a.tot_out + colbase + i - 1] : rng_uniform(rs);
+      const uint32_t bit = (u < __ddiv_rn(p1, prev)) ? 1u : 0u;
+      if (decide_param_bit<WL>(pv, F + i - 1, bit, i + 1 < ng))
+        set_param_bit<WL>(pv, F + i, 1u);
+      prev = bit ? p1 : __dsub_rn(prev, p1);
     }
-    const double u =
-        a.U ? a.U[s * (int64_t)a.tot_out + colbase + i - 1] : rng_uniform(rs);
-    const uint32_t bit = (u < __ddiv_rn(p1, prev)) ? 1u : 0u;
-    if (decide_param_bit<WL>(pv, F + i - 1, bit, i + 1 < ng))
-      set_param_bit<WL>(pv, F + i, 1u);
-    prev = bit ? p1 : __dsub_rn(prev, p1);
+  } else {
+    for (int i = 0; i < ng;) {
+      switch (ghs[(size_t)i * GH_DWORDS + GH_W32]) {
+#define KSIM_RUN(w)                                                                          \
+  case w:                                                                                  \
+    if constexpr (WL >= w)                                                                 \
+      i = run_graphs<w, WL, SPLIT, WIDE>(a, P, ghs, i, ng, F, colbase, s, pv, lt, tb, sp,  \
+                                         prev, rs, err, xs);                               \
+    else                                                                                   \
+      i = ng;                                                                              \
+    break;
+        KSIM_RUN(1) KSIM_RUN(2) KSIM_RUN(3) KSIM_RUN(4)
+        KSIM_RUN(5) KSIM_RUN(6) KSIM_RUN(7) KSIM_RUN(8)
+#undef KSIM_RUN
+        default: i = ng; break;  // widths are checked at upload
+      }
+    }
   }
   if (xs_io) *xs_io = xs;
   if (!valid || (SPLIT && sp.part != 0)) {

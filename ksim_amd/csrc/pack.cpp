@@ -625,6 +625,11 @@ std::vector<uint32_t> pack_program(const ksim_graph_view *graphs, const int32_t 
     w.resize(gh + (size_t)ng * GH_DWORDS, 0u);
     for (int i = 0; i < ng; ++i) {
       Job &j = jobs[gi + i];
+      // graph i has P = F + i parameters, so its width is ceil((F + i) / 32);
+      // a graph without terms (whose P is not checked) takes that width too,
+      // so that every graph's last parameter, the trying bit of the
+      // sampler, lies in its last mask word
+      if (j.hdr[GH_NTERMS] == 0) j.hdr[GH_W32] = (uint32_t)std::max(1, (F + i + 31) / 32);
       j.hdr[GH_TERM_OFF] += (uint32_t)w.size();  // relative -> absolute
       for (int p = 0; p < GH_PARTS - 1; ++p) j.hdr[GH_PART_O1 + p] += (uint32_t)w.size();
       std::memcpy(&w[gh + (size_t)i * GH_DWORDS], j.hdr, sizeof(j.hdr));

@@ -95,6 +95,7 @@ void validate_words(const uint32_t *w, size_t n) {
       const uint32_t *gh = w + ch[CH_GRAPH_OFF] + (size_t)g * GH_DWORDS;
       const uint32_t W32 = gh[GH_W32];
       if (W32 < 1 || W32 > cw) bad("graph mask words");
+      if (W32 != std::max<uint32_t>(1u, (F + g + 31) / 32)) bad("graph width is not ceil((F + i) / 32)");
       const uint32_t nt = gh[GH_NTERMS];
       if (gh[GH_FLAGS] & ~(GF_MONO | GF_NARROW)) bad("graph flags");
       if ((gh[GH_FLAGS] & GF_MONO) && nt != 1) bad("single-term flag on a graph of several terms");

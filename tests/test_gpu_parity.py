@@ -316,3 +316,20 @@ def test_f_row_reads_agree(n_components, extra_cols, n_out):
     f = synth.random_f(B, num_f, 0.4, 5)
     U = np.random.default_rng(6).random((B, s.tot_out))
     assert np.array_equal(s.sample_uniforms(f, U), exact.sample_program(prog, f, U))
+
+
+def test_widths_change_and_a_graph_without_terms():
+    # the sampler runs the graphs a mask width at a time; here the width
+    # steps from 1 to 2 words between outputs 0 and 1 (F = 31), and output 1
+    # comes from a graph without terms (probability 0)
+    g = synth.shape_program(n_q=6, t=4, F=31, n_out=4, seed=61).components[0].graphs
+    graphs = [g[0], g[1], synth.empty_graph(0, 0), g[3], g[4]]
+    prog = synth.single_program(graphs, F=31)
+    s = HipProgramSampler(prog)
+    B = 200
+    f = synth.random_f(B, 31, 0.3, 7)
+    U = np.random.default_rng(8).random((B, s.tot_out))
+    got = s.sample_uniforms(f, U)
+    assert np.array_equal(got, exact.sample_program(prog, f, U))
+    assert not got[:, 1].any()
+    assert np.array_equal(s.sample(f, 5)[:, 1], np.zeros(B, np.uint8))

@@ -218,3 +218,16 @@ def test_pack_stats_ops_match_the_formula():
         assert st["words"] == len(w)
         assert st["ops_per_shot"] == algorithmic_ops(prog)
         assert 0 < st["sum_bits"] <= 128
+
+
+def test_graph_widths_follow_their_position():
+    # graph i is ceil((F + i) / 32) mask words wide, and so is a graph
+    # without terms (whose own P is not checked): the sampler keeps graph
+    # i's trying bit, parameter F + i - 1, in the graph's last mask word
+    F = 31
+    graphs = [synth.constant_graph(F), synth.constant_graph(F + 1), synth.empty_graph(0, 0),
+              synth.constant_graph(F + 3)]
+    w = pack_words(synth.single_program(graphs, F=F))
+    c0 = int(w[8])
+    gh = int(w[c0 + 7])
+    assert [int(w[gh + 16 * i + 1]) for i in range(4)] == [1, 1, 2, 2]

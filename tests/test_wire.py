@@ -48,7 +48,7 @@ def test_blob_holds_the_packed_words(prog, blob):
     st = {}
     w = pack_words(prog, st)
     assert blob[:8] == b"KSIMPRG\0"
-    assert struct.unpack("<II", blob[8:16]) == (2, HDR)
+    assert struct.unpack("<II", blob[8:16]) == (3, HDR)
     assert np.array_equal(words_of(blob), w)
     assert struct.unpack("<Q", blob[24:32])[0] == fnv1a64(w.tobytes())
     got = blob_stats(blob)
@@ -102,6 +102,7 @@ def structural_cases(w):
     mut("f_selection", fsel, int(w[1]) + 3)
     mut("output column", dest, int(w[2]))
     mut("graph mask words", g1 + 1, 9)
+    mut("graph width", g1 + 1, 2 if int(w[g1 + 1]) == 1 else 1)
     mut("term offset", g1 + 3, len(w) - 4)
     mut("term count", g1 + 0, int(w[g1 + 0]) + 1000)
     mut("term size", t0 + 14, int(w[t0 + 14]) + 1)
@@ -119,7 +120,7 @@ def structural_cases(w):
 def test_rejects_structural_corruption(blob):
     w = words_of(blob)
     cases = structural_cases(w)
-    assert len(cases) == 15
+    assert len(cases) == 16
     for name, v in cases:
         with pytest.raises(ValueError):
             blob_stats(with_words(blob, v))

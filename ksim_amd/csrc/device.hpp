@@ -1077,8 +1077,15 @@ __device__ __forceinline__ void run_shot(const ShotArgs &a, int comp, int64_t s,
   if (a.out_words > 0) {
     write_packed_row<WL>(a, pv, ch, dest, F, ng - 1, s);
   } else {
+    // output i is parameter F + i: walk the words that hold them with
+    // indices known at compile time (no runtime word switch per output)
     uint8_t *orow = a.out + s * (int64_t)a.num_outputs;
-    for (int i = 1; i < ng; ++i) orow[dest[i - 1]] = (uint8_t)get_param_bit<WL>(pv, F + i - 1);
+    const int nout = ng - 1;
+#pragma unroll
+    for (int w = 0; w < WL; ++w) {
+      const int i0 = max(32 * w - F, 0), i1 = min(32 * w + 32 - F, nout);
+      for (int i = i0; i < i1; ++i) orow[dest[i]] = (uint8_t)((pv[w] >> ((F + i) & 31)) & 1u);
+    }
   }
   if (err) atomicOr(a.err, err);
 }

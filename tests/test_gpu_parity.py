@@ -333,3 +333,53 @@ def test_widths_change_and_a_graph_without_terms():
     assert np.array_equal(got, exact.sample_program(prog, f, U))
     assert not got[:, 1].any()
     assert np.array_equal(s.sample(f, 5)[:, 1], np.zeros(B, np.uint8))
+
+
+def _boundary_uniforms(prog, f, rng):
+    """Fed uniforms that sit on the rounding boundary of every draw: along
+    the oracle's own path, u is fl(p1 / prev) itself (bit 0), the double
+    just below it (bit 1), the double just above (bit 0) or a plain draw."""
+    from ksim_amd.program import graph_to_dict
+    B = f.shape[0]
+    cols = []
+    for comp in prog.components:
+        gds = [graph_to_dict(g) for g in comp.graphs]
+        f_sel = f[:, np.asarray(comp.f_selection, dtype=np.int64)].astype(np.int64)
+        U = np.zeros((B, len(gds) - 1))
+        for s in range(B):
+            prev = exact.magnitude(*exact.graph_value(gds[0], f_sel[s]))
+            m = []
+            for i in range(len(gds) - 1):
+                p1 = exact.magnitude(*exact.graph_value(gds[i + 1], exact.params_row(f_sel[s], m, i)))
+                with np.errstate(divide="ignore", invalid="ignore"):
+                    q = np.float64(p1) / np.float64(prev)
+                k = rng.integers(4)
+                if not np.isfinite(q) or k == 3:
+                    u = rng.random()
+                else:
+                    u = (q, np.nextafter(q, -np.inf), np.nextafter(q, np.inf))[k]
+                U[s, i] = u
+                bit = 1 if u < q else 0
+                m.append(bit)
+                prev = p1 if bit else prev - p1
+        cols.append(U)
+    return np.concatenate(cols, axis=1)
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_draws_at_the_rounding_boundary(seed):
+    # bit = u < fl(p1 / prev): uniforms placed on the rounding boundary of
+    # every draw (u = fl(q) and its neighbours) give the reference's bits,
+    # one wave per tile and four
+    rng = np.random.default_rng(seed)
+    progs = [synth.shape_program(n_q=5, t=4, F=6, n_out=6, seed=70 + seed),
+             synth.shape_program(n_q=6, t=2, F=40, n_out=30, seed=72 + seed, n_components=2),
+             synth.htkh_program(3)]
+    for prog in progs:
+        s = HipProgramSampler(prog)
+        f = synth.random_f(64, max(s.min_num_f, 1), 0.3, 9 + seed)
+        U = _boundary_uniforms(prog, f, rng)
+        want = exact.sample_program(prog, f, U)
+        assert np.array_equal(s.sample_uniforms(f, U), want)
+        s.set_term_split(4)
+        assert np.array_equal(s.sample_uniforms(f, U), want)

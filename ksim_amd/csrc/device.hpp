@@ -609,9 +609,11 @@ __device__ __forceinline__ void eval_graph_any(const cword *__restrict__ P,
 }
 
 // set parameter bit `pos` of the lane's words to `bit` (pos is wave-uniform:
-// a scalar branch picks the one word, so pv stays in registers; measured
-// faster than rewriting every word with a uniform word mask, which costs
-// one VALU op per word)
+// a scalar switch picks the word).  The compiler copies the other words at
+// the switch's merge point, so the plain kernel's graph loop names the word
+// at compile time instead (run_graphs); this form serves the split and
+// batch kernels and the noise path, where it measured faster than
+// rewriting every word with a uniform word mask (one VALU op per word)
 template <int WL>
 __device__ __forceinline__ void set_param_bit(uint32_t (&pv)[WL], int pos, uint32_t bit) {
   const int word = __builtin_amdgcn_readfirstlane(pos >> 5);

@@ -184,6 +184,25 @@ def scale_cpu_shots(us_guess, budget_s=12.0, lo=200, hi=200_000):
     return int(min(hi, max(lo, budget_s / max(us_guess, 1e-3) * 1e6)))
 
 
+def host_path(pl, reps=3):
+    """End to end through the host-array API, the reference's calling
+    convention: f from host memory, H2D, kernel, D2H of the output rows
+    (u8 rows, and bit-packed rows), returned as numpy arrays.  Reported
+    beside the resident rate, never as the headline value."""
+    f = pl.f.cpu().numpy()
+    res = {}
+    for name, fn in (("u8_rows", pl.s.sample), ("packed_rows", pl.s.sample_packed)):
+        fn(f, 12345)  # warm-up: staging buffers
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn(f, 12345)
+        dt = (time.perf_counter() - t0) / reps
+        res[name] = dict(ms=dt * 1e3, shots_per_s=pl.B / dt)
+    res["note"] = ("HipProgramSampler.sample / sample_packed with numpy f in and numpy rows "
+                   "out (pageable host memory), mean of %d calls" % reps)
+    return res
+
+
 def plain_result(name, steps, warmup, with_cpu=True):
     import torch
     w = make_workload(name)
@@ -199,6 +218,7 @@ def plain_result(name, steps, warmup, with_cpu=True):
                shots_per_s=shots_s, us_per_shot=1e6 / shots_s,
                kernel_ms=kern * 1e3, ops_per_shot=pl.ops, stats=pl.s.stats(),
                roofline=valu_roofline(achieved, pl.bytes_per_shot * w["B"] / kern, name))
+    res["host_path"] = host_path(pl)
     if with_cpu:
         # about 10-20 s of CPU work
         probe = cpu_baseline_port(w["prog"], w["F"], 200)
@@ -492,13 +512,15 @@ def main():
     }
     if "noise" in res:
         line["noise"] = res["noise"]
+    line["host_path"] = res.get("host_path")
     if args.suite:
         line["suite"] = {}
         for name in ("config1", "config3"):
             r = plain_result(name, max(2, args.steps // 2), 1, with_cpu=not args.no_cpu)
             line["suite"][name] = {k: r[k] for k in ("shots_per_s", "us_per_shot", "kernel_ms",
                                                      "roofline", "cpu_baseline",
-                                                     "cpu_baseline_numpy") if k in r}
+                                                     "cpu_baseline_numpy", "host_path")
+                                   if k in r}
         line["suite"]["config4"] = batched_result(max(2, args.steps // 2), 1)
         line["suite"]["tcount_sweep"] = tcount_sweep()
         line["suite"]["config5"] = serving_result()

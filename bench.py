@@ -419,7 +419,7 @@ def serving_result(n_requests=48, n_pool=8, load=0.7, t_max=40, q_max=85, seed=5
         reqs.append(Request(key=k, prog=pool[k], f=f, seed=20_000 + i))
     total_ops = sum(pool[r.key].ops_per_shot * r.shots for r in reqs)
     # saturation: the whole stream at once (no arrival gaps)
-    loop = ServingLoop(hbm_budget=96 << 30, max_slots=64, step_ms=8.0)
+    loop = ServingLoop(hbm_budget=96 << 30, max_slots=64, step_ms=30.0)
     warm = [Request(key=r.key, prog=r.prog, f=r.f[:64], seed=r.seed) for r in reqs[:4]]
     loop.run(warm, realtime=False, progress=True, max_wall_s=60)
     log(f"[bench] config5: saturation run over {n_requests} requests, {total_ops:.3g} ops")

@@ -383,3 +383,63 @@ def test_draws_at_the_rounding_boundary(seed):
         assert np.array_equal(s.sample_uniforms(f, U), want)
         s.set_term_split(4)
         assert np.array_equal(s.sample_uniforms(f, U), want)
+
+
+def _kokkos_uniforms_at(seed, shots, n_out):
+    """The per-shot xorshift64 draws (exact.kokkos_uniforms) of the given
+    shot indices rather than of shots 0..B-1."""
+    m64 = (1 << 64) - 1
+    out = np.zeros((len(shots), n_out))
+    for j, s in enumerate(shots):
+        r = exact.kokkos_seed(seed, int(s))
+        for i in range(n_out):
+            r ^= (r << 13) & m64
+            r ^= r >> 7
+            r ^= (r << 17) & m64
+            out[j, i] = (r >> 11) * (1.0 / 9007199254740992.0)
+    return out
+
+
+@pytest.mark.parametrize("name", ["config2", "config3"])
+def test_full_size_runs(name):
+    # the benchmark programs at their full 1M shots: every launch
+    # decomposition (term split, shot ranges of any alignment, packed rows)
+    # gives the same rows, and shots drawn from anywhere in the run equal
+    # the oracle's under the same seeded stream
+    import torch
+    from ksim_amd import _lib
+    from ksim_amd.sampler import unpack_outputs
+    prog = getattr(synth, f"{name}_program")()
+    s = HipProgramSampler(prog)
+    B, F, seed = 1_000_000, s.min_num_f, 4242
+    f_np = synth.random_f(B, F, 1e-3, 11)
+    f = torch.from_numpy(f_np).cuda()
+    lib = _lib.load()
+
+    def run(L, cuts=(0, B)):
+        s.set_term_split(L)
+        out = torch.full((B, s.num_outputs), 7, dtype=torch.uint8, device="cuda")
+        for lo, hi in zip(cuts[:-1], cuts[1:]):
+            _lib.check(lib.ksim_program_sample_device(s.handle, f[lo:hi].data_ptr(), hi - lo, F,
+                                                      lo, seed, None, out[lo:hi].data_ptr(),
+                                                      None))
+        s.check()
+        return out
+
+    ref = run(0)
+    for L in (1, 4):
+        assert torch.equal(run(L), ref), L
+    assert torch.equal(run(0, (0, 1, 333_333, 700_001, B)), ref)
+    s.set_term_split(0)
+    packed = torch.zeros((B, s.out_words), dtype=torch.int32, device="cuda")
+    _lib.check(lib.ksim_program_sample_device_packed(s.handle, f.data_ptr(), B, F, 0, seed, None,
+                                                     packed.data_ptr(), None))
+    s.check()
+    ref_np = ref.cpu().numpy()
+    assert np.array_equal(unpack_outputs(packed.cpu().numpy().view(np.uint32), s.num_outputs),
+                          ref_np)
+    idx = np.sort(np.random.default_rng(3).choice(B, 12, replace=False))
+    cols = [_kokkos_uniforms_at(seed + 0x9E37 * k, idx, len(c.graphs) - 1)
+            for k, c in enumerate(prog.components)]
+    want = exact.sample_program(prog, f_np[idx], np.concatenate(cols, axis=1))
+    assert np.array_equal(ref_np[idx], want)

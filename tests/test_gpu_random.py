@@ -1,0 +1,62 @@
+"""GPU: seeded random programs around the 32-bit word boundaries of the
+parameter vector (F and output counts of 31-33, 63-65, ...; one to three
+components; components without outputs), through every plain entry point
+at term splits 1, 2 and 4, and batched in groups of eight, against the
+oracle and against each other."""
+import numpy as np
+import pytest
+
+from ksim_amd import synth
+from ksim_amd.sampler import HipProgramSampler, unpack_outputs
+from oracle import exact
+
+pytestmark = pytest.mark.gpu
+
+F_CHOICES = [0, 1, 5, 30, 31, 32, 33, 62, 63, 64, 65, 95, 96, 97, 120]
+OUT_CHOICES = [1, 2, 7, 30, 31, 32, 33, 40, 64]
+
+
+def random_programs(n, seed):
+    rng = np.random.default_rng(seed)
+    for _ in range(n):
+        nc = int(rng.integers(1, 4))
+        F = int(rng.choice(F_CHOICES))
+        n_out = max(int(rng.choice(OUT_CHOICES)), nc)
+        t = int(rng.integers(0, 11))
+        prog = synth.shape_program(n_q=int(rng.integers(4, 10)), t=t, F=F, n_out=n_out,
+                                   seed=int(rng.integers(1 << 30)), n_components=nc)
+        B = int(rng.integers(1, 200))
+        yield prog, B, float(rng.choice([1e-3, 0.3, 0.5])), rng
+
+
+@pytest.mark.parametrize("seed", [11, 12])
+def test_random_programs(seed):
+    import torch
+    from ksim_amd.batch import BatchedSampler
+    pool = []
+    for prog, B, p_f, rng in random_programs(24, seed):
+        s = HipProgramSampler(prog)
+        f = synth.random_f(B, max(s.min_num_f, 1), p_f, B)
+        U = rng.random((B, s.tot_out))
+        nb = min(B, 16)
+        want = exact.sample_program(prog, f[:nb], U[:nb])
+        got = {}
+        for L in (1, 2, 4):
+            s.set_term_split(L)
+            got[L] = (s.sample_uniforms(f, U), s.sample(f, 7), s.sample_packed(f, 7))
+        assert np.array_equal(got[1][0][:nb], want)
+        for L in (2, 4):
+            for j in range(3):
+                assert np.array_equal(got[L][j], got[1][j]), (L, j)
+        assert np.array_equal(unpack_outputs(got[1][2], s.num_outputs), got[1][1])
+        pool.append((s, f, got[1][1]))
+        if len(pool) == 8:
+            tile_ops = float(rng.choice([1.0, 0.0, 1e300]))
+            b = BatchedSampler([q[0] for q in pool], [torch.from_numpy(q[1]).cuda() for q in pool],
+                               [7] * 8, wave_shots=64, tile_ops=tile_ops)
+            b.run()
+            torch.cuda.synchronize()
+            b.check()
+            for j, q in enumerate(pool):
+                assert np.array_equal(b.out[j].cpu().numpy()[:, :q[0].num_outputs], q[2]), j
+            pool = []

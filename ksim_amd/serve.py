@@ -32,7 +32,7 @@ from collections import OrderedDict, deque
 import numpy as np
 
 from . import _lib
-from .sampler import HipProgramSampler, program_blob
+from .sampler import HipProgramSampler, blob_stats, program_blob
 
 
 @dataclasses.dataclass
@@ -142,6 +142,17 @@ class ServingLoop:
             r.key = self.content_key(bytes(r.prog))
         return r.key
 
+    def _footprint(self, r) -> int:
+        """Device bytes the program of request ``r`` takes once resident
+        (its packed words), known before any upload: read from the blob
+        header, from an existing sampler, or by packing a FlatProgram into a
+        blob on the host here (the upload then starts from that blob)."""
+        if isinstance(r.prog, HipProgramSampler):
+            return int(self.lib.ksim_program_device_bytes(r.prog.handle))
+        if not isinstance(r.prog, (bytes, bytearray)):
+            r.prog = program_blob(r.prog)
+        return 4 * int(blob_stats(bytes(r.prog))["words"])
+
     def _program(self, key, prog):
         ent = self.programs.get(key)
         if ent is not None:
@@ -228,11 +239,14 @@ class ServingLoop:
                     ent = self.programs.get(self._resolve(r))
                     need = r.shots * (r.f.shape[1] + 64)
                     if ent is None:
-                        need += 0  # program bytes are only known after packing
-                    used = self.program_bytes + sum(a.bytes for a in active)
+                        need += self._footprint(r)  # a program not resident yet
+                    # requests whose output is still being copied back hold
+                    # their buffers and program too
+                    used = (self.program_bytes + sum(a.bytes for a in active)
+                            + sum(c[1].bytes for c in copies))
                     if used + need > self.hbm_budget:
                         used = self._evict_for(need, used)
-                        if used + need > self.hbm_budget and active:
+                        if used + need > self.hbm_budget and (active or copies):
                             break  # wait for memory to free up
                     ent = self._program(r.key, r.prog)
                     ent[2] += 1

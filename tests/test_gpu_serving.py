@@ -76,3 +76,25 @@ def test_content_keyed_residency():
         ref = sa if r.key == loop.content_key(program_blob(prog_a)) else sb
         assert np.array_equal(r.out, ref.sample(r.f, r.seed))
     loop.close()
+
+
+def test_admission_counts_program_bytes():
+    # a budget that holds one of two programs (with its f and output rows)
+    # at a time: the second request is admitted only once the first has
+    # finished and its idle program has been evicted
+    from ksim_amd.sampler import program_blob, blob_stats
+    prog_a = synth.shape_program(n_q=8, t=10, F=6, n_out=12, seed=21)
+    prog_b = synth.shape_program(n_q=8, t=10, F=6, n_out=12, seed=22)
+    words = [4 * int(blob_stats(program_blob(p))["words"]) for p in (prog_a, prog_b)]
+    shots = 500
+    rows = shots * (6 + 64)
+    budget = max(words) + rows + max(words) // 2
+    reqs = [Request(key="a", prog=prog_a, f=synth.random_f(shots, 6, 0.3, 1), seed=5),
+            Request(key="b", prog=prog_b, f=synth.random_f(shots, 6, 0.3, 2), seed=6)]
+    loop = ServingLoop(max_slots=4, step_ms=0.2, hbm_budget=budget)
+    loop.run(reqs, realtime=False)
+    assert reqs[1].admitted >= reqs[0].done
+    assert list(loop.programs) == ["b"]
+    for r, p in zip(reqs, (prog_a, prog_b)):
+        assert np.array_equal(r.out, HipProgramSampler(p).sample(r.f, r.seed))
+    loop.close()

@@ -48,6 +48,7 @@ class Request:
     seed: int
     arrival: float = 0.0
     # filled by the loop
+    footprint: int | None = None   # device bytes of its program, once known
     out: np.ndarray | None = None
     admitted: float | None = None
     done: float | None = None
@@ -147,11 +148,14 @@ class ServingLoop:
         (its packed words), known before any upload: read from the blob
         header, from an existing sampler, or by packing a FlatProgram into a
         blob on the host here (the upload then starts from that blob)."""
-        if isinstance(r.prog, HipProgramSampler):
-            return int(self.lib.ksim_program_device_bytes(r.prog.handle))
-        if not isinstance(r.prog, (bytes, bytearray)):
-            r.prog = program_blob(r.prog)
-        return 4 * int(blob_stats(bytes(r.prog))["words"])
+        if r.footprint is None:  # once per request, however long it waits
+            if isinstance(r.prog, HipProgramSampler):
+                r.footprint = int(self.lib.ksim_program_device_bytes(r.prog.handle))
+            else:
+                if not isinstance(r.prog, (bytes, bytearray)):
+                    r.prog = program_blob(r.prog)
+                r.footprint = 4 * int(blob_stats(bytes(r.prog))["words"])
+        return r.footprint
 
     def _program(self, key, prog):
         ent = self.programs.get(key)

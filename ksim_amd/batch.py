@@ -43,7 +43,13 @@ class BatchedSampler:
     def __init__(self, samplers: list[HipProgramSampler], f: list, seeds: list[int],
                  wave_shots: int = 128, blocks_per_cu: int = 6, tile_ops: float = 0.0):
         import torch
+        from .sampler import _check_seed, device_u8_2d
         torch.cuda.init()
+        if not len(samplers) == len(f) == len(seeds):
+            raise ValueError("samplers, f and seeds must have the same length")
+        for k, (s, fk) in enumerate(zip(samplers, f)):
+            device_u8_2d(fk, f"f[{k}]")
+            _check_seed(seeds[k], s.n_comp)
         self.torch = torch
         self.samplers = samplers
         self.f = f

@@ -103,6 +103,8 @@ class NoiseModel:
     def sample_device(self, d_out, shot_base: int = 0, stream=None):
         """Fill a resident (B, f_cols) uint8 CUDA tensor with f rows."""
         import torch
+        from .sampler import device_u8_2d
+        device_u8_2d(d_out, "d_out")
         st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
         _lib.check(_lib.load().ksim_noise_sample_device(
             self._h, int(d_out.shape[0]), int(shot_base), self.seed,

@@ -43,6 +43,20 @@ def _as_u8_2d(a, name: str) -> np.ndarray:
     return a
 
 
+def device_u8_2d(t, name: str, cols: tuple | None = None):
+    """Check a caller's device buffer before its address goes to a kernel:
+    a 2-D, contiguous uint8 CUDA tensor (TypeError otherwise), with a column
+    count among ``cols`` when given (ValueError otherwise): kernels take
+    the row stride from the program, not from the tensor."""
+    import torch
+    if not isinstance(t, torch.Tensor) or t.dtype != torch.uint8 or t.dim() != 2 \
+            or not t.is_cuda or not t.is_contiguous():
+        raise TypeError(f"{name}: expected a contiguous 2-D uint8 CUDA tensor")
+    if cols is not None and int(t.shape[1]) not in cols:
+        raise ValueError(f"{name}: {int(t.shape[1])} columns, expected one of {sorted(set(cols))}")
+    return t
+
+
 def _check_seed(seed: int, n_comp: int) -> int:
     if isinstance(seed, (bool, np.bool_)) or not isinstance(seed, (int, np.integer)):
         raise TypeError("seed must be an int")
@@ -372,6 +386,8 @@ class HipProgramSampler:
                             shot_base: int = 0, stream=None):
         """Resident variant: d_out a (B, num_outputs) uint8 CUDA tensor."""
         import torch
+        device_u8_2d(d_out, "d_out", (self.num_outputs, max(self.num_outputs, 1)))
+        seed = _check_seed(seed, self.n_comp)
         st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
         ns = noise.seed if noise_seed is None else int(noise_seed)
         _lib.check(_lib.load().ksim_program_sample_noisy_device(

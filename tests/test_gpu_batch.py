@@ -129,3 +129,28 @@ def test_batch_with_128bit_graphs_equals_alone(tile_ops):
         torch.cuda.synchronize()
         b.check()
         check_same(b, samplers, alone)
+
+
+def test_rejects_bad_device_buffers():
+    # caller tensors are checked before their addresses reach a kernel
+    import torch
+    s = HipProgramSampler(synth.shape_program(n_q=5, t=2, F=3, n_out=4, seed=1))
+    f = torch.zeros((10, 3), dtype=torch.uint8, device="cuda")
+    with pytest.raises(TypeError):
+        BatchedSampler([s], [f.to(torch.int32)], [1])
+    with pytest.raises(TypeError):
+        BatchedSampler([s], [torch.zeros((3, 10), dtype=torch.uint8, device="cuda").t()], [1])
+    with pytest.raises(TypeError):
+        BatchedSampler([s], [f.cpu()], [1])
+    with pytest.raises(ValueError):
+        BatchedSampler([s], [f], [1, 2])
+    from ksim_amd.noise import NoiseModel
+    from tests.test_gpu_noise import model_b
+    probs, T = model_b(num_f=3, n_ch=4, p=0.1, seed=1)
+    nm = NoiseModel(probs, T, seed=2)
+    with pytest.raises(ValueError):
+        s.sample_noisy_device(nm, torch.zeros((10, 7), dtype=torch.uint8, device="cuda"), 5)
+    out = torch.zeros((10, 4), dtype=torch.uint8, device="cuda")
+    s.sample_noisy_device(nm, out, 5)
+    s.check()
+    assert np.array_equal(out.cpu().numpy(), s.sample_noisy(nm, 10, 5))

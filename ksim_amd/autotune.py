@@ -18,8 +18,8 @@ The persistent kernel has three launch-shape knobs:
 All depend on the mix (term counts, shot counts, parameter words), so the
 tuner times every candidate on the actual batch (one untimed warm-up and the
 median of ``reps`` hipGraph replays each) and keeps the fastest.  Results are
-cached on disk per (device, batch signature); ``KSIM_TUNE_CACHE`` names the
-cache file (default ``~/.cache/ksim_amd/batch_tune.json``).
+cached on disk per (device, kernel build, batch signature); ``KSIM_TUNE_CACHE``
+names the cache file (default ``~/.cache/ksim_amd/batch_tune.json``).
 """
 from __future__ import annotations
 
@@ -49,6 +49,17 @@ def batch_signature(samplers, fs) -> str:
         h.update(f"{st['words']}:{st['terms']}:{st['ops_per_shot']}:"
                  f"{int(f.shape[0])}:{int(f.shape[1])};".encode())
     return h.hexdigest()[:32]
+
+
+def library_key() -> str:
+    """Hash of the built kernel library: tunings are kept per build, since
+    a kernel change can move the best launch shape."""
+    from . import _lib
+    h = hashlib.sha256()
+    with open(_lib.LIB_PATH, "rb") as fh:
+        for block in iter(lambda: fh.read(1 << 20), b""):
+            h.update(block)
+    return h.hexdigest()[:16]
 
 
 def _load(path):
@@ -104,7 +115,7 @@ def tune_batch(samplers, fs, seeds, reps: int = 3, use_cache: bool = True,
     "trials", "cached"}``."""
     from .batch import BatchedSampler, device_info
     info = device_info()
-    dev_key = f"cus{info['cus']}-clk{info['clock_khz']}"
+    dev_key = f"cus{info['cus']}-clk{info['clock_khz']}-lib{library_key()}"
     key = f"{dev_key}/{batch_signature(samplers, fs)}"
     path = cache_path()
     table = _load(path) if use_cache else {}

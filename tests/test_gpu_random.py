@@ -1,8 +1,9 @@
 """GPU: seeded random programs around the 32-bit word boundaries of the
 parameter vector (F and output counts of 31-33, 63-65, ...; one to three
 components; components without outputs), through every plain entry point
-at term splits 1, 2 and 4, and batched in groups of eight, against the
-oracle and against each other."""
+at term splits 1, 2 and 4, with f bits drawn on the device from random
+noise models, and batched in groups of eight, against the oracle and
+against each other."""
 import numpy as np
 import pytest
 
@@ -29,6 +30,21 @@ def random_programs(n, seed):
         yield prog, B, float(rng.choice([1e-3, 0.3, 0.5])), rng
 
 
+def random_noise(num_f, rng):
+    """A noise model of 1- and 2-bit channels on random f columns."""
+    from ksim_amd.noise import NoiseModel
+    probs, cols = [], []
+    for _ in range(int(rng.integers(1, 60))):
+        k = int(rng.integers(1, 3))
+        p = float(rng.choice([1e-3, 0.02, 0.1]))
+        probs.append(np.array([1 - (2 ** k - 1) * p] + [p] * (2 ** k - 1)))
+        for _ in range(k):
+            col = np.zeros(num_f, np.uint8)
+            col[rng.choice(num_f, size=int(rng.integers(1, min(4, num_f) + 1)), replace=False)] = 1
+            cols.append(col)
+    return NoiseModel(probs, np.stack(cols, axis=1), seed=int(rng.integers(1 << 30)))
+
+
 @pytest.mark.parametrize("seed", [11, 12])
 def test_random_programs(seed):
     import torch
@@ -49,6 +65,16 @@ def test_random_programs(seed):
             for j in range(3):
                 assert np.array_equal(got[L][j], got[1][j]), (L, j)
         assert np.array_equal(unpack_outputs(got[1][2], s.num_outputs), got[1][1])
+        if s.min_num_f > 0 and rng.random() < 0.5:
+            # f bits drawn in the kernel from a random noise model equal
+            # sampling the rows the same model draws
+            nm = random_noise(s.min_num_f + int(rng.integers(0, 3)), rng)
+            Bn = int(rng.integers(1, 600))
+            rows = nm.sample(Bn, shot_base=0)
+            for L in (1, 4):
+                s.set_term_split(L)
+                assert np.array_equal(s.sample_noisy(nm, Bn, 9), s.sample(rows, 9)), L
+            s.set_term_split(1)
         pool.append((s, f, got[1][1]))
         if len(pool) == 8:
             tile_ops = float(rng.choice([1.0, 0.0, 1e300]))

@@ -315,7 +315,8 @@ int ksim_sched_create(int32_t max_slots, int32_t max_items, ksim_sched **out);
 /* Bind slot `slot` to a resident program with its resident f matrix
  * d_f (B, f_cols) u8, output d_out (B, num_outputs) u8 and seed; a request's
  * shots use the same per-shot streams as ksim_program_sample_device with
- * shot_base 0, however they are split into items. */
+ * shot_base 0, however they are split into items.  B < 2^31 (items carry
+ * 32-bit shot offsets). */
 int ksim_sched_set_slot(ksim_sched *q, int32_t slot, ksim_program *p,
                         const uint8_t *d_f, int32_t f_cols, int64_t B,
                         uint64_t seed, uint8_t *d_out);

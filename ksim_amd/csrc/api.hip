@@ -1257,6 +1257,8 @@ int ksim_sched_set_slot(ksim_sched *q, int32_t slot, ksim_program *p, const uint
     if (!q || !p) throw std::invalid_argument("NULL argument");
     if (slot < 0 || slot >= q->max_slots) throw std::invalid_argument("slot out of range");
     if (B < 0) throw std::invalid_argument("B < 0");
+    // work items address a request's shots with 32-bit offsets
+    if (B > INT32_MAX) throw std::invalid_argument("more than 2^31 - 1 shots in one request");
     if (f_cols < p->num_f) throw std::invalid_argument("f matrix has fewer columns than f_selection needs");
     if (B > 0 && ((!d_f && f_cols > 0) || !d_out)) throw std::invalid_argument("NULL buffer");
     q->slots[slot] = BatchProg{p->d_prog, d_f, d_out, seed, B, f_cols, p->num_outputs,

@@ -98,3 +98,22 @@ def test_admission_counts_program_bytes():
     for r, p in zip(reqs, (prog_a, prog_b)):
         assert np.array_equal(r.out, HipProgramSampler(p).sample(r.f, r.seed))
     loop.close()
+
+
+def test_sched_rejects_requests_beyond_32bit_offsets():
+    import ctypes as C
+    import torch
+    from ksim_amd import _lib
+    s = HipProgramSampler(synth.shape_program(n_q=5, t=2, F=3, n_out=4, seed=1))
+    lib = _lib.load()
+    h = C.c_void_p()
+    _lib.check(lib.ksim_sched_create(2, 16, C.byref(h)))
+    f = torch.zeros((4, 3), dtype=torch.uint8, device="cuda")
+    out = torch.zeros((4, 4), dtype=torch.uint8, device="cuda")
+    try:
+        with pytest.raises(ValueError, match="2\\^31"):
+            _lib.check(lib.ksim_sched_set_slot(h, 0, s.handle, f.data_ptr(), 3, 1 << 31, 1,
+                                               out.data_ptr()))
+        _lib.check(lib.ksim_sched_set_slot(h, 0, s.handle, f.data_ptr(), 3, 4, 1, out.data_ptr()))
+    finally:
+        lib.ksim_sched_destroy(h)

@@ -974,11 +974,26 @@ __device__ __forceinline__ void run_shot(const ShotArgs &a, int comp, int64_t s,
                    }
                  }
                });
-  } else if (fs.st != nullptr && fsel0 != 0xffffffffu && a.num_f <= 32 * WL &&
-             ((uintptr_t)a.f & 15u) == 0) {
+  } else if (fs.st != nullptr && a.num_f <= 32 * WL && ((uintptr_t)a.f & 15u) == 0) {
     // the wave's rows through its LDS bit string (one read of each line)
     const int lead = stage_f_rows(a, fs);
-    staged_words<WL>(fs.st, lead + (int)(s - fs.r0) * a.num_f + (int)fsel0, F, pv);
+    const int row = lead + (int)(s - fs.r0) * a.num_f;
+    if (fsel0 != 0xffffffffu) {
+      staged_words<WL>(fs.st, row + (int)fsel0, F, pv);
+    } else {
+      // any column order: each selected bit from the string (one LDS read),
+      // a word at a time so every word index is known at compile time
+#pragma unroll
+      for (int w = 0; w < WL; ++w) {
+        uint32_t word = 0;
+        const int jn = min(32, F - 32 * w);
+        for (int jj = 0; jj < jn; ++jj) {
+          const int b = row + (int)fsel[32 * w + jj];
+          word |= ((fs.st[b >> 5] >> (b & 31)) & 1u) << jj;
+        }
+        pv[w] = word;
+      }
+    }
   } else if (fsel0 != 0xffffffffu &&
              (((uint32_t)a.num_f | fsel0 | (uint32_t)(uintptr_t)a.f) & 3u) == 0) {
     // contiguous selection on a dword-aligned row: 16 f bytes per load,

@@ -109,7 +109,10 @@ def test_batch_with_128bit_graphs_equals_alone(tile_ops):
     # a program whose graphs need 128-bit sums in the batch selects the
     # kernel with that path for the whole batch; results stay the same
     import torch
-    progs = [synth.wide_exponent_program(seed=5), synth.wide_exponent_program(n_out=2, seed=7),
+    # (the second reads its f columns in a shuffled order)
+    from tests.test_gpu_parity import permute_selections
+    progs = [synth.wide_exponent_program(seed=5),
+             permute_selections(synth.wide_exponent_program(n_out=2, F=40, seed=7), 8),
              synth.shape_program(n_q=8, t=10, F=6, n_out=7, seed=12)]
     samplers, fs, seeds, alone = [], [], [], []
     for k, prog in enumerate(progs):

@@ -283,17 +283,29 @@ def test_plain_term_split_is_identical(n_components):
     assert np.array_equal(got[4][2][:300], exact.sample_program(prog, f[:300], U[:300]))
 
 
-@pytest.mark.parametrize("n_components,extra_cols,n_out", [(1, 0, 6), (2, 3, 40), (2, 300, 40)])
-def test_f_row_reads_agree(n_components, extra_cols, n_out):
+def permute_selections(prog, seed):
+    """The same program reading its f columns in a shuffled order."""
+    rng = np.random.default_rng(seed)
+    for c in prog.components:
+        c.f_selection = rng.permutation(np.asarray(c.f_selection))
+    return prog
+
+
+@pytest.mark.parametrize("n_components,extra_cols,n_out,permuted",
+                         [(1, 0, 6, False), (2, 3, 40, False), (2, 300, 40, False),
+                          (1, 0, 6, True), (2, 3, 40, True), (2, 300, 40, True)])
+def test_f_row_reads_agree(n_components, extra_cols, n_out, permuted):
     # a 16-byte aligned f with rows of at most 32 * WL bytes is read by the
     # whole wave through LDS; any other f lane by lane: the same bits, on
     # row widths that are not multiples of 4 or 16 (extra_cols), a second
     # component whose f bits start mid-row, shot counts that end inside a
-    # tile and inside a 16-byte piece, and (300 extra columns) rows too
-    # long to stage
+    # tile and inside a 16-byte piece, (300 extra columns) rows too long to
+    # stage, and f columns selected in a shuffled order
     import torch
     from ksim_amd import _lib
     prog = synth.shape_program(n_q=8, t=8, F=37, n_out=n_out, seed=51, n_components=n_components)
+    if permuted:
+        prog = permute_selections(prog, 52)
     s = HipProgramSampler(prog)
     num_f = s.min_num_f + extra_cols
     lib = _lib.load()

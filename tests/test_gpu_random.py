@@ -1,6 +1,7 @@
 """GPU: seeded random programs around the 32-bit word boundaries of the
 parameter vector (F and output counts of 31-33, 63-65, ...; one to three
-components; components without outputs), through every plain entry point
+components; components without outputs; f columns selected in a shuffled
+order), through every plain entry point
 at term splits 1, 2 and 4, with f bits drawn on the device from random
 noise models, and batched in groups of eight, against the oracle and
 against each other."""
@@ -26,6 +27,9 @@ def random_programs(n, seed):
         t = int(rng.integers(0, 11))
         prog = synth.shape_program(n_q=int(rng.integers(4, 10)), t=t, F=F, n_out=n_out,
                                    seed=int(rng.integers(1 << 30)), n_components=nc)
+        if rng.random() < 0.3:
+            for c in prog.components:  # f columns read in a shuffled order
+                c.f_selection = rng.permutation(np.asarray(c.f_selection))
         B = int(rng.integers(1, 200))
         yield prog, B, float(rng.choice([1e-3, 0.3, 0.5])), rng
 

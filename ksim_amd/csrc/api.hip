@@ -483,35 +483,97 @@ void check_err_sync(ksim_program *p, hipStream_t st) {
 // most blocks of a term-split plain launch (they loop over tile groups)
 constexpr int kSplitBlocks = 2048;
 
+// Resident 256-thread blocks per CU, i.e. waves per SIMD, of the plain
+// kernel (split false) or the split kernel for lane words wl, 64-bit or
+// 128-bit sums and with or without in-kernel noise: the runtime's occupancy
+// figure for the compiled kernel, asked once per process.
+int resident_waves(int wl, bool narrow, bool nz, bool split) {
+  static const std::array<int, 32> occ = [] {
+    std::array<int, 32> o{};
+    auto q = [](const void *k) {
+      int nb = 0;
+      hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, kBlock, 0), "occupancy");
+      return std::max(nb, 1);
+    };
+#define KSIM_OCC(I, W, N, WD)                                                              \
+  o[(I * 2 + !(WD)) * 4 + (N) * 2 + 0] = q(reinterpret_cast<const void *>(&k_sample<W, N, WD>)); \
+  o[(I * 2 + !(WD)) * 4 + (N) * 2 + 1] =                                                  \
+      q(reinterpret_cast<const void *>(&k_sample_split<W, N, WD>));
+#define KSIM_OCC_W(I, W) \
+  KSIM_OCC(I, W, false, false) KSIM_OCC(I, W, true, false) KSIM_OCC(I, W, false, true) KSIM_OCC(I, W, true, true)
+    KSIM_OCC_W(0, 1) KSIM_OCC_W(1, 2) KSIM_OCC_W(2, 4) KSIM_OCC_W(3, 8)
+#undef KSIM_OCC_W
+#undef KSIM_OCC
+    return o;
+  }();
+  const int i = wl <= 1 ? 0 : wl <= 2 ? 1 : wl <= 4 ? 2 : 3;
+  return occ[(i * 2 + (narrow ? 1 : 0)) * 4 + (nz ? 2 : 0) + (split ? 1 : 0)];
+}
+
+// A SIMD whose last round holds fewer waves than this is latency bound: on
+// 30-qubit programs (t = 24, 32) a round of 1-3 resident waves took about
+// as long as 3.5 waves' worth of issue at full occupancy.
+constexpr double kLatencyWaves = 3.5;
+
 // Waves per 64-shot tile for a plain launch of B shots: p->term_split when
-// set; else 1 when the tiles fill the resident wave slots, otherwise the
-// smallest L in {2, 4} that fills them among the splits whose estimated
-// useful share of the waves' work is at least 3/4 (every component), or
-// the largest such L (measured on 30-qubit programs at 100k shots: t = 8
-// is fastest at L = 2, t = 12..32 at L = 4).
-int plain_split(const ksim_program *p, int64_t B) {
+// set.  Otherwise the fill rule: 1 when the tiles fill the resident wave
+// slots, else the smallest L in {2, 4} that fills them among the splits
+// whose estimated useful share of the waves' work is at least 3/4 (every
+// component), or the largest such L (measured on 30-qubit programs at 100k
+// shots: t = 8 is fastest at L = 2, t = 12..32 at L = 4).  The fill rule
+// ignores how the waves fall into rounds of resident slots: L = 2 on
+// 2600 tiles is two full rounds where L = 4 is three rounds of half the
+// work each.  So every allowed L is also priced as (waves per SIMD in
+// whole rounds + the last round, at least kLatencyWaves) x per-wave work,
+// and one that is at least 20 % cheaper than the fill rule's choice is
+// taken instead (the margin covers the exchange cost the work estimate
+// leaves out; measured 15-27 % faster launches between 2600 and 6000
+// tiles, none slower).
+int plain_split(const ksim_program *p, int64_t B, bool nz) {
   if (p->term_split > 0) return p->term_split;
-  static int slots = 0;
-  if (!slots) {
+  static const int simds = [] {
     int dev = 0, cus = 256;
     hip_check(hipGetDevice(&dev), "hipGetDevice");
     hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "CU count");
-    slots = cus * 4 * KSIM_WAVES_PER_EU;
-  }
+    return cus * 4;
+  }();
+  const int slots = simds * KSIM_WAVES_PER_EU;
   const int64_t tiles = (B + 63) / 64;
-  if (tiles >= slots) return 1;
-  int best = 1;
-  const int Ls[2] = {2, 4};
-  for (int k = 0; k < 2; ++k) {
-    double eff = 1.0;
-    for (int c = 0; c < p->n_comp; ++c) {
-      double w[3];
-      comp_work(p, c, w);
-      if (w[0] > 0) eff = std::min(eff, w[0] / (Ls[k] * w[k + 1]));
+  const int Ls[3] = {1, 2, 4};
+  double w[3] = {0.0, 0.0, 0.0}, eff[3] = {1.0, 1.0, 1.0};
+  for (int c = 0; c < p->n_comp; ++c) {
+    double wc[3];
+    comp_work(p, c, wc);
+    for (int k = 0; k < 3; ++k) w[k] += wc[k];
+    if (wc[0] > 0)
+      for (int k = 1; k < 3; ++k) eff[k] = std::min(eff[k], wc[0] / (Ls[k] * wc[k]));
+  }
+  int fill = 1;
+  if (tiles < slots) {
+    for (int k = 1; k < 3; ++k) {
+      if (eff[k] < 0.75) break;
+      fill = Ls[k];
+      if (Ls[k] * tiles >= slots) break;
     }
-    if (eff < 0.75) break;
-    best = Ls[k];
-    if (Ls[k] * tiles >= slots) break;
+  }
+  if (w[0] <= 0) return fill;
+  auto cost = [&](int k) {
+    const int o = resident_waves(p->wl, p->all_narrow, nz, Ls[k] > 1);
+    const int64_t m = (Ls[k] * tiles * p->n_comp + simds - 1) / simds;  // waves per SIMD
+    const int64_t full = m / o, rem = m % o;
+    return ((double)(full * o) + (rem ? std::max((double)rem, kLatencyWaves) : 0.0)) * w[k];
+  };
+  const int kf = fill == 1 ? 0 : fill == 2 ? 1 : 2;
+  const double cf = cost(kf);
+  int best = fill;
+  double cb = cf;
+  for (int k = 0; k < 3; ++k) {
+    if (k > 0 && eff[k] < 0.75) break;
+    const double ck = cost(k);
+    if (ck < 0.8 * cf && ck < cb) {
+      best = Ls[k];
+      cb = ck;
+    }
   }
   return best;
 }
@@ -540,7 +602,7 @@ void launch_sample(ksim_program *p, const uint8_t *d_f, int64_t B, int32_t f_col
   const int64_t nb = (B + kBlock - 1) / kBlock;
   if (nb > 0x7fffffff) throw std::invalid_argument("too many shots for one launch");
   const bool nz = a.noise.C > 0;
-  const int L = plain_split(p, B);
+  const int L = plain_split(p, B, nz);
   if (L > 1) {
     // blocks: one per group of 4/L tiles, at most kSplitBlocks; blocks
     // loop over the groups

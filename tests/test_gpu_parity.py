@@ -283,6 +283,30 @@ def test_plain_term_split_is_identical(n_components):
     assert np.array_equal(got[4][2][:300], exact.sample_program(prog, f[:300], U[:300]))
 
 
+def test_automatic_split_past_the_resident_slots():
+    # shot counts whose tiles fill the resident wave slots 1.02x and 1.17x
+    # over, where the automatic choice weighs how the waves fall into
+    # rounds: still the bits of one wave per tile
+    import torch
+    from ksim_amd import _lib
+    lib = _lib.load()
+    prog = synth.shape_program(n_q=30, t=24, F=48, n_out=30, seed=924)
+    s = HipProgramSampler(prog)
+    for B in (64 * 5200 - 17, 64 * 6000):
+        f = torch.from_numpy(synth.random_f(B, 48, 1e-3, 24)).cuda()
+        rows = {}
+        for L in (1, 4, 0):
+            s.set_term_split(L)
+            out = torch.full((B, s.num_outputs), 7, dtype=torch.uint8, device="cuda")
+            _lib.check(lib.ksim_program_sample_device(s.handle, f.data_ptr(), B, 48, 0, 31,
+                                                      None, out.data_ptr(), None))
+            s.check()
+            rows[L] = out.cpu().numpy()
+        s.set_term_split(0)
+        assert np.array_equal(rows[0], rows[1]), B
+        assert np.array_equal(rows[4], rows[1]), B
+
+
 def permute_selections(prog, seed):
     """The same program reading its f columns in a shuffled order."""
     rng = np.random.default_rng(seed)

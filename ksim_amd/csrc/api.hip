@@ -149,6 +149,7 @@ template <int WL, bool NOISE = false, bool WIDE = true>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WIDE ? KSIM_WAVES_PER_EU : KSIM_NARROW_WAVES_PER_EU))) void k_sample(ShotArgs a, Tables tb) {
   __shared__ LdsTables lt;
   __shared__ uint32_t stage[4][NOISE ? 1 : stage_words(WL)];
+  prefetch_program(a.prog, a.prefetch_dwords);
   load_tables(&lt, tb);
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -170,6 +171,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WIDE ? K
     ShotArgs a, Tables tb, int L) {
   __shared__ LdsTables lt;
   __shared__ int4 xch[XCH_INT4];
+  prefetch_program(a.prog, a.prefetch_dwords);
   load_tables(&lt, tb);
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -336,7 +338,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WIDE ? K
     ShotArgs a;
     a.prog = bp.prog; a.f = bp.f; a.U = nullptr; a.out = bp.out; a.err = err;
     a.B = bp.B; a.shot_base = 0; a.seed = bp.seed;
-    a.num_f = bp.num_f; a.num_outputs = bp.num_outputs; a.tot_out = bp.tot_out; a.pad = 0;
+    a.num_f = bp.num_f; a.num_outputs = bp.num_outputs; a.tot_out = bp.tot_out; a.prefetch_dwords = 0;
     a.out_words = 0; a.out_atomic = 0;
     a.noise = NoiseDev{};
     const int L = it.split, G = 4 / L;
@@ -480,6 +482,10 @@ void check_err_sync(ksim_program *p, hipStream_t st) {
   if (e) throw RangeError(err_text(e));
 }
 
+// program dwords a plain launch pulls into L2 before its waves start (a
+// quarter of an XCD's 4 MiB L2)
+constexpr size_t kPrefetchDwords = 256 * 1024;
+
 // most blocks of a term-split plain launch (they loop over tile groups)
 constexpr int kSplitBlocks = 2048;
 
@@ -588,7 +594,8 @@ void launch_sample(ksim_program *p, const uint8_t *d_f, int64_t B, int32_t f_col
   ShotArgs a;
   a.prog = p->d_prog; a.f = d_f; a.U = d_U; a.out = d_out; a.err = p->d_err;
   a.B = B; a.shot_base = shot_base; a.seed = seed;
-  a.num_f = f_cols; a.num_outputs = p->num_outputs; a.tot_out = p->tot_out; a.pad = 0;
+  a.num_f = f_cols; a.num_outputs = p->num_outputs; a.tot_out = p->tot_out;
+  a.prefetch_dwords = (uint32_t)std::min<size_t>(p->n_words, kPrefetchDwords);
   a.out_words = out_words;
   a.out_atomic = p->n_comp > 1 ? 1 : 0;
   a.noise = noise ? *noise : NoiseDev{};

@@ -744,9 +744,29 @@ struct ShotArgs {
   int32_t num_f, num_outputs, tot_out;
   int32_t out_words;  // 0: one byte per output; > 0: packed rows of this many words
   int32_t out_atomic; // packed rows shared by several components: OR words in
-  int32_t pad;
+  uint32_t prefetch_dwords;  // leading program dwords to pull into L2 first (0: none)
   NoiseDev noise;     // noise.C > 0: f bits drawn on the device, f unused
 };
+
+// Pull this block's share of the first `ndw` program dwords into the L2 of
+// the block's XCD: one 4-byte load per 128-byte line, spread over the
+// blocks that run on that XCD (consecutive workgroup ids go to the 8 XCDs in
+// turn).  Every wave then walks the program's records with scalar loads;
+// from a cold L2 the waves of a launch that fits in one round of resident
+// slots would otherwise walk the misses in lockstep, one HBM round trip at
+// a time.  Loads only; no result is used.
+__device__ __forceinline__ void prefetch_program(const uint32_t *prog, uint32_t ndw) {
+  if (ndw == 0) return;
+  const uint32_t nblk = gridDim.x * gridDim.y;
+  const uint32_t id = blockIdx.y * gridDim.x + blockIdx.x;
+  const uint32_t on_xcd = (nblk + 7 - (id & 7)) >> 3;  // blocks on this block's XCD
+  const uint32_t lines = (ndw + 31) >> 5;
+  const uint32_t chunk = (lines + on_xcd - 1) / on_xcd;
+  const uint32_t l0 = (id >> 3) * chunk;
+  const uint32_t l1 = min(lines, l0 + chunk);
+  const volatile uint32_t *vp = prog;
+  for (uint32_t l = l0 + threadIdx.x; l < l1; l += blockDim.x) (void)vp[(size_t)l << 5];
+}
 
 // 32 bits of the lane's parameter words starting at bit `off` (wave-uniform,
 // may be negative or run past the words: missing bits read as 0)

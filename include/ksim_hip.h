@@ -164,8 +164,9 @@ int ksim_program_sample_device_packed(ksim_program *p, const uint8_t *d_f, int64
 /* Launch shape of the plain kernel: threads per block (256) and term
  * split, the waves that share one 64-shot tile's term sums: 1 (each lane
  * evaluates every term of its own shot), 2 or 4, or 0 (default: chosen per
- * launch, splitting only launches whose tiles cannot fill the GPU's
- * resident wave slots).  The waves of a split tile meet in LDS once per
+ * launch, splitting launches whose tiles cannot fill the GPU's resident
+ * wave slots, or whose waves would leave a nearly empty last round of
+ * slots).  The waves of a split tile meet in LDS once per
  * graph of several terms.  No reference counterpart (the reference runs
  * one thread per shot). */
 int ksim_program_set_tiles(ksim_program *p, int32_t block_threads,

@@ -529,12 +529,13 @@ constexpr double kLatencyWaves = 3.5;
 // shots: t = 8 is fastest at L = 2, t = 12..32 at L = 4).  The fill rule
 // ignores how the waves fall into rounds of resident slots: L = 2 on
 // 2600 tiles is two full rounds where L = 4 is three rounds of half the
-// work each.  So every allowed L is also priced as (waves per SIMD in
-// whole rounds + the last round, at least kLatencyWaves) x per-wave work,
-// and one that is at least 20 % cheaper than the fill rule's choice is
-// taken instead (the margin covers the exchange cost the work estimate
-// leaves out; measured 15-27 % faster launches between 2600 and 6000
-// tiles, none slower).
+// work each.  So every allowed L above the fill rule's is also priced as
+// (waves per SIMD in whole rounds + the last round, at least
+// kLatencyWaves) x per-wave work, and one that is at least 20 % cheaper is
+// taken instead (measured 15-27 % faster launches between 2600 and 6000
+// tiles).  Only larger L: the work estimate overprices the exchanges of
+// graphs with few terms, and letting it pick a smaller L made t = 12 and
+// 16 at 100k shots 7-17 % slower.
 int plain_split(const ksim_program *p, int64_t B, bool nz) {
   if (p->term_split > 0) return p->term_split;
   static const int simds = [] {
@@ -573,8 +574,8 @@ int plain_split(const ksim_program *p, int64_t B, bool nz) {
   const double cf = cost(kf);
   int best = fill;
   double cb = cf;
-  for (int k = 0; k < 3; ++k) {
-    if (k > 0 && eff[k] < 0.75) break;
+  for (int k = kf + 1; k < 3; ++k) {
+    if (eff[k] < 0.75) break;
     const double ck = cost(k);
     if (ck < 0.8 * cf && ck < cb) {
       best = Ls[k];

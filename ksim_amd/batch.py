@@ -53,6 +53,7 @@ class BatchedSampler:
         self.torch = torch
         self.samplers = samplers
         self.f = f
+        self.seeds = [int(x) for x in seeds]
         self.out = [torch.zeros((int(fk.shape[0]), max(s.num_outputs, 1)), dtype=torch.uint8,
                                 device=fk.device) for s, fk in zip(samplers, f)]
         n = len(samplers)

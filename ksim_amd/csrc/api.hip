@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -522,8 +523,8 @@ int resident_waves(int wl, bool narrow, bool nz, bool split) {
 constexpr double kLatencyWaves = 3.5;
 
 // Waves per 64-shot tile for a plain launch of B shots: p->term_split when
-// set.  Otherwise the fill rule: 1 when the tiles fill the resident wave
-// slots, else the smallest L in {2, 4} that fills them among the splits
+// set.  Otherwise the fill rule: 1 when the tiles of all components fill
+// the resident wave slots, else the smallest L in {2, 4} that fills them among the splits
 // whose estimated useful share of the waves' work is at least 3/4 (every
 // component), or the largest such L (measured on 30-qubit programs at 100k
 // shots: t = 8 is fastest at L = 2, t = 12..32 at L = 4).  The fill rule
@@ -531,7 +532,8 @@ constexpr double kLatencyWaves = 3.5;
 // 2600 tiles is two full rounds where L = 4 is three rounds of half the
 // work each.  So every allowed L above the fill rule's is also priced as
 // (waves per SIMD in whole rounds + the last round, at least
-// kLatencyWaves) x per-wave work, and one that is at least 20 % cheaper is
+// kLatencyWaves) x group iterations per block x per-wave work, on the grid
+// the launch would have, and one that is at least 20 % cheaper is
 // taken instead (measured 15-27 % faster launches between 2600 and 6000
 // tiles).  Only larger L: the work estimate overprices the exchanges of
 // graphs with few terms, and letting it pick a smaller L made t = 12 and
@@ -555,20 +557,33 @@ int plain_split(const ksim_program *p, int64_t B, bool nz) {
     if (wc[0] > 0)
       for (int k = 1; k < 3; ++k) eff[k] = std::min(eff[k], wc[0] / (Ls[k] * wc[k]));
   }
+  // the grid holds tiles x n_comp wave tiles (one grid row per component)
+  const int64_t wave_tiles = tiles * p->n_comp;
   int fill = 1;
-  if (tiles < slots) {
+  if (wave_tiles < slots) {
     for (int k = 1; k < 3; ++k) {
       if (eff[k] < 0.75) break;
       fill = Ls[k];
-      if (Ls[k] * tiles >= slots) break;
+      if (Ls[k] * wave_tiles >= slots) break;
     }
   }
   if (w[0] <= 0) return fill;
+  // Price the launched grid: L = 1 runs one block per 4 tiles, L > 1 one
+  // block per group of 4/L tiles, at most kSplitBlocks blocks per component
+  // that loop over the groups.  Every block puts one wave on each SIMD of
+  // its CU for `iters` group iterations, so a CU's SIMDs each take b waves
+  // of iters x w[k] work, in rounds of o resident waves.
+  const int cus = simds / 4;
   auto cost = [&](int k) {
     const int o = resident_waves(p->wl, p->all_narrow, nz, Ls[k] > 1);
-    const int64_t m = (Ls[k] * tiles * p->n_comp + simds - 1) / simds;  // waves per SIMD
-    const int64_t full = m / o, rem = m % o;
-    return ((double)(full * o) + (rem ? std::max((double)rem, kLatencyWaves) : 0.0)) * w[k];
+    const int64_t per = 4 / Ls[k];  // tiles per block iteration
+    const int64_t groups = (tiles + per - 1) / per;
+    const int64_t blocks = Ls[k] == 1 ? groups : std::min<int64_t>(groups, kSplitBlocks);
+    const int64_t iters = (groups + blocks - 1) / blocks;
+    const int64_t b = (blocks * p->n_comp + cus - 1) / cus;  // waves per SIMD
+    const int64_t full = b / o, rem = b % o;
+    return ((double)(full * o) + (rem ? std::max((double)rem, kLatencyWaves) : 0.0)) *
+           (double)iters * w[k];
   };
   const int kf = fill == 1 ? 0 : fill == 2 ? 1 : 2;
   const double cf = cost(kf);
@@ -1128,11 +1143,7 @@ int ksim_batch_create(ksim_program *const *progs, int32_t n, const uint8_t *cons
     if (!out || (n > 0 && (!progs || !d_f || !f_cols || !B || !seed || !d_out)))
       throw std::invalid_argument("NULL argument");
     if (wave_shots < 64 || wave_shots % 64) throw std::invalid_argument("wave_shots must be a multiple of 64");
-    auto *b = new ksim_batch();
-    b->wave_shots = wave_shots;
-    std::vector<BatchProg> hp(n);
-    struct Cost { double c; BatchItem it; };
-    std::vector<Cost> items;
+    if (n < 0) throw std::invalid_argument("n < 0");
     double total_ops = 0.0;
     for (int k = 0; k < n; ++k) {
       ksim_program *p = progs[k];
@@ -1142,6 +1153,13 @@ int ksim_batch_create(ksim_program *const *progs, int32_t n, const uint8_t *cons
       if (f_cols[k] < p->num_f) throw std::invalid_argument("f matrix has fewer columns than f_selection needs");
       total_ops += p->st.ops * (double)B[k];
     }
+    // owned until handed out, so every throw below releases it
+    std::unique_ptr<ksim_batch, void (*)(ksim_batch *)> hold(new ksim_batch(), ksim_batch_destroy);
+    ksim_batch *b = hold.get();
+    b->wave_shots = wave_shots;
+    std::vector<BatchProg> hp(n);
+    struct Cost { double c; BatchItem it; };
+    std::vector<Cost> items;
     // a tile should take each wave at most 1/8 of its share of the batch
     // (waves: 5 resident blocks of 4 waves per CU), so the heaviest-first
     // order leaves a short tail
@@ -1176,20 +1194,15 @@ int ksim_batch_create(ksim_program *const *progs, int32_t n, const uint8_t *cons
     for (size_t i = 0; i < items.size(); ++i) hi[i] = items[i].it;
     if (hi.size() >= 0xffffffffull) throw std::invalid_argument("too many work items");
     b->n_items = (uint32_t)hi.size();
-    try {
-      hip_check(hipMalloc(&b->d_progs, sizeof(BatchProg) * std::max(1, n)), "hipMalloc");
-      hip_check(hipMalloc(&b->d_items, sizeof(BatchItem) * std::max<size_t>(1, hi.size())), "hipMalloc");
-      hip_check(hipMalloc(&b->d_cursor, 64), "hipMalloc");
-      if (n) hip_check(hipMemcpy(b->d_progs, hp.data(), sizeof(BatchProg) * n, hipMemcpyHostToDevice), "H2D");
-      if (!hi.empty())
-        hip_check(hipMemcpy(b->d_items, hi.data(), sizeof(BatchItem) * hi.size(), hipMemcpyHostToDevice), "H2D");
-      hip_check(hipMemset(b->d_cursor, 0, 64), "hipMemset");
-      tables_for_current_device();
-    } catch (...) {
-      ksim_batch_destroy(b);
-      throw;
-    }
-    *out = b;
+    hip_check(hipMalloc(&b->d_progs, sizeof(BatchProg) * std::max(1, n)), "hipMalloc");
+    hip_check(hipMalloc(&b->d_items, sizeof(BatchItem) * std::max<size_t>(1, hi.size())), "hipMalloc");
+    hip_check(hipMalloc(&b->d_cursor, 64), "hipMalloc");
+    if (n) hip_check(hipMemcpy(b->d_progs, hp.data(), sizeof(BatchProg) * n, hipMemcpyHostToDevice), "H2D");
+    if (!hi.empty())
+      hip_check(hipMemcpy(b->d_items, hi.data(), sizeof(BatchItem) * hi.size(), hipMemcpyHostToDevice), "H2D");
+    hip_check(hipMemset(b->d_cursor, 0, 64), "hipMemset");
+    tables_for_current_device();
+    *out = hold.release();
   });
 }
 

@@ -273,12 +273,18 @@ def program_blob(prog: FlatProgram) -> bytes:
 
 
 def blob_stats(blob: bytes) -> dict:
-    """Validate a program blob and return its stats (ValueError if rejected)."""
+    """Validate a program blob and return its stats (ValueError if rejected),
+    with the program header fields n_comp, num_f, num_outputs, tot_out and
+    lane_words (the u32 fields at bytes 96-116 of the validated header)."""
     buf = np.frombuffer(blob, dtype=np.uint8)
     st = np.zeros(8, np.float64)
     _lib.check(_lib.load().ksim_blob_check(_lib.ptr(buf, C.c_uint8) if buf.size else None,
                                            buf.size, _lib.ptr(st, C.c_double)))
-    return _stats_dict(st)
+    d = _stats_dict(st)
+    hdr = np.frombuffer(bytes(blob[96:116]), dtype="<u4")
+    d.update(n_comp=int(hdr[0]), num_f=int(hdr[1]), num_outputs=int(hdr[2]),
+             tot_out=int(hdr[3]), lane_words=int(hdr[4]))
+    return d
 
 
 def lookup_tables():
@@ -496,8 +502,21 @@ def sample_flat(prog: FlatProgram, f_params: np.ndarray, rng: np.random.Generato
 
 
 def run_sampler(flat: FlatProgram, f: np.ndarray, seed: int, backend: str = "gpu") -> np.ndarray:
-    """The GPU branch of the reference's backend switch
-    (src/python/ksim/run.py:79-91); "auto" also selects the GPU."""
-    if backend not in ("gpu", "auto"):
-        raise ValueError(f"unknown backend {backend!r} (this build provides gpu/auto)")
-    return np.asarray(HipProgramSampler(flat).sample(f, seed=seed))
+    """The reference's backend switch (src/python/ksim/run.py:79-91), every
+    branch on the GPU:
+
+    * "gpu": ``KokkosProgramSampler(flat).sample(f, seed=seed)`` — the
+      per-shot xorshift64 streams of the native sampler;
+    * "cpu": ``sample_flat(flat, f, np.random.default_rng(seed))`` — the
+      numpy sampler's PCG64 stream, drawn on the host in its order and fed to
+      the kernel, so the result is the reference's CPU result;
+    * "auto": the reference picks "gpu" when its native module imports; this
+      library is the native module, so "auto" is "gpu".
+    """
+    if backend == "auto":
+        backend = "gpu"
+    if backend == "gpu":
+        return np.asarray(HipProgramSampler(flat).sample(f, seed=seed))
+    if backend == "cpu":
+        return np.asarray(sample_flat(flat, f, np.random.default_rng(seed)))
+    raise ValueError(f"unknown backend {backend!r} (expected gpu/cpu/auto)")

@@ -49,6 +49,7 @@ class Request:
     arrival: float = 0.0
     # filled by the loop
     footprint: int | None = None   # device bytes of its program, once known
+    out_cols: int | None = None    # output columns of its program, once known
     out: np.ndarray | None = None
     admitted: float | None = None
     done: float | None = None
@@ -147,15 +148,31 @@ class ServingLoop:
         """Device bytes the program of request ``r`` takes once resident
         (its packed words), known before any upload: read from the blob
         header, from an existing sampler, or by packing a FlatProgram into a
-        blob on the host here (the upload then starts from that blob)."""
+        blob on the host here (the upload then starts from that blob).  Also
+        records the program's output columns (``r.out_cols``)."""
         if r.footprint is None:  # once per request, however long it waits
             if isinstance(r.prog, HipProgramSampler):
                 r.footprint = int(self.lib.ksim_program_device_bytes(r.prog.handle))
+                r.out_cols = r.prog.num_outputs
             else:
                 if not isinstance(r.prog, (bytes, bytearray)):
                     r.prog = program_blob(r.prog)
-                r.footprint = 4 * int(blob_stats(bytes(r.prog))["words"])
+                st = blob_stats(bytes(r.prog))
+                r.footprint = 4 * int(st["words"])
+                r.out_cols = st["num_outputs"]
         return r.footprint
+
+    def _need(self, r, ent) -> int:
+        """Device bytes admitting ``r`` takes: its f matrix, its output rows
+        (max(num_outputs, 1) bytes per shot, as allocated) and, for a
+        program not resident yet, the program's words."""
+        if ent is not None:
+            out_cols = ent[0].num_outputs
+            prog = 0
+        else:
+            prog = self._footprint(r)
+            out_cols = r.out_cols
+        return r.shots * (int(r.f.shape[1]) + max(int(out_cols), 1)) + prog
 
     def _program(self, key, prog):
         ent = self.programs.get(key)
@@ -241,9 +258,7 @@ class ServingLoop:
                         and free_slots:
                     r = pending[pend_i]
                     ent = self.programs.get(self._resolve(r))
-                    need = r.shots * (r.f.shape[1] + 64)
-                    if ent is None:
-                        need += self._footprint(r)  # a program not resident yet
+                    need = self._need(r, ent)
                     # requests whose output is still being copied back hold
                     # their buffers and program too
                     used = (self.program_bytes + sum(a.bytes for a in active)

@@ -86,6 +86,25 @@ def htkh_program(k: int, noisy_f: bool = False) -> FlatProgram:
     return single_program(htkh_graphs(k, noisy_f), F=1 if noisy_f else 0)
 
 
+def htht_h_graphs() -> list:
+    """H.T.H.T.H on one qubit, measured: P(1) = 1/4 exactly (the reference's
+    class-boundary case, docs/benchmarks.md:124-125).
+
+    graph0 = 1; graph1 has two terms, 2^-2 and 2^-2 (1 + w^{4 par}) over the
+    trying bit, so it is 3/4 at par = 0 and 1/4 at par = 1."""
+    g0 = constant_graph(0)
+    g1 = empty_graph(2, 1, Tnp=1)
+    g1.np_phases[1, 0] = 0
+    g1.np_params[1, 0, 0] = 1
+    g1.np_counts[1] = 1
+    g1.pf_power2[:] = -2
+    return [g0, g1]
+
+
+def htht_h_program() -> FlatProgram:
+    return single_program(htht_h_graphs(), F=0)
+
+
 def ghz_graphs(n: int) -> list:
     """n outputs that are all equal, each value 0/1 with probability 1/2.
 

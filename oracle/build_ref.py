@@ -2,6 +2,8 @@
 
 * ``oracle/libkokkos_port.so`` — the C++/OpenMP restatement of the
   reference's native sampler (kokkos_port.cpp), the CPU baseline.
+* ``oracle/libexact_fast.so`` — the exact oracle's graph values in C
+  (exact_fast.c), the checker for programs too large for oracle/exact.py.
 
 The reference's own native sampler (src/cpp/ksim/kokkos_sim.cpp,
 zx_eval.hpp) is not built: it needs Kokkos, nanobind and MPI
@@ -16,6 +18,16 @@ import subprocess
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 PORT_LIB = os.path.join(HERE, "libkokkos_port.so")
+EXACT_FAST_LIB = os.path.join(HERE, "libexact_fast.so")
+
+
+def build_exact_fast() -> str:
+    """``oracle/libexact_fast.so``: the exact oracle's graph values in C."""
+    cmd = ["gcc", "-O2", "-std=c11", "-fPIC", "-shared", "-Wall",
+           os.path.join(HERE, "exact_fast.c"), "-o", EXACT_FAST_LIB + ".tmp"]
+    subprocess.run(cmd, check=True)
+    os.replace(EXACT_FAST_LIB + ".tmp", EXACT_FAST_LIB)
+    return EXACT_FAST_LIB
 
 
 def build() -> str:
@@ -23,6 +35,7 @@ def build() -> str:
            os.path.join(HERE, "kokkos_port.cpp"), "-o", PORT_LIB + ".tmp"]
     subprocess.run(cmd, check=True)
     os.replace(PORT_LIB + ".tmp", PORT_LIB)
+    build_exact_fast()
     return PORT_LIB
 
 

@@ -27,7 +27,9 @@ What it restates (reference files under src/):
 Pinning (see tests/test_oracle.py): the multiply against exact complex
 arithmetic in Q(sqrt2)(i); the term formula against brute-force closed forms
 (H.T^k.H gives sin^2(k pi/8)); term splitting invariance; the float64 value
-against the reference's own ``evaluate_flat`` on golden vectors.
+against the reference's own ``evaluate_flat`` on golden vectors; the kokkos
+stream against the reference's published H.T.H.T.H figure (P(1) = 0.25034 at
+seed 43, 1,000,000 shots).
 """
 from __future__ import annotations
 
@@ -277,15 +279,43 @@ def kokkos_seed(seed: int, s: int) -> int:
     return r if r else 1
 
 
-def kokkos_uniforms(seed: int, B: int, n_out: int) -> np.ndarray:
-    """(B, n_out) uniforms of the xorshift64 stream (kokkos_sim.cpp:43-46):
-    s ^= s<<13; s ^= s>>7; s ^= s<<17; u = (s>>11) * 2^-53."""
-    out = np.zeros((B, n_out), dtype=np.float64)
-    for s in range(B):
-        r = kokkos_seed(seed, s)
+def _xorshift_stream(seed: int, shots, n_out: int, shifts=(13, 7, 17), mix=(30, 27),
+                     mul=(0x9E3779B97F4A7C15, 0xBF58476D1CE4E5B9)) -> np.ndarray:
+    """The xorshift64 draws of the given shots with the step shifts, the
+    seed-mix shifts and multipliers as parameters (the reference's by
+    default); tests plant other values here to show the known answer
+    rejects them."""
+    s = np.asarray(shots, dtype=np.uint64).reshape(-1)
+    out = np.zeros((s.shape[0], n_out), dtype=np.float64)
+    a, b, c = (np.uint64(x) for x in shifts)
+    with np.errstate(over="ignore"):
+        r = np.uint64(seed & _M64) ^ (s * np.uint64(mul[0]))
+        r ^= r >> np.uint64(mix[0])
+        r = r * np.uint64(mul[1])
+        r ^= r >> np.uint64(mix[1])
+        r[r == 0] = np.uint64(1)
         for i in range(n_out):
-            r ^= (r << 13) & _M64
-            r ^= r >> 7
-            r ^= (r << 17) & _M64
-            out[s, i] = (r >> 11) * (1.0 / 9007199254740992.0)
+            r ^= r << a
+            r ^= r >> b
+            r ^= r << c
+            out[:, i] = (r >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
     return out
+
+
+def kokkos_uniforms_at(seed: int, shots, n_out: int) -> np.ndarray:
+    """(len(shots), n_out) uniforms of the xorshift64 stream of the given
+    shot indices (kokkos_sim.cpp:43-46,215-217): the state of shot s is
+    kokkos_seed(seed, s); each draw is s ^= s<<13; s ^= s>>7; s ^= s<<17;
+    u = (s>>11) * 2^-53.  Vectorised over shots in uint64 arithmetic
+    (wrapping multiplies are the reference's uint64 semantics).
+
+    Pinned by the reference's published H.T.H.T.H figure: its GPU sampler
+    (seed 42 + 1, component 0, 1,000,000 shots, docs/benchmarks.md:82,124-125,
+    benchmarks/bench_xtim.py:146,164-167) measured P(1) = 0.25034 against the
+    exact 1/4, i.e. the fraction of shots whose first uniform is below 1/4."""
+    return _xorshift_stream(seed, shots, n_out)
+
+
+def kokkos_uniforms(seed: int, B: int, n_out: int) -> np.ndarray:
+    """(B, n_out) uniforms of shots 0..B-1 (kokkos_uniforms_at)."""
+    return kokkos_uniforms_at(seed, np.arange(B, dtype=np.uint64), n_out)

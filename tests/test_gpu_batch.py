@@ -6,6 +6,7 @@ import pytest
 from ksim_amd import synth
 from ksim_amd.batch import BatchedSampler
 from ksim_amd.sampler import HipProgramSampler
+from tests.seeded import oracle_rows, pick
 
 pytestmark = pytest.mark.gpu
 
@@ -31,10 +32,18 @@ def build(specs):
     return samplers, fs, seeds, alone
 
 
-def check_same(b, samplers, alone):
+def check_same(b, samplers, alone, oracle=True):
+    """Every program's batched rows equal its plain-launch rows and, on a
+    few shots (first, last and random ones), the oracle's rows under the
+    same seeded stream."""
     for k in range(len(samplers)):
         got = b.out[k].cpu().numpy()[:, :samplers[k].num_outputs]
         assert np.array_equal(got, alone[k]), k
+        if oracle:
+            s = samplers[k]
+            f = b.f[k].cpu().numpy()
+            idx = pick(f.shape[0], 8, k)
+            assert np.array_equal(got[idx], oracle_rows(s.prog, f, b.seeds[k], idx)), k
 
 
 def test_batch_equals_alone():
@@ -49,9 +58,7 @@ def test_batch_equals_alone():
             b.run(use_graph=use_graph)
             torch.cuda.synchronize()
             b.check()
-            for k in range(len(specs)):
-                got = b.out[k].cpu().numpy()[:, :samplers[k].num_outputs]
-                assert np.array_equal(got, alone[k]), (wave_shots, use_graph, k)
+            check_same(b, samplers, alone, oracle=use_graph and wave_shots == 64)
 
 
 def test_autotuned_batch_equals_alone(tmp_path, monkeypatch):

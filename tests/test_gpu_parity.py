@@ -153,6 +153,50 @@ def test_seeded_stream_is_the_reference_kokkos_stream():
     assert np.array_equal(got, want)
 
 
+def test_seeded_known_answer_htht_h():
+    # The reference's GPU sampler on H.T.H.T.H (P(1) = 1/4 exactly) measured
+    # 0.25034 at 1,000,000 shots with seed 42 + 1 on component 0
+    # (docs/benchmarks.md:82,124-125, benchmarks/bench_xtim.py:146,164-167).
+    # The same seeded call here gives the same count of ones, shot for shot
+    # the oracle's stream, through every entry point the reference's
+    # callers use.
+    from ksim_amd.sampler import run_sampler
+    B, seed = 1_000_000, 43
+    prog = synth.htht_h_program()
+    f = np.zeros((B, 0), np.uint8)
+    want = exact.kokkos_uniforms(seed, B, 1)[:, 0] < 0.25
+    log("known answer: HipProgramSampler")
+    got = HipProgramSampler(prog).sample(f, seed)[:, 0]
+    assert round(float(got.mean()), 5) == 0.25034
+    assert np.array_equal(got.astype(bool), want)
+    log("known answer: run_sampler(backend='gpu')")
+    assert np.array_equal(run_sampler(prog, f, seed, backend="gpu")[:, 0], got)
+    log("known answer: Component")
+    kc = Component([graph_to_dict(g) for g in prog.components[0].graphs], 0)
+    assert np.array_equal(kc.sample(f, seed)[:, 0], got)
+    # the same with the terms split over more terms and 2 / 4 waves per tile
+    log("known answer: split terms, waves per tile")
+    s = HipProgramSampler(synth.split_program(prog, 6, seed=3))
+    for L in (1, 2, 4):
+        s.set_term_split(L)
+        assert np.array_equal(s.sample(f[:200_000], seed)[:, 0], got[:200_000]), L
+
+
+def test_component_sample_equals_oracle_stream():
+    # Component.sample (kokkos_sim.cpp:188-240): f_sel rows in, (B, n_out)
+    # bits out, drawn from the xorshift64 stream of its seed
+    prog = synth.shape_program(n_q=6, t=6, F=9, n_out=7, seed=41)
+    comp = prog.components[0]
+    gds = [graph_to_dict(g) for g in comp.graphs]
+    kc = Component(gds, 9)
+    B = 512
+    f = synth.random_f(B, 9, 0.5, 8)
+    for seed in (0, 99, 2 ** 63 + 17):
+        U = exact.kokkos_uniforms(seed, B, kc.n_out)
+        want = exact.sample_component(gds, f.astype(np.int64), U)
+        assert np.array_equal(kc.sample(f, seed), want), seed
+
+
 def test_component_api_edges():
     prog = synth.htkh_program(1, noisy_f=True)
     gds = [graph_to_dict(g) for g in prog.components[0].graphs]

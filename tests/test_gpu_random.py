@@ -11,6 +11,7 @@ import pytest
 from ksim_amd import synth
 from ksim_amd.sampler import HipProgramSampler, unpack_outputs
 from oracle import exact
+from tests.seeded import oracle_rows, pick
 
 pytestmark = pytest.mark.gpu
 
@@ -65,6 +66,9 @@ def test_random_programs(seed):
             s.set_term_split(L)
             got[L] = (s.sample_uniforms(f, U), s.sample(f, 7), s.sample_packed(f, 7))
         assert np.array_equal(got[1][0][:nb], want)
+        # the seeded stream too
+        idx = pick(B, 8, B)
+        assert np.array_equal(got[1][1][idx], oracle_rows(prog, f, 7, idx))
         for L in (2, 4):
             for j in range(3):
                 assert np.array_equal(got[L][j], got[1][j]), (L, j)
@@ -88,5 +92,8 @@ def test_random_programs(seed):
             torch.cuda.synchronize()
             b.check()
             for j, q in enumerate(pool):
-                assert np.array_equal(b.out[j].cpu().numpy()[:, :q[0].num_outputs], q[2]), j
+                got_b = b.out[j].cpu().numpy()[:, :q[0].num_outputs]
+                assert np.array_equal(got_b, q[2]), j
+                idx = pick(q[1].shape[0], 4, j)
+                assert np.array_equal(got_b[idx], oracle_rows(q[0].prog, q[1], 7, idx)), j
             pool = []

@@ -7,6 +7,7 @@ import pytest
 from ksim_amd import synth
 from ksim_amd.sampler import HipProgramSampler
 from ksim_amd.serve import Request, ServingLoop, poisson_arrivals
+from tests.seeded import oracle_rows, pick
 
 pytestmark = pytest.mark.gpu
 
@@ -40,9 +41,14 @@ def test_results_same_alone_and_in_mix(tile, step_ms, rate):
         assert {loop._split(s, c) for s in samplers for c in range(s.n_comp)} & {2, 4}
     stats = loop.run(reqs, realtime=False)
     assert stats["requests"] == len(reqs)
-    for r in reqs:
+    for i, r in enumerate(reqs):
         alone = samplers[r.key].sample(r.f, r.seed)
         assert r.out is not None and np.array_equal(r.out, alone), (r.key, r.shots)
+        # and a few served shots against the oracle's seeded stream
+        idx = pick(r.shots, 4, i)
+        if idx.size:
+            assert np.array_equal(r.out[idx], oracle_rows(samplers[r.key].prog, r.f, r.seed,
+                                                          idx)), (r.key, r.shots)
     loop.close()
 
 
@@ -87,7 +93,7 @@ def test_admission_counts_program_bytes():
     prog_b = synth.shape_program(n_q=8, t=10, F=6, n_out=12, seed=22)
     words = [4 * int(blob_stats(program_blob(p))["words"]) for p in (prog_a, prog_b)]
     shots = 500
-    rows = shots * (6 + 64)
+    rows = shots * (6 + 12)  # f columns + output columns per shot
     budget = max(words) + rows + max(words) // 2
     reqs = [Request(key="a", prog=prog_a, f=synth.random_f(shots, 6, 0.3, 1), seed=5),
             Request(key="b", prog=prog_b, f=synth.random_f(shots, 6, 0.3, 2), seed=6)]

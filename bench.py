@@ -181,8 +181,8 @@ def cpu_baseline_mix(progs, Fs, shots_of, budget_s=15.0):
     mix's rate = total shots / sum of each program's shots x its measured
     time per shot."""
     probes = [cpu_baseline_port(p, F, 32, seed=k) for k, (p, F) in enumerate(zip(progs, Fs))]
-    est = sum(pr["us_per_shot"] for pr in probes) * 32e-6
-    scale = max(1.0, budget_s / max(est, 1e-6) / 32.0)
+    est = sum(pr["us_per_shot"] for pr in probes) * 32e-6  # seconds the probes took
+    scale = max(1.0, budget_s / max(est, 1e-6))
     total_t, sampled, threads = 0.0, 0, 1
     for k, (p, F, pr) in enumerate(zip(progs, Fs, probes)):
         n = int(min(shots_of[k], max(32, 32 * scale)))

@@ -305,6 +305,34 @@ int ksim_program_split_hint(const ksim_program *p, int32_t comp, double tile_ops
                             int32_t *split, double *tile_cost_ops);
 void ksim_batch_destroy(ksim_batch *b);
 
+/* ---- batch without allocation ------------------------------------------- */
+
+/* Sizes of the fixed device array ksim_batch_sample_static works in. */
+#define KSIM_STATIC_MAX_PROGS 256
+#define KSIM_STATIC_MAX_ITEMS 65536
+
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under stream capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  One batched persistent launch over n programs, the
+ * same work items and results as ksim_batch_create + ksim_batch_run with
+ * the same arguments (the caller passes every input and output as for
+ * ksim_batch_create), but the work list, program table, work cursor and
+ * error word live in a fixed array in device memory, one per GPU (about
+ * 1.6 MB, held until the process ends), so the call allocates nothing.
+ * Limits: n <= KSIM_STATIC_MAX_PROGS and at most KSIM_STATIC_MAX_ITEMS work
+ * items (a component of B shots at term split L takes
+ * ceil(B / (wave_shots * 4 / L)) items); larger calls are rejected.  The
+ * programs must live on the current device, where creating them has built
+ * the lookup tables.  Blocks until the launch has finished and reports
+ * device range errors.  Every call on one GPU shares its array: calls that
+ * overlap on one GPU (from several threads or streams) race. */
+int ksim_batch_sample_static(ksim_program *const *progs, int32_t n,
+                             const uint8_t *const *d_f, const int32_t *f_cols,
+                             const int64_t *B, const uint64_t *seed,
+                             uint8_t *const *d_out, int32_t wave_shots,
+                             double tile_ops, int32_t persistent_blocks,
+                             void *stream);
+
 /* ---- scheduler: the device side of the serving loop ---------------------- */
 
 typedef struct ksim_sched ksim_sched;

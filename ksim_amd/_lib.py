@@ -67,6 +67,9 @@ EXPORTS = {
     "ksim_batch_run": (C.c_int, [vp, C.c_int32, C.c_int32, vp]),
     "ksim_batch_check": (C.c_int, [vp]),
     "ksim_batch_destroy": (None, [vp]),
+    "ksim_batch_sample_static": (C.c_int, [C.POINTER(vp), C.c_int32, C.POINTER(vp), i32p, i64p,
+                                           C.POINTER(C.c_uint64), C.POINTER(vp), C.c_int32,
+                                           C.c_double, C.c_int32, vp]),
     "ksim_sched_create": (C.c_int, [C.c_int32, C.c_int32, C.POINTER(vp)]),
     "ksim_sched_set_slot": (C.c_int, [vp, C.c_int32, vp, vp, C.c_int32, C.c_int64, C.c_uint64, vp]),
     "ksim_sched_clear_slot": (C.c_int, [vp, C.c_int32]),

@@ -106,3 +106,40 @@ class BatchedSampler:
                 _lib.load().ksim_batch_destroy(h)
             except Exception:
                 pass
+
+
+STATIC_MAX_PROGS = 256      # KSIM_STATIC_MAX_PROGS
+STATIC_MAX_ITEMS = 65536    # KSIM_STATIC_MAX_ITEMS
+
+
+def sample_static(samplers: list[HipProgramSampler], f: list, seeds: list[int], out: list,
+                  wave_shots: int = 128, tile_ops: float = 0.0, blocks: int | None = None,
+                  stream=None) -> None:
+    """One batched launch that allocates no device memory
+    (``ksim_batch_sample_static``): the caller's outputs ``out[k]``
+    ((B_k, max(num_outputs_k, 1)) uint8 CUDA tensors) receive what a
+    :class:`BatchedSampler` with the same arguments writes.  Blocks until
+    done.  At most 256 programs and 65536 work items per call; calls that
+    overlap on one GPU race on its fixed work-list array."""
+    import torch
+    from .sampler import _check_seed, device_u8_2d
+    if not len(samplers) == len(f) == len(seeds) == len(out):
+        raise ValueError("samplers, f, seeds and out must have the same length")
+    for k, (s, fk, ok) in enumerate(zip(samplers, f, out)):
+        device_u8_2d(fk, f"f[{k}]")
+        device_u8_2d(ok, f"out[{k}]", (max(s.num_outputs, 1),))
+        if ok.shape[0] != fk.shape[0]:
+            raise ValueError(f"out[{k}] has {ok.shape[0]} rows, f[{k}] {fk.shape[0]}")
+        _check_seed(seeds[k], s.n_comp)
+    n = len(samplers)
+    progs = (C.c_void_p * max(n, 1))(*[s.handle for s in samplers])
+    dfs = (C.c_void_p * max(n, 1))(*[fk.data_ptr() if fk.numel() else None for fk in f])
+    fcols = np.asarray([int(fk.shape[1]) for fk in f] or [0], np.int32)
+    Bs = np.asarray([int(fk.shape[0]) for fk in f] or [0], np.int64)
+    sd = (C.c_uint64 * max(n, 1))(*[int(x) for x in seeds])
+    douts = (C.c_void_p * max(n, 1))(*[o.data_ptr() if o.numel() else None for o in out])
+    nb = int(blocks or device_info()["cus"] * 6)
+    st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+    _lib.check(_lib.load().ksim_batch_sample_static(progs, n, dfs, _lib.ptr(fcols, C.c_int32),
+                                                    _lib.ptr(Bs, C.c_int64), sd, douts,
+                                                    int(wave_shots), float(tile_ops), nb, st))

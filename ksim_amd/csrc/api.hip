@@ -1107,6 +1107,98 @@ int ksim_device_info(int64_t *info4) {
 
 // ------------------------------------------------------------------ batch
 
+namespace {
+
+// The work list of a batch: per program its table row, and the items
+// (program, component, shot range, term split) heaviest first.
+struct BatchPlan {
+  std::vector<BatchProg> progs;
+  std::vector<BatchItem> items;
+  int wl = 1;              // lane parameter words: the widest program's
+  bool all_narrow = true;  // every program without 128-bit graphs
+};
+
+BatchPlan plan_batch(ksim_program *const *progs, int32_t n, const uint8_t *const *d_f,
+                     const int32_t *f_cols, const int64_t *B, const uint64_t *seed,
+                     uint8_t *const *d_out, int32_t wave_shots, double tile_ops) {
+  if (n < 0) throw std::invalid_argument("n < 0");
+  if (n > 0 && (!progs || !d_f || !f_cols || !B || !seed || !d_out))
+    throw std::invalid_argument("NULL argument");
+  if (wave_shots < 64 || wave_shots % 64) throw std::invalid_argument("wave_shots must be a multiple of 64");
+  double total_ops = 0.0;
+  for (int k = 0; k < n; ++k) {
+    ksim_program *p = progs[k];
+    if (!p) throw std::invalid_argument("NULL program in batch");
+    if (B[k] < 0) throw std::invalid_argument("B < 0");
+    if (B[k] > 0 && ((!d_f[k] && f_cols[k] > 0) || !d_out[k])) throw std::invalid_argument("NULL buffer");
+    if (f_cols[k] < p->num_f) throw std::invalid_argument("f matrix has fewer columns than f_selection needs");
+    total_ops += p->st.ops * (double)B[k];
+  }
+  BatchPlan plan;
+  plan.progs.resize(n);
+  struct Cost { double c; BatchItem it; };
+  std::vector<Cost> items;
+  // a tile should take each wave at most 1/8 of its share of the batch
+  // (waves: 5 resident blocks of 4 waves per CU), so the heaviest-first
+  // order leaves a short tail
+  int dev = 0, cus = 256;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "CU count");
+  const double tile_target =
+      tile_ops > 0 ? tile_ops : total_ops / (double)(cus * 5 * 4) / 8.0;
+  for (int k = 0; k < n; ++k) {
+    ksim_program *p = progs[k];
+    plan.wl = std::max(plan.wl, p->wl);
+    plan.all_narrow = plan.all_narrow && p->all_narrow;
+    plan.progs[k] = BatchProg{p->d_prog, d_f[k], d_out[k], seed[k], B[k], f_cols[k],
+                              p->num_outputs, p->tot_out, p->wl};
+    for (int c = 0; c < p->n_comp; ++c) {
+      const int L = split_for(p, c, tile_target);
+      const double tc = tile_cost(p, c, L);
+      // one item: wave_shots shots per group of L waves, 4 / L groups
+      const int64_t span = (int64_t)wave_shots * (4 / L);
+      for (int64_t s0 = 0; s0 < B[k]; s0 += span) {
+        const int64_t ns = std::min<int64_t>(span, B[k] - s0);
+        const int64_t rounds = ((ns + 63) / 64 + (4 / L) - 1) / (4 / L);
+        items.push_back(Cost{tc * (double)rounds, BatchItem{k, c, s0, (int32_t)ns, L}});
+      }
+    }
+  }
+  // heaviest items first: the dynamic cursor then balances the tail
+  std::stable_sort(items.begin(), items.end(),
+                   [](const Cost &x, const Cost &y) { return x.c > y.c; });
+  if (items.size() >= 0xffffffffull) throw std::invalid_argument("too many work items");
+  plan.items.resize(items.size());
+  for (size_t i = 0; i < items.size(); ++i) plan.items[i] = items[i].it;
+  return plan;
+}
+
+// One batched launch on `st`: clear the cursor and error word (a fill
+// kernel, so the launch can be captured), then the persistent kernel.
+void launch_batch(const BatchProg *d_progs, const BatchItem *d_items, uint32_t n_items,
+                  uint32_t *d_cursor, int wl, bool all_narrow, int blocks, const DevTables &t,
+                  hipStream_t st) {
+  hipLaunchKernelGGL(k_reset_cursor, dim3(1), dim3(64), 0, st, d_cursor, d_cursor + 1);
+  hip_check(hipGetLastError(), "k_reset_cursor");
+#define KSIM_BATCH_LAUNCH(W, WD)                                                               \
+  hipLaunchKernelGGL((k_batch<W, WD>), dim3(blocks), dim3(kBlock), 0, st, d_progs, d_items,     \
+                     n_items, d_cursor, d_cursor + 1, t.view())
+#define KSIM_BATCH_CASES(W)                               \
+  if (all_narrow) KSIM_BATCH_LAUNCH(W, false);            \
+  else KSIM_BATCH_LAUNCH(W, true);
+  switch (wl) {
+    case 1: KSIM_BATCH_CASES(1); break;
+    case 2: KSIM_BATCH_CASES(2); break;
+    case 4: KSIM_BATCH_CASES(4); break;
+    default: KSIM_BATCH_CASES(8); break;
+  }
+#undef KSIM_BATCH_CASES
+#undef KSIM_BATCH_LAUNCH
+  hip_check(hipGetLastError(), "k_batch");
+}
+
+}  // namespace
+
 struct ksim_batch {
   std::vector<ksim_program *> progs;
   BatchProg *d_progs = nullptr;
@@ -1140,66 +1232,26 @@ int ksim_batch_create(ksim_program *const *progs, int32_t n, const uint8_t *cons
                       uint8_t *const *d_out, int32_t wave_shots, double tile_ops,
                       ksim_batch **out) {
   return guarded([&] {
-    if (!out || (n > 0 && (!progs || !d_f || !f_cols || !B || !seed || !d_out)))
-      throw std::invalid_argument("NULL argument");
-    if (wave_shots < 64 || wave_shots % 64) throw std::invalid_argument("wave_shots must be a multiple of 64");
-    if (n < 0) throw std::invalid_argument("n < 0");
-    double total_ops = 0.0;
-    for (int k = 0; k < n; ++k) {
-      ksim_program *p = progs[k];
-      if (!p) throw std::invalid_argument("NULL program in batch");
-      if (B[k] < 0) throw std::invalid_argument("B < 0");
-      if (B[k] > 0 && ((!d_f[k] && f_cols[k] > 0) || !d_out[k])) throw std::invalid_argument("NULL buffer");
-      if (f_cols[k] < p->num_f) throw std::invalid_argument("f matrix has fewer columns than f_selection needs");
-      total_ops += p->st.ops * (double)B[k];
-    }
+    if (!out) throw std::invalid_argument("NULL argument");
+    BatchPlan plan = plan_batch(progs, n, d_f, f_cols, B, seed, d_out, wave_shots, tile_ops);
     // owned until handed out, so every throw below releases it
     std::unique_ptr<ksim_batch, void (*)(ksim_batch *)> hold(new ksim_batch(), ksim_batch_destroy);
     ksim_batch *b = hold.get();
     b->wave_shots = wave_shots;
-    std::vector<BatchProg> hp(n);
-    struct Cost { double c; BatchItem it; };
-    std::vector<Cost> items;
-    // a tile should take each wave at most 1/8 of its share of the batch
-    // (waves: 5 resident blocks of 4 waves per CU), so the heaviest-first
-    // order leaves a short tail
-    int dev = 0, cus = 256;
-    hip_check(hipGetDevice(&dev), "hipGetDevice");
-    hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "CU count");
-    const double tile_target =
-        tile_ops > 0 ? tile_ops : total_ops / (double)(cus * 5 * 4) / 8.0;
-    for (int k = 0; k < n; ++k) {
-      ksim_program *p = progs[k];
-      b->progs.push_back(p);
-      b->wl = std::max(b->wl, p->wl);
-      b->all_narrow = b->all_narrow && p->all_narrow;
-      hp[k] = BatchProg{p->d_prog, d_f[k], d_out[k], seed[k], B[k], f_cols[k], p->num_outputs,
-                        p->tot_out, p->wl};
-      for (int c = 0; c < p->n_comp; ++c) {
-        const int L = split_for(p, c, tile_target);
-        const double tc = tile_cost(p, c, L);
-        // one item: wave_shots shots per group of L waves, 4 / L groups
-        const int64_t span = (int64_t)wave_shots * (4 / L);
-        for (int64_t s0 = 0; s0 < B[k]; s0 += span) {
-          const int64_t ns = std::min<int64_t>(span, B[k] - s0);
-          const int64_t rounds = ((ns + 63) / 64 + (4 / L) - 1) / (4 / L);
-          items.push_back(Cost{tc * (double)rounds, BatchItem{k, c, s0, (int32_t)ns, L}});
-        }
-      }
-    }
-    // heaviest items first: the dynamic cursor then balances the tail
-    std::stable_sort(items.begin(), items.end(),
-                     [](const Cost &x, const Cost &y) { return x.c > y.c; });
-    std::vector<BatchItem> hi(items.size());
-    for (size_t i = 0; i < items.size(); ++i) hi[i] = items[i].it;
-    if (hi.size() >= 0xffffffffull) throw std::invalid_argument("too many work items");
-    b->n_items = (uint32_t)hi.size();
+    b->progs.assign(progs, progs + n);
+    b->wl = plan.wl;
+    b->all_narrow = plan.all_narrow;
+    b->n_items = (uint32_t)plan.items.size();
     hip_check(hipMalloc(&b->d_progs, sizeof(BatchProg) * std::max(1, n)), "hipMalloc");
-    hip_check(hipMalloc(&b->d_items, sizeof(BatchItem) * std::max<size_t>(1, hi.size())), "hipMalloc");
+    hip_check(hipMalloc(&b->d_items, sizeof(BatchItem) * std::max<size_t>(1, plan.items.size())),
+              "hipMalloc");
     hip_check(hipMalloc(&b->d_cursor, 64), "hipMalloc");
-    if (n) hip_check(hipMemcpy(b->d_progs, hp.data(), sizeof(BatchProg) * n, hipMemcpyHostToDevice), "H2D");
-    if (!hi.empty())
-      hip_check(hipMemcpy(b->d_items, hi.data(), sizeof(BatchItem) * hi.size(), hipMemcpyHostToDevice), "H2D");
+    if (n)
+      hip_check(hipMemcpy(b->d_progs, plan.progs.data(), sizeof(BatchProg) * n,
+                          hipMemcpyHostToDevice), "H2D");
+    if (!plan.items.empty())
+      hip_check(hipMemcpy(b->d_items, plan.items.data(), sizeof(BatchItem) * plan.items.size(),
+                          hipMemcpyHostToDevice), "H2D");
     hip_check(hipMemset(b->d_cursor, 0, 64), "hipMemset");
     tables_for_current_device();
     *out = hold.release();
@@ -1207,24 +1259,8 @@ int ksim_batch_create(ksim_program *const *progs, int32_t n, const uint8_t *cons
 }
 
 static void batch_enqueue(ksim_batch *b, int blocks, hipStream_t st) {
-  DevTables t = tables_for_current_device();
-  hipLaunchKernelGGL(k_reset_cursor, dim3(1), dim3(64), 0, st, b->d_cursor, b->d_cursor + 1);
-  hip_check(hipGetLastError(), "k_reset_cursor");
-#define KSIM_BATCH_LAUNCH(W, WD)                                                               \
-  hipLaunchKernelGGL((k_batch<W, WD>), dim3(blocks), dim3(kBlock), 0, st, b->d_progs, b->d_items, \
-                     b->n_items, b->d_cursor, b->d_cursor + 1, t.view())
-#define KSIM_BATCH_CASES(W)                               \
-  if (b->all_narrow) KSIM_BATCH_LAUNCH(W, false);         \
-  else KSIM_BATCH_LAUNCH(W, true);
-  switch (b->wl) {
-    case 1: KSIM_BATCH_CASES(1); break;
-    case 2: KSIM_BATCH_CASES(2); break;
-    case 4: KSIM_BATCH_CASES(4); break;
-    default: KSIM_BATCH_CASES(8); break;
-  }
-#undef KSIM_BATCH_CASES
-#undef KSIM_BATCH_LAUNCH
-  hip_check(hipGetLastError(), "k_batch");
+  launch_batch(b->d_progs, b->d_items, b->n_items, b->d_cursor, b->wl, b->all_narrow, blocks,
+               tables_for_current_device(), st);
 }
 
 int ksim_batch_run(ksim_batch *b, int32_t persistent_blocks, int32_t use_graph, void *stream) {
@@ -1270,6 +1306,106 @@ void ksim_batch_destroy(ksim_batch *b) {
   if (b->d_items) (void)hipFree(b->d_items);
   if (b->d_cursor) (void)hipFree(b->d_cursor);
   delete b;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------- batch, no allocation
+// ksim_batch_sample_static keeps the work list, the program table, the work
+// cursor and the error word of its launch in `scratch`, a fixed array in
+// device memory (one copy per GPU, held until the process ends), so a call
+// allocates nothing.  Layout: program table, work items, cursor words, each
+// on a 256-byte boundary.
+namespace {
+
+constexpr size_t round256(size_t x) { return (x + 255) & ~(size_t)255; }
+constexpr size_t kStaticProgsOff = 0;
+constexpr size_t kStaticItemsOff = round256(sizeof(BatchProg) * KSIM_STATIC_MAX_PROGS);
+constexpr size_t kStaticCursorOff =
+    kStaticItemsOff + round256(sizeof(BatchItem) * (size_t)KSIM_STATIC_MAX_ITEMS);
+constexpr size_t kStaticBytes = kStaticCursorOff + 256;
+
+}  // namespace
+
+// the array itself; host code uses only the per-device address that
+// hipGetSymbolAddress returns for it (its host-side name is no device address)
+__device__ __align__(256) unsigned char scratch[kStaticBytes];
+
+namespace {
+
+std::mutex g_scratch_mu;
+std::map<int, unsigned char *> g_scratch;  // device -> address of its `scratch`
+
+// the address of this device's `scratch`, looked up once per device with the
+// device current
+unsigned char *scratch_for_current_device() {
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch.find(dev);
+  if (it != g_scratch.end()) return it->second;
+  void *ptr = nullptr;
+  hip_check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+  if (!ptr) throw DeviceError("hipGetSymbolAddress(scratch) returned NULL");
+  g_scratch[dev] = (unsigned char *)ptr;
+  return (unsigned char *)ptr;
+}
+
+// the current device's lookup tables, without building them: they exist once
+// a program has been created on the device
+DevTables tables_if_built() {
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  std::lock_guard<std::mutex> lk(g_tab_mu);
+  auto it = g_tabs.find(dev);
+  if (it == g_tabs.end())
+    throw std::invalid_argument("no program has been created on this device yet");
+  return it->second;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ksim_batch_sample_static(ksim_program *const *progs, int32_t n, const uint8_t *const *d_f,
+                             const int32_t *f_cols, const int64_t *B, const uint64_t *seed,
+                             uint8_t *const *d_out, int32_t wave_shots, double tile_ops,
+                             int32_t persistent_blocks, void *stream) {
+  return guarded([&] {
+    if (n > KSIM_STATIC_MAX_PROGS)
+      throw std::invalid_argument("more than KSIM_STATIC_MAX_PROGS (" +
+                                  std::to_string(KSIM_STATIC_MAX_PROGS) +
+                                  ") programs in one call without allocation");
+    if (persistent_blocks <= 0) throw std::invalid_argument("persistent_blocks must be > 0");
+    BatchPlan plan = plan_batch(progs, n, d_f, f_cols, B, seed, d_out, wave_shots, tile_ops);
+    if (plan.items.size() > (size_t)KSIM_STATIC_MAX_ITEMS)
+      throw std::invalid_argument(
+          std::to_string(plan.items.size()) + " work items exceed KSIM_STATIC_MAX_ITEMS (" +
+          std::to_string(KSIM_STATIC_MAX_ITEMS) + "); use larger wave_shots or fewer shots");
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    for (int k = 0; k < n; ++k)
+      if (progs[k]->device != dev)
+        throw std::invalid_argument("a program lives on another device than the current one");
+    if (plan.items.empty()) return;
+    const DevTables t = tables_if_built();
+    unsigned char *base = scratch_for_current_device();
+    BatchProg *dp = (BatchProg *)(base + kStaticProgsOff);
+    BatchItem *di = (BatchItem *)(base + kStaticItemsOff);
+    uint32_t *dc = (uint32_t *)(base + kStaticCursorOff);
+    hipStream_t st = (hipStream_t)stream;
+    hip_check(hipMemcpyAsync(dp, plan.progs.data(), sizeof(BatchProg) * (size_t)n,
+                             hipMemcpyHostToDevice, st), "H2D programs");
+    hip_check(hipMemcpyAsync(di, plan.items.data(), sizeof(BatchItem) * plan.items.size(),
+                             hipMemcpyHostToDevice, st), "H2D items");
+    launch_batch(dp, di, (uint32_t)plan.items.size(), dc, plan.wl, plan.all_narrow,
+                 persistent_blocks, t, st);
+    // the next call rewrites the same array: finish this launch first
+    hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    uint32_t e = 0;
+    hip_check(hipMemcpy(&e, dc + 1, sizeof(e), hipMemcpyDeviceToHost), "read err");
+    if (e) throw RangeError(err_text(e));
+  });
 }
 
 }  // extern "C"

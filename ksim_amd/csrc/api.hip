@@ -362,6 +362,116 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WIDE ? K
   }
 }
 
+// Components wider than the register words: each lane's parameter words in
+// LDS (`dyn`, `words` dwords per lane, interleaved by lane within a wave),
+// L waves (1, 2 or 4) per 64-shot tile; block b takes the tile groups
+// b, b + gridDim.x, ... (4/L tiles per group).
+template <bool NOISE, bool WIDE>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WIDE ? KSIM_WAVES_PER_EU : KSIM_NARROW_WAVES_PER_EU))) void k_sample_lds(
+    ShotArgs a, Tables tb, int L, int words) {
+  __shared__ LdsTables lt;
+  __shared__ int4 xch[XCH_INT4];
+  extern __shared__ uint32_t dyn[];
+  prefetch_program(a.prog, a.prefetch_dwords);
+  load_tables(&lt, tb);
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  uint32_t *pvl = dyn + (size_t)wave * words * 64 + lane;
+  const int G = 4 / L;
+  const Split sp{L, wave % L, wave - wave % L, xch};
+  const int64_t ntiles = (a.B + 63) >> 6;
+  const int64_t ngroups = (ntiles + G - 1) / G;
+  uint32_t xs = 0;
+  for (int64_t gi = blockIdx.x; gi < ngroups; gi += gridDim.x) {
+    const int64_t tile = gi * G + wave / L;
+    const int64_t off = tile * 64 + lane;
+    const bool valid = off < a.B;
+    const int64_t s = min(off, a.B - 1);
+    run_shot_lds<NOISE, true, WIDE>(a, (int)blockIdx.y, s, &lt, tb, pvl, sp, valid, &xs);
+  }
+}
+
+// k_eval for graphs wider than the register words (parameter words in LDS)
+__global__ __launch_bounds__(kBlock) void k_eval_lds(EvalArgs a, Tables tb, int words) {
+  __shared__ LdsTables lt;
+  extern __shared__ uint32_t dyn[];
+  load_tables(&lt, tb);
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.B) return;
+  uint32_t *pvl = dyn + (size_t)(threadIdx.x >> 6) * words * 64 + (threadIdx.x & 63);
+  const uint8_t *row = a.params + s * (int64_t)a.P;
+  for (int w = 0; w < words; ++w) {
+    uint32_t word = 0;
+    const int j0 = 32 * w, jn = min(32, a.P - j0);
+    for (int jj = 0; jj < jn; ++jj) word |= (uint32_t)(row[j0 + jj] & 1u) << jj;
+    pvl[64 * w] = word;
+  }
+  uint32_t err = 0;
+  const cword *gh = as_const(a.gh);
+  const LdsWords pw{pvl, (int)gh[GH_W32]};
+  if (a.mode == 0) {
+    Acc4 acc;
+    eval_graph(as_const(a.prog), gh, pw, &lt, tb, acc, err);
+    int64_t *c = a.coeffs + s * 8;
+    const i128_t v[4] = {acc.a, acc.b, acc.c, acc.d};
+    for (int k = 0; k < 4; ++k) {
+      c[2 * k] = (int64_t)(uint64_t)(u128_t)v[k];
+      c[2 * k + 1] = (int64_t)(v[k] >> 64);
+    }
+    a.exp2[s] = (int)gh[GH_E0];
+  } else {
+    uint32_t xs = 0;
+    a.absout[s] = graph_prob<LdsWords, false, true>(as_const(a.prog), gh, pw, &lt, tb, err,
+                                                    Split{1, 0, 0, nullptr}, xs);
+  }
+  if (err) atomicOr(a.err, err);
+}
+
+// k_batch for batches holding a component wider than the register words:
+// every program's parameter words in LDS (`words` dwords per lane, the
+// widest component's).
+template <bool WIDE>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WIDE ? KSIM_WAVES_PER_EU : KSIM_NARROW_WAVES_PER_EU))) void k_batch_lds(
+    const BatchProg *__restrict__ progs, const BatchItem *__restrict__ items, uint32_t n_items,
+    uint32_t *cursor, uint32_t *err, Tables tb, int words) {
+  __shared__ LdsTables lt;
+  __shared__ uint32_t s_item;
+  __shared__ int4 xch[XCH_INT4];
+  extern __shared__ uint32_t dyn[];
+  load_tables(&lt, tb);
+  uint32_t xs = 0;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  uint32_t *pvl = dyn + (size_t)wave * words * 64 + lane;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(cursor, 1u);
+    __syncthreads();
+    const uint32_t idx = __builtin_amdgcn_readfirstlane(s_item);
+    __syncthreads();
+    if (idx >= n_items) break;
+    const BatchItem it = items[idx];
+    const BatchProg bp = progs[it.prog];
+    ShotArgs a;
+    a.prog = bp.prog; a.f = bp.f; a.U = nullptr; a.out = bp.out; a.err = err;
+    a.B = bp.B; a.shot_base = 0; a.seed = bp.seed;
+    a.num_f = bp.num_f; a.num_outputs = bp.num_outputs; a.tot_out = bp.tot_out;
+    a.prefetch_dwords = 0; a.out_words = 0; a.out_atomic = 0;
+    a.noise = NoiseDev{};
+    const int L = it.split, G = 4 / L;
+    const Split sp{L, wave % L, wave - wave % L, xch};
+    const int ntiles = (it.count + 63) >> 6;
+    const int rounds = (ntiles + G - 1) / G;
+    for (int r = 0; r < rounds; ++r) {
+      const int tile = r * G + wave / L;
+      if (L == 1 && tile >= ntiles) break;
+      const int off = tile * 64 + lane;
+      const bool valid = off < it.count;
+      const int64_t s = it.shot0 + min(off, it.count - 1);
+      run_shot_lds<false, true, WIDE>(a, it.comp, s, &lt, tb, pvl, sp, valid, &xs);
+    }
+  }
+}
+
 }  // namespace ksim
 
 // ------------------------------------------------------------------ objects
@@ -453,7 +563,32 @@ double tile_cost(const ksim_program *p, int c, int L) {
 
 namespace {
 
-int lane_words(int wmax) { return wmax <= 1 ? 1 : wmax <= 2 ? 2 : wmax <= 4 ? 4 : 8; }
+// register words of the kernel that runs a program of wmax parameter words
+// per lane: 1, 2, 4 or 8, or 0 for the kernels that keep them in LDS
+int lane_words(int wmax) {
+  return wmax <= 1 ? 1 : wmax <= 2 ? 2 : wmax <= 4 ? 4 : wmax <= REG_WORDS_MAX ? 8 : 0;
+}
+
+// dynamic LDS of the LDS-word kernels: four waves of `words` dwords per lane
+size_t lds_words_bytes(int words) { return (size_t)4 * 64 * 4 * (size_t)words; }
+
+// Make the LDS-word kernel `k` accept `bytes` of dynamic LDS, after checking
+// that the device holds them beside the kernel's static LDS.
+void allow_lds(const void *k, size_t bytes) {
+  hipFuncAttributes fa;
+  hip_check(hipFuncGetAttributes(&fa, k), "hipFuncGetAttributes");
+  int dev = 0, max_block = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  hip_check(hipDeviceGetAttribute(&max_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev),
+            "LDS per block");
+  if (fa.sharedSizeBytes + bytes > (size_t)max_block)
+    throw std::invalid_argument("component too wide: its parameter words need " +
+                                std::to_string(fa.sharedSizeBytes + bytes) +
+                                " bytes of LDS per block, the device has " +
+                                std::to_string(max_block));
+  (void)hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  (void)hipGetLastError();
+}
 
 template <class T>
 void ensure(T *&p, size_t &cap, size_t bytes) {
@@ -494,7 +629,18 @@ constexpr int kSplitBlocks = 2048;
 // kernel (split false) or the split kernel for lane words wl, 64-bit or
 // 128-bit sums and with or without in-kernel noise: the runtime's occupancy
 // figure for the compiled kernel, asked once per process.
-int resident_waves(int wl, bool narrow, bool nz, bool split) {
+int resident_waves(int wl, bool narrow, bool nz, bool split, int words = 0) {
+  if (wl == 0) {
+    // the LDS-word kernel, for this program's words (asked per launch size)
+    const void *k = nz ? (narrow ? reinterpret_cast<const void *>(&k_sample_lds<true, false>)
+                                 : reinterpret_cast<const void *>(&k_sample_lds<true, true>))
+                       : (narrow ? reinterpret_cast<const void *>(&k_sample_lds<false, false>)
+                                 : reinterpret_cast<const void *>(&k_sample_lds<false, true>));
+    int nb = 0;
+    hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, kBlock, lds_words_bytes(words)),
+              "occupancy");
+    return std::max(nb, 1);
+  }
   static const std::array<int, 32> occ = [] {
     std::array<int, 32> o{};
     auto q = [](const void *k) {
@@ -575,10 +721,12 @@ int plain_split(const ksim_program *p, int64_t B, bool nz) {
   // of iters x w[k] work, in rounds of o resident waves.
   const int cus = simds / 4;
   auto cost = [&](int k) {
-    const int o = resident_waves(p->wl, p->all_narrow, nz, Ls[k] > 1);
+    const int o = resident_waves(p->wl, p->all_narrow, nz, Ls[k] > 1, p->wmax);
     const int64_t per = 4 / Ls[k];  // tiles per block iteration
     const int64_t groups = (tiles + per - 1) / per;
-    const int64_t blocks = Ls[k] == 1 ? groups : std::min<int64_t>(groups, kSplitBlocks);
+    // the LDS-word kernel loops over tile groups at every L
+    const bool looping = Ls[k] > 1 || p->wl == 0;
+    const int64_t blocks = looping ? std::min<int64_t>(groups, kSplitBlocks) : groups;
     const int64_t iters = (groups + blocks - 1) / blocks;
     const int64_t b = (blocks * p->n_comp + cus - 1) / cus;  // waves per SIMD
     const int64_t full = b / o, rem = b % o;
@@ -626,6 +774,30 @@ void launch_sample(ksim_program *p, const uint8_t *d_f, int64_t B, int32_t f_col
   if (nb > 0x7fffffff) throw std::invalid_argument("too many shots for one launch");
   const bool nz = a.noise.C > 0;
   const int L = plain_split(p, B, nz);
+  if (p->wl == 0) {
+    // parameter words in LDS: one kernel for every L, blocks looping over
+    // the tile groups
+    const int64_t groups = ((B + 63) / 64 + (4 / L) - 1) / (4 / L);
+    const int blocks = (int)std::min<int64_t>(groups, kSplitBlocks);
+    dim3 g2((unsigned)blocks, (unsigned)p->n_comp);
+    const size_t dyn = lds_words_bytes(p->wmax);
+#define KSIM_LDS_LAUNCH(N, WD)                                                             \
+  do {                                                                                     \
+    allow_lds(reinterpret_cast<const void *>(&k_sample_lds<N, WD>), dyn);                  \
+    hipLaunchKernelGGL((k_sample_lds<N, WD>), g2, dim3(kBlock), dyn, st, a, t.view(), L,    \
+                       p->wmax);                                                           \
+  } while (0)
+    if (p->all_narrow) {
+      if (nz) KSIM_LDS_LAUNCH(true, false);
+      else KSIM_LDS_LAUNCH(false, false);
+    } else {
+      if (nz) KSIM_LDS_LAUNCH(true, true);
+      else KSIM_LDS_LAUNCH(false, true);
+    }
+#undef KSIM_LDS_LAUNCH
+    hip_check(hipGetLastError(), "k_sample_lds launch");
+    return;
+  }
   if (L > 1) {
     // blocks: one per group of 4/L tiles, at most kSplitBlocks; blocks
     // loop over the groups
@@ -899,7 +1071,7 @@ static void eval_host(ksim_program *p, int32_t comp, int32_t graph, const uint8_
   if (B < 0) throw std::invalid_argument("B < 0");
   if (B == 0) return;
   const int W32 = std::max(1, (P + 31) / 32);
-  const int wl = lane_words(W32);
+  const int wl = lane_words(W32);  // 0: parameter words in LDS
   hipStream_t st = nullptr;
   ensure(p->d_params, p->params_cap, (size_t)B * P);
   if ((size_t)B * P)
@@ -930,7 +1102,13 @@ static void eval_host(ksim_program *p, int32_t comp, int32_t graph, const uint8_
     case 1: hipLaunchKernelGGL(k_eval<1>, grid, dim3(kBlock), 0, st, a, t.view()); break;
     case 2: hipLaunchKernelGGL(k_eval<2>, grid, dim3(kBlock), 0, st, a, t.view()); break;
     case 4: hipLaunchKernelGGL(k_eval<4>, grid, dim3(kBlock), 0, st, a, t.view()); break;
-    default: hipLaunchKernelGGL(k_eval<8>, grid, dim3(kBlock), 0, st, a, t.view()); break;
+    case 8: hipLaunchKernelGGL(k_eval<8>, grid, dim3(kBlock), 0, st, a, t.view()); break;
+    default: {
+      const size_t dyn = lds_words_bytes(W32);
+      allow_lds(reinterpret_cast<const void *>(&k_eval_lds), dyn);
+      hipLaunchKernelGGL(k_eval_lds, grid, dim3(kBlock), dyn, st, a, t.view(), W32);
+      break;
+    }
   }
   hip_check(hipGetLastError(), "k_eval launch");
   if (mode == 0) {
@@ -1114,7 +1292,8 @@ namespace {
 struct BatchPlan {
   std::vector<BatchProg> progs;
   std::vector<BatchItem> items;
-  int wl = 1;              // lane parameter words: the widest program's
+  int wmax = 1;            // parameter words per lane of the widest program
+  int wl = 1;              // register words of the kernel (0: words in LDS)
   bool all_narrow = true;  // every program without 128-bit graphs
 };
 
@@ -1148,7 +1327,7 @@ BatchPlan plan_batch(ksim_program *const *progs, int32_t n, const uint8_t *const
       tile_ops > 0 ? tile_ops : total_ops / (double)(cus * 5 * 4) / 8.0;
   for (int k = 0; k < n; ++k) {
     ksim_program *p = progs[k];
-    plan.wl = std::max(plan.wl, p->wl);
+    plan.wmax = std::max(plan.wmax, p->wmax);
     plan.all_narrow = plan.all_narrow && p->all_narrow;
     plan.progs[k] = BatchProg{p->d_prog, d_f[k], d_out[k], seed[k], B[k], f_cols[k],
                               p->num_outputs, p->tot_out, p->wl};
@@ -1164,6 +1343,7 @@ BatchPlan plan_batch(ksim_program *const *progs, int32_t n, const uint8_t *const
       }
     }
   }
+  plan.wl = lane_words(plan.wmax);
   // heaviest items first: the dynamic cursor then balances the tail
   std::stable_sort(items.begin(), items.end(),
                    [](const Cost &x, const Cost &y) { return x.c > y.c; });
@@ -1176,10 +1356,24 @@ BatchPlan plan_batch(ksim_program *const *progs, int32_t n, const uint8_t *const
 // One batched launch on `st`: clear the cursor and error word (a fill
 // kernel, so the launch can be captured), then the persistent kernel.
 void launch_batch(const BatchProg *d_progs, const BatchItem *d_items, uint32_t n_items,
-                  uint32_t *d_cursor, int wl, bool all_narrow, int blocks, const DevTables &t,
-                  hipStream_t st) {
+                  uint32_t *d_cursor, int wl, int wmax, bool all_narrow, int blocks,
+                  const DevTables &t, hipStream_t st) {
   hipLaunchKernelGGL(k_reset_cursor, dim3(1), dim3(64), 0, st, d_cursor, d_cursor + 1);
   hip_check(hipGetLastError(), "k_reset_cursor");
+  if (wl == 0) {
+    const size_t dyn = lds_words_bytes(wmax);
+    if (all_narrow) {
+      allow_lds(reinterpret_cast<const void *>(&k_batch_lds<false>), dyn);
+      hipLaunchKernelGGL((k_batch_lds<false>), dim3(blocks), dim3(kBlock), dyn, st, d_progs,
+                         d_items, n_items, d_cursor, d_cursor + 1, t.view(), wmax);
+    } else {
+      allow_lds(reinterpret_cast<const void *>(&k_batch_lds<true>), dyn);
+      hipLaunchKernelGGL((k_batch_lds<true>), dim3(blocks), dim3(kBlock), dyn, st, d_progs,
+                         d_items, n_items, d_cursor, d_cursor + 1, t.view(), wmax);
+    }
+    hip_check(hipGetLastError(), "k_batch_lds");
+    return;
+  }
 #define KSIM_BATCH_LAUNCH(W, WD)                                                               \
   hipLaunchKernelGGL((k_batch<W, WD>), dim3(blocks), dim3(kBlock), 0, st, d_progs, d_items,     \
                      n_items, d_cursor, d_cursor + 1, t.view())
@@ -1206,7 +1400,8 @@ struct ksim_batch {
   uint32_t n_items = 0;
   uint32_t *d_cursor = nullptr;  // [0] cursor, [1] err
   int wave_shots = 64;
-  int wl = 1;  // lane parameter words: the widest program of the batch
+  int wmax = 1;  // parameter words per lane of the widest program of the batch
+  int wl = 1;    // register words of its kernel (0: words in LDS)
   bool all_narrow = true;  // every program without 128-bit graphs
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -1240,6 +1435,7 @@ int ksim_batch_create(ksim_program *const *progs, int32_t n, const uint8_t *cons
     b->wave_shots = wave_shots;
     b->progs.assign(progs, progs + n);
     b->wl = plan.wl;
+    b->wmax = plan.wmax;
     b->all_narrow = plan.all_narrow;
     b->n_items = (uint32_t)plan.items.size();
     hip_check(hipMalloc(&b->d_progs, sizeof(BatchProg) * std::max(1, n)), "hipMalloc");
@@ -1259,8 +1455,8 @@ int ksim_batch_create(ksim_program *const *progs, int32_t n, const uint8_t *cons
 }
 
 static void batch_enqueue(ksim_batch *b, int blocks, hipStream_t st) {
-  launch_batch(b->d_progs, b->d_items, b->n_items, b->d_cursor, b->wl, b->all_narrow, blocks,
-               tables_for_current_device(), st);
+  launch_batch(b->d_progs, b->d_items, b->n_items, b->d_cursor, b->wl, b->wmax, b->all_narrow,
+               blocks, tables_for_current_device(), st);
 }
 
 int ksim_batch_run(ksim_batch *b, int32_t persistent_blocks, int32_t use_graph, void *stream) {
@@ -1398,7 +1594,7 @@ int ksim_batch_sample_static(ksim_program *const *progs, int32_t n, const uint8_
                              hipMemcpyHostToDevice, st), "H2D programs");
     hip_check(hipMemcpyAsync(di, plan.items.data(), sizeof(BatchItem) * plan.items.size(),
                              hipMemcpyHostToDevice, st), "H2D items");
-    launch_batch(dp, di, (uint32_t)plan.items.size(), dc, plan.wl, plan.all_narrow,
+    launch_batch(dp, di, (uint32_t)plan.items.size(), dc, plan.wl, plan.wmax, plan.all_narrow,
                  persistent_blocks, t, st);
     // the next call rewrites the same array: finish this launch first
     hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
@@ -1423,7 +1619,7 @@ constexpr int KSIM_SCHED_BUFS = 4;
 struct ksim_sched {
   int max_slots = 0, max_items = 0;
   std::vector<BatchProg> slots;
-  std::vector<int> slot_wl, slot_ncomp;
+  std::vector<int> slot_wmax, slot_ncomp;  // parameter words per lane, components
   std::vector<char> slot_narrow;
   // per slot and component: per-wave cost of a 64-shot tile at split 1, 2, 4
   std::vector<std::vector<std::array<double, 3>>> slot_cost;
@@ -1447,7 +1643,7 @@ int ksim_sched_create(int32_t max_slots, int32_t max_items, ksim_sched **out) {
     q->max_slots = max_slots;
     q->max_items = max_items;
     q->slots.assign(max_slots, BatchProg{nullptr, nullptr, nullptr, 0, 0, 0, 0, 0, 1});
-    q->slot_wl.assign(max_slots, 0);
+    q->slot_wmax.assign(max_slots, 0);
     q->slot_ncomp.assign(max_slots, 0);
     q->slot_narrow.assign(max_slots, 1);
     q->slot_cost.assign(max_slots, {});
@@ -1482,7 +1678,7 @@ int ksim_sched_set_slot(ksim_sched *q, int32_t slot, ksim_program *p, const uint
     if (B > 0 && ((!d_f && f_cols > 0) || !d_out)) throw std::invalid_argument("NULL buffer");
     q->slots[slot] = BatchProg{p->d_prog, d_f, d_out, seed, B, f_cols, p->num_outputs,
                                p->tot_out, p->wl};
-    q->slot_wl[slot] = p->wl;
+    q->slot_wmax[slot] = p->wmax;
     q->slot_ncomp[slot] = p->n_comp;
     q->slot_narrow[slot] = p->all_narrow ? 1 : 0;
     q->slot_cost[slot].assign(p->n_comp, {0.0, 0.0, 0.0});
@@ -1496,7 +1692,7 @@ int ksim_sched_clear_slot(ksim_sched *q, int32_t slot) {
     if (!q) throw std::invalid_argument("NULL argument");
     if (slot < 0 || slot >= q->max_slots) throw std::invalid_argument("slot out of range");
     q->slots[slot] = BatchProg{nullptr, nullptr, nullptr, 0, 0, 0, 0, 0, 1};
-    q->slot_wl[slot] = 0;
+    q->slot_wmax[slot] = 0;
     q->slot_ncomp[slot] = 0;
     q->slot_narrow[slot] = 1;
     q->slot_cost[slot].clear();
@@ -1518,7 +1714,7 @@ int ksim_sched_launch(ksim_sched *q, const int32_t *items, int32_t n, int32_t bl
     q->next = (q->next + 1) % KSIM_SCHED_BUFS;
     // the set's previous launch (possibly on another stream) must be over
     if (q->done_valid[k]) hip_check(hipEventSynchronize(q->done[k]), "hipEventSynchronize");
-    int wl = 1;
+    int wmax = 1;
     bool narrow = true;
     for (int i = 0; i < n; ++i) {
       const int32_t slot = items[5 * i], comp = items[5 * i + 1];
@@ -1533,7 +1729,7 @@ int ksim_sched_launch(ksim_sched *q, const int32_t *items, int32_t n, int32_t bl
         throw std::invalid_argument("item component index out of range");
       if (shot0 < 0 || count <= 0 || (int64_t)shot0 + count > bp.B)
         throw std::invalid_argument("item shot range outside its request");
-      wl = std::max(wl, q->slot_wl[slot]);
+      wmax = std::max(wmax, q->slot_wmax[slot]);
       narrow = narrow && q->slot_narrow[slot];
       q->h_items[k][i] = BatchItem{slot, comp, shot0, count, split};
     }
@@ -1562,23 +1758,44 @@ int ksim_sched_launch(ksim_sched *q, const int32_t *items, int32_t n, int32_t bl
                              hipMemcpyHostToDevice, st), "H2D items");
     DevTables t = tables_for_current_device();
     uint32_t *cursor = q->d_cursor + 2 * k;
-    hipLaunchKernelGGL(k_reset_cursor, dim3(1), dim3(64), 0, st, cursor, cursor + 1);
-    hip_check(hipGetLastError(), "k_reset_cursor");
+    const int wl = lane_words(wmax);
+    if (wl == 0) {
+      // a component wider than the register words: every item's parameter
+      // words in LDS
+      hipLaunchKernelGGL(k_reset_cursor, dim3(1), dim3(64), 0, st, cursor, cursor + 1);
+      hip_check(hipGetLastError(), "k_reset_cursor");
+      const size_t dyn = lds_words_bytes(wmax);
+      if (narrow) {
+        allow_lds(reinterpret_cast<const void *>(&k_batch_lds<false>), dyn);
+        hipLaunchKernelGGL((k_batch_lds<false>), dim3(blocks), dim3(kBlock), dyn, st,
+                           q->d_progs[k], q->d_items[k], (uint32_t)n, cursor, q->d_cursor + 64,
+                           t.view(), wmax);
+      } else {
+        allow_lds(reinterpret_cast<const void *>(&k_batch_lds<true>), dyn);
+        hipLaunchKernelGGL((k_batch_lds<true>), dim3(blocks), dim3(kBlock), dyn, st,
+                           q->d_progs[k], q->d_items[k], (uint32_t)n, cursor, q->d_cursor + 64,
+                           t.view(), wmax);
+      }
+      hip_check(hipGetLastError(), "k_batch_lds");
+    } else {
+      hipLaunchKernelGGL(k_reset_cursor, dim3(1), dim3(64), 0, st, cursor, cursor + 1);
+      hip_check(hipGetLastError(), "k_reset_cursor");
 #define KSIM_SCHED_LAUNCH(W, WD)                                                               \
   hipLaunchKernelGGL((k_batch<W, WD>), dim3(blocks), dim3(kBlock), 0, st, q->d_progs[k],        \
                      q->d_items[k], (uint32_t)n, cursor, q->d_cursor + 64, t.view())
 #define KSIM_SCHED_CASES(W)                        \
   if (narrow) KSIM_SCHED_LAUNCH(W, false);         \
   else KSIM_SCHED_LAUNCH(W, true);
-    switch (lane_words(wl)) {
-      case 1: KSIM_SCHED_CASES(1); break;
-      case 2: KSIM_SCHED_CASES(2); break;
-      case 4: KSIM_SCHED_CASES(4); break;
-      default: KSIM_SCHED_CASES(8); break;
-    }
+      switch (wl) {
+        case 1: KSIM_SCHED_CASES(1); break;
+        case 2: KSIM_SCHED_CASES(2); break;
+        case 4: KSIM_SCHED_CASES(4); break;
+        default: KSIM_SCHED_CASES(8); break;
+      }
 #undef KSIM_SCHED_CASES
 #undef KSIM_SCHED_LAUNCH
-    hip_check(hipGetLastError(), "k_batch");
+      hip_check(hipGetLastError(), "k_batch");
+    }
     hip_check(hipEventRecord(q->done[k], st), "hipEventRecord");
     q->done_valid[k] = true;
   });

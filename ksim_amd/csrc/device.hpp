@@ -167,15 +167,40 @@ __device__ __forceinline__ uint32_t masked_xor(const uint32_t (&pv)[WL],
   return t;
 }
 
-// -------------------------------------------------------------- one term
-// Exponents of one term for this lane's parameter words pv[0..W32):
-// the term is w^S (1+w)^V (1+sqrt2)^N pi^A pibar^B [R] 2^E, or zero when
-// bit 0 of `dead` is set.  th points at the term header.
-template <int W32, int WL, bool LONG = false>
-__device__ __forceinline__ void term_exps(const cword *__restrict__ th,
-                                          const uint32_t (&pv)[WL], int &S, int &V, int &N,
-                                          int &A, int &Bq, uint32_t &dead) {
+// The lane's parameter words as the records see them: mxor(m) is the XOR
+// over the words of (word & mask word) for a mask of width() words.
+// RegWords: the words in registers, the width fixed at compile time (the
+// programs of at most 256 parameters).  LdsWords: the words in LDS, word w
+// of the lane at pv[64 w] (a wave's words interleaved by lane, so the 64
+// lanes of one read hit 64 consecutive dwords), any width at run time.
+template <int W32, int WL>
+struct RegWords {
   static_assert(W32 <= WL, "graph wider than the lane's parameter words");
+  const uint32_t (&pv)[WL];
+  __device__ __forceinline__ int width() const { return W32; }
+  __device__ __forceinline__ uint32_t mxor(const cword *__restrict__ m) const {
+    return masked_xor<W32, WL>(pv, m);
+  }
+};
+struct LdsWords {
+  const uint32_t *pv;
+  int W32;
+  __device__ __forceinline__ int width() const { return W32; }
+  __device__ __forceinline__ uint32_t mxor(const cword *__restrict__ m) const {
+    uint32_t t = pv[0] & m[0];
+    for (int w = 1; w < W32; ++w) t = __builtin_amdgcn_bitop3_b32(m[w], t, pv[64 * w], 0x6c);
+    return t;
+  }
+};
+
+// -------------------------------------------------------------- one term
+// Exponents of one term for this lane's parameter words pw: the term is
+// w^S (1+w)^V (1+sqrt2)^N pi^A pibar^B [R] 2^E, or zero when bit 0 of
+// `dead` is set.  th points at the term header.
+template <class PW, bool LONG = false>
+__device__ __forceinline__ void term_exps(const cword *__restrict__ th, const PW &pw, int &S,
+                                          int &V, int &N, int &A, int &Bq, uint32_t &dead) {
+  const int W32 = pw.width();
   const uint32_t nproj = th[TH_NPROJ], nppx = th[TH_NPPX], nchunk = th[TH_NCHUNK];
   S = (int)th[TH_S]; V = (int)th[TH_V]; N = (int)th[TH_N];
   A = (int)th[TH_A]; Bq = (int)th[TH_B];
@@ -183,7 +208,7 @@ __device__ __forceinline__ void term_exps(const cword *__restrict__ th,
   // PROJ: the term vanishes unless every projector parity matches
   dead = 0;
   for (uint32_t k = 0; k < nproj; ++k, rp += W32 + 1) {
-    dead |= rp[W32] + __popc(masked_xor<W32>(pv, rp));
+    dead |= rp[W32] + __popc(pw.mxor(rp));
   }
   // PPX: (-1)^{psi phi}.  LONG (the kernels with 128-bit sums, which serve
   // the programs with long terms): unrolled by 4, which amortises the loop's
@@ -193,14 +218,14 @@ __device__ __forceinline__ void term_exps(const cword *__restrict__ th,
   if constexpr (LONG) {
 #pragma unroll 4
     for (uint32_t k = 0; k < nppx; ++k, rp += 2 * W32 + 2) {
-      const uint32_t cs = rp[2 * W32] + __popc(masked_xor<W32>(pv, rp));
-      const uint32_t cf = rp[2 * W32 + 1] + __popc(masked_xor<W32>(pv, rp + W32));
+      const uint32_t cs = rp[2 * W32] + __popc(pw.mxor(rp));
+      const uint32_t cf = rp[2 * W32 + 1] + __popc(pw.mxor(rp + W32));
       ex ^= cs & cf;
     }
   } else {
     for (uint32_t k = 0; k < nppx; ++k, rp += 2 * W32 + 2) {
-      const uint32_t cs = rp[2 * W32] + __popc(masked_xor<W32>(pv, rp));
-      const uint32_t cf = rp[2 * W32 + 1] + __popc(masked_xor<W32>(pv, rp + W32));
+      const uint32_t cs = rp[2 * W32] + __popc(pw.mxor(rp));
+      const uint32_t cf = rp[2 * W32 + 1] + __popc(pw.mxor(rp + W32));
       ex ^= cs & cf;
     }
   }
@@ -214,28 +239,28 @@ __device__ __forceinline__ void term_exps(const cword *__restrict__ th,
     if constexpr (LONG) {
 #pragma unroll 8
       for (uint32_t k = 0; k < nl; ++k, rp += W32 + 1) {
-        const uint32_t c = __popc(masked_xor<W32>(pv, rp));
+        const uint32_t c = __popc(pw.mxor(rp));
         x += (0u - (c & 1u)) & rp[W32];
       }
     } else {
 #pragma unroll 4
       for (uint32_t k = 0; k < nl; ++k, rp += W32 + 1) {
-        const uint32_t c = __popc(masked_xor<W32>(pv, rp));
+        const uint32_t c = __popc(pw.mxor(rp));
         x += (0u - (c & 1u)) & rp[W32];
       }
     }
     if constexpr (LONG) {
 #pragma unroll 2
       for (uint32_t k = 0; k < npr; ++k, rp += 2 * W32 + 3) {
-        const uint32_t ca = __popc(masked_xor<W32>(pv, rp));
-        const uint32_t cb = __popc(masked_xor<W32>(pv, rp + W32));
+        const uint32_t ca = __popc(pw.mxor(rp));
+        const uint32_t cb = __popc(pw.mxor(rp + W32));
         const uint32_t ma = 0u - (ca & 1u), mb = 0u - (cb & 1u);
         x += (ma & rp[2 * W32]) + (mb & rp[2 * W32 + 1]) + (ma & mb & rp[2 * W32 + 2]);
       }
     } else {
       for (uint32_t k = 0; k < npr; ++k, rp += 2 * W32 + 3) {
-        const uint32_t ca = __popc(masked_xor<W32>(pv, rp));
-        const uint32_t cb = __popc(masked_xor<W32>(pv, rp + W32));
+        const uint32_t ca = __popc(pw.mxor(rp));
+        const uint32_t cb = __popc(pw.mxor(rp + W32));
         const uint32_t ma = 0u - (ca & 1u), mb = 0u - (cb & 1u);
         x += (ma & rp[2 * W32]) + (mb & rp[2 * W32 + 1]) + (ma & mb & rp[2 * W32 + 2]);
       }
@@ -394,9 +419,9 @@ __device__ __forceinline__ void graph_sum_clear(GraphSum &g, uint32_t gflags) {
 // Add terms [0, nt) starting at program dword `th` into g, by the graph's
 // flags: a single table magnitude (GF_MONO), 64-bit sums (GF_NARROW) or
 // 128-bit sums.
-template <int W32, int WL, bool WIDE = true>
+template <class PW, bool WIDE = true>
 __device__ __forceinline__ void sum_terms(const cword *__restrict__ th, uint32_t nt,
-                                          uint32_t gflags, int e0, const uint32_t (&pv)[WL],
+                                          uint32_t gflags, int e0, const PW &pw,
                                           const LdsTables *lt, const Tables &tb, GraphSum &g,
                                           uint32_t &err) {
   for (uint32_t t = 0; t < nt; ++t) {
@@ -404,7 +429,7 @@ __device__ __forceinline__ void sum_terms(const cword *__restrict__ th, uint32_t
     const int E = (int)th[TH_E];
     int S, V, N, A, Bq;
     uint32_t dead;
-    term_exps<W32, WL, WIDE>(th, pv, S, V, N, A, Bq, dead);
+    term_exps<PW, WIDE>(th, pw, S, V, N, A, Bq, dead);
     const TermIdx ti = term_index(S, V, N);
     const bool vanished = dead & 1u;
     const int sh = ti.q + E;
@@ -528,12 +553,11 @@ __device__ __forceinline__ void exchange_sums(GraphSum &g, bool narrow, const Sp
 // value magnitude() gives on the exact term sum.  SPLIT: the graph's terms
 // are shared by the waves of sp's group (single-term graphs are evaluated
 // by every wave; they need no exchange).
-template <int W32, int WL, bool SPLIT, bool WIDE = true>
+template <class PW, bool SPLIT, bool WIDE = true>
 __device__ __forceinline__ double graph_prob(const cword *__restrict__ P,
-                                             const cword *__restrict__ gh,
-                                             const uint32_t (&pv)[WL], const LdsTables *lt,
-                                             const Tables &tb, uint32_t &err, const Split &sp,
-                                             uint32_t &xs) {
+                                             const cword *__restrict__ gh, const PW &pw,
+                                             const LdsTables *lt, const Tables &tb, uint32_t &err,
+                                             const Split &sp, uint32_t &xs) {
   const uint32_t gflags = gh[GH_FLAGS];
   const int e0 = (int)gh[GH_E0];
   const uint32_t nt = gh[GH_NTERMS];
@@ -550,7 +574,7 @@ __device__ __forceinline__ double graph_prob(const cword *__restrict__ P,
   }
   GraphSum g;
   graph_sum_clear(g, gflags);
-  sum_terms<W32, WL, WIDE>(P + off, t1 - t0, gflags, e0, pv, lt, tb, g, err);
+  sum_terms<PW, WIDE>(P + off, t1 - t0, gflags, e0, pw, lt, tb, g, err);
   if (gflags & GF_MONO) return g.pm;
   const bool narrow = !WIDE || (gflags & GF_NARROW);
   if (share) exchange_sums(g, narrow, sp, xs);
@@ -567,7 +591,9 @@ __device__ __forceinline__ double graph_prob_any(const cword *__restrict__ P,
   switch (gh[GH_W32]) {
 #define KSIM_GP(w)                                                                       \
   case w:                                                                                \
-    if constexpr (WL >= w) return graph_prob<w, WL, SPLIT, WIDE>(P, gh, pv, lt, tb, err, sp, xs); \
+    if constexpr (WL >= w)                                                               \
+      return graph_prob<RegWords<w, WL>, SPLIT, WIDE>(P, gh, RegWords<w, WL>{pv}, lt, tb, err, \
+                                                      sp, xs);                           \
     break;
     KSIM_GP(1) KSIM_GP(2) KSIM_GP(3) KSIM_GP(4) KSIM_GP(5) KSIM_GP(6) KSIM_GP(7) KSIM_GP(8)
 #undef KSIM_GP
@@ -579,14 +605,14 @@ __device__ __forceinline__ double graph_prob_any(const cword *__restrict__ P,
 // Exact amplitude of one graph for explicit parameter words (the exact
 // path of k_eval): acc = sum_terms value * 2^{-e0} in 128 bits, every term
 // through the general path whatever the graph's flags.
-template <int W32, int WL>
+template <class PW>
 __device__ __forceinline__ void eval_graph(const cword *__restrict__ P,
-                                           const cword *__restrict__ gh,
-                                           const uint32_t (&pv)[WL], const LdsTables *lt,
-                                           const Tables &tb, Acc4 &acc, uint32_t &err) {
+                                           const cword *__restrict__ gh, const PW &pw,
+                                           const LdsTables *lt, const Tables &tb, Acc4 &acc,
+                                           uint32_t &err) {
   GraphSum g;
   graph_sum_clear(g, 0u);
-  sum_terms<W32, WL>(P + gh[GH_TERM_OFF], gh[GH_NTERMS], 0u, (int)gh[GH_E0], pv, lt, tb, g, err);
+  sum_terms<PW>(P + gh[GH_TERM_OFF], gh[GH_NTERMS], 0u, (int)gh[GH_E0], pw, lt, tb, g, err);
   acc = g.acc;
 }
 
@@ -596,14 +622,14 @@ __device__ __forceinline__ void eval_graph_any(const cword *__restrict__ P,
                                                const uint32_t (&pv)[WL], const LdsTables *lt,
                                                const Tables &tb, Acc4 &acc, uint32_t &err) {
   switch (gh[GH_W32]) {
-    case 1: eval_graph<1, WL>(P, gh, pv, lt, tb, acc, err); break;
-    case 2: if constexpr (WL >= 2) eval_graph<2, WL>(P, gh, pv, lt, tb, acc, err); break;
-    case 3: if constexpr (WL >= 3) eval_graph<3, WL>(P, gh, pv, lt, tb, acc, err); break;
-    case 4: if constexpr (WL >= 4) eval_graph<4, WL>(P, gh, pv, lt, tb, acc, err); break;
-    case 5: if constexpr (WL >= 5) eval_graph<5, WL>(P, gh, pv, lt, tb, acc, err); break;
-    case 6: if constexpr (WL >= 6) eval_graph<6, WL>(P, gh, pv, lt, tb, acc, err); break;
-    case 7: if constexpr (WL >= 7) eval_graph<7, WL>(P, gh, pv, lt, tb, acc, err); break;
-    case 8: if constexpr (WL >= 8) eval_graph<8, WL>(P, gh, pv, lt, tb, acc, err); break;
+    case 1: eval_graph(P, gh, RegWords<1, WL>{pv}, lt, tb, acc, err); break;
+    case 2: if constexpr (WL >= 2) eval_graph(P, gh, RegWords<2, WL>{pv}, lt, tb, acc, err); break;
+    case 3: if constexpr (WL >= 3) eval_graph(P, gh, RegWords<3, WL>{pv}, lt, tb, acc, err); break;
+    case 4: if constexpr (WL >= 4) eval_graph(P, gh, RegWords<4, WL>{pv}, lt, tb, acc, err); break;
+    case 5: if constexpr (WL >= 5) eval_graph(P, gh, RegWords<5, WL>{pv}, lt, tb, acc, err); break;
+    case 6: if constexpr (WL >= 6) eval_graph(P, gh, RegWords<6, WL>{pv}, lt, tb, acc, err); break;
+    case 7: if constexpr (WL >= 7) eval_graph(P, gh, RegWords<7, WL>{pv}, lt, tb, acc, err); break;
+    case 8: if constexpr (WL >= 8) eval_graph(P, gh, RegWords<8, WL>{pv}, lt, tb, acc, err); break;
     default: break;
   }
 }
@@ -928,7 +954,8 @@ __device__ __forceinline__ int run_graphs(const ShotArgs &a, const cword *__rest
     if ((int)gh[GH_W32] != W32) break;
     const uint32_t m = 1u << ((F + i - 1) & 31);
     if (i > 0) pv[W32 - 1] |= m;  // trying bit
-    const double p1 = graph_prob<W32, WL, SPLIT, WIDE>(P, gh, pv, lt, tb, err, sp, xs);
+    const double p1 =
+        graph_prob<RegWords<W32, WL>, SPLIT, WIDE>(P, gh, RegWords<W32, WL>{pv}, lt, tb, err, sp, xs);
     if (i == 0) {
       prev = p1;
       continue;
@@ -1123,6 +1150,129 @@ __device__ __forceinline__ void run_shot(const ShotArgs &a, int comp, int64_t s,
       const int i0 = max(32 * w - F, 0), i1 = min(32 * w + 32 - F, nout);
       for (int i = i0; i < i1; ++i) orow[dest[i]] = (uint8_t)((pv[w] >> ((F + i) & 31)) & 1u);
     }
+  }
+  if (err) atomicOr(a.err, err);
+}
+
+// ------------------------------------------------ parameter words in LDS
+// A component of more than 32 * REG_WORDS_MAX parameters keeps each lane's
+// parameter words in LDS: the wave's area holds CH_WMAX words per lane,
+// word w of the lane at pvl[64 w] (pvl = area + lane).  Only the lane
+// itself reads and writes its words, so no barrier orders them.
+
+__device__ __forceinline__ uint32_t lds_bit(const uint32_t *pvl, int pos) {
+  return (pvl[64 * (pos >> 5)] >> (pos & 31)) & 1u;
+}
+
+__device__ __forceinline__ void lds_set_bit(uint32_t *pvl, int pos, uint32_t bit) {
+  uint32_t &w = pvl[64 * (pos >> 5)];
+  const uint32_t m = 1u << (pos & 31);
+  w = (w & ~m) | (m & (0u - (bit & 1u)));
+}
+
+// 32 bits of the lane's nw words from bit `off` (may be negative or run
+// past the words: missing bits read as 0)
+__device__ __forceinline__ uint32_t lds_bits32(const uint32_t *pvl, int off, int nw) {
+  const int w0 = off >> 5, r = off & 31;
+  const uint32_t lo = (w0 >= 0 && w0 < nw) ? pvl[64 * w0] : 0u;
+  const uint32_t hi = (w0 + 1 >= 0 && w0 + 1 < nw) ? pvl[64 * (w0 + 1)] : 0u;
+  return r ? (lo >> r) | (hi << (32 - r)) : lo;
+}
+
+// The autoregressive loop of one shot of component `comp` with the lane's
+// parameter words in LDS at pvl (run_shot's contract otherwise: SPLIT
+// shares the graphs' terms over sp's group, only `valid` lanes of the
+// group's first wave write).
+template <bool NOISE, bool SPLIT, bool WIDE>
+__device__ __forceinline__ void run_shot_lds(const ShotArgs &a, int comp, int64_t s,
+                                             const LdsTables *lt, const Tables &tb, uint32_t *pvl,
+                                             const Split &sp, bool valid, uint32_t *xs_io) {
+  const cword *__restrict__ P = as_const(a.prog);
+  const cword *__restrict__ ch = P + P[PH_DWORDS + comp];
+  const int ng = (int)ch[CH_NGRAPHS], F = (int)ch[CH_F], nw = (int)ch[CH_WMAX];
+  const int colbase = (int)ch[CH_COLBASE];
+  const cword *__restrict__ fsel = P + ch[CH_FSEL_OFF];
+  const cword *__restrict__ dest = P + ch[CH_DEST_OFF];
+  const cword *__restrict__ ghs = P + ch[CH_GRAPH_OFF];
+  const uint32_t fsel0 = ch[CH_FSEL0];
+  if constexpr (NOISE) {
+    for (int w = 0; w < nw; ++w) pvl[64 * w] = 0u;
+    noise_walk(a.noise, noise_stream(a.noise.seed, (uint64_t)(a.shot_base + s)),
+               [&](int, uint32_t row) {
+                 const uint32_t *pr = a.noise.pat + row;
+                 for (int j = 0; j < F; ++j) {
+                   const uint32_t b = fsel[j];
+                   pvl[64 * (j >> 5)] ^= ((pr[b >> 5] >> (b & 31)) & 1u) << (j & 31);
+                 }
+               });
+  } else {
+    const uint8_t *frow = a.f + s * (int64_t)a.num_f;
+    const bool aligned = fsel0 != 0xffffffffu &&
+                         (((uint32_t)a.num_f | fsel0 | (uint32_t)(uintptr_t)a.f) & 3u) == 0;
+    for (int w = 0; w < nw; ++w) {
+      uint32_t word = 0;
+      const int j0 = 32 * w, jn = min(32, F - j0);
+      if (aligned && jn == 32) {
+        // 32 contiguous f bytes in two 16-byte loads, bit 0 of each byte
+        const U32x4 q0 = *(const U32x4 *)(frow + fsel0 + j0);
+        const U32x4 q1 = *(const U32x4 *)(frow + fsel0 + j0 + 16);
+        word = low_bits16(q0) | (low_bits16(q1) << 16);
+      } else {
+        for (int jj = 0; jj < jn; ++jj) word |= (uint32_t)(frow[fsel[j0 + jj]] & 1u) << jj;
+      }
+      pvl[64 * w] = word;
+    }
+  }
+  uint64_t rs = rng_seed(a.seed + 0x9E37ull * (uint64_t)comp, (uint64_t)(a.shot_base + s));
+  uint32_t err = 0;
+  uint32_t xs = xs_io ? *xs_io : 0u;
+  double prev = 0.0;
+  // graph i > 0 runs with its trying bit (parameter F + i - 1) at 1
+  if (ng > 1) lds_set_bit(pvl, F, 1u);
+  for (int i = 0; i < ng; ++i) {
+    const cword *gh = ghs + (size_t)i * GH_DWORDS;
+    const double p1 =
+        graph_prob<LdsWords, SPLIT, WIDE>(P, gh, LdsWords{pvl, (int)gh[GH_W32]}, lt, tb, err, sp, xs);
+    if (i == 0) {
+      prev = p1;
+      continue;
+    }
+    const double u = a.U ? a.U[s * (int64_t)a.tot_out + colbase + i - 1] : rng_uniform(rs);
+    const uint32_t bit = (u < __ddiv_rn(p1, prev)) ? 1u : 0u;
+    lds_set_bit(pvl, F + i - 1, bit);
+    if (i + 1 < ng) lds_set_bit(pvl, F + i, 1u);
+    prev = bit ? p1 : __dsub_rn(prev, p1);
+  }
+  if (xs_io) *xs_io = xs;
+  if (!valid || (SPLIT && sp.part != 0)) {
+    if (err) atomicOr(a.err, err);
+    return;
+  }
+  const int nout = ng - 1;
+  if (a.out_words > 0) {
+    uint32_t *orow = (uint32_t *)a.out + s * (int64_t)a.out_words;
+    const uint32_t dest0 = ch[CH_DEST0];
+    if (dest0 != 0xffffffffu) {
+      const int k0 = (int)dest0 >> 5, k1 = ((int)dest0 + nout - 1) >> 5;
+      for (int k = k0; k <= k1; ++k) {
+        uint32_t v = lds_bits32(pvl, F - (int)dest0 + 32 * k, nw);
+        const int lo = max((int)dest0 - 32 * k, 0), hi = min((int)dest0 + nout - 32 * k, 32);
+        v &= (hi >= 32 ? 0xffffffffu : ((1u << hi) - 1u)) & ~((1u << lo) - 1u);
+        if (a.out_atomic) {
+          if (v) atomicOr(orow + k, v);
+        } else {
+          orow[k] = v;
+        }
+      }
+    } else {
+      for (int i = 0; i < nout; ++i) {
+        const uint32_t c = dest[i];
+        if (lds_bit(pvl, F + i)) atomicOr(orow + (c >> 5), 1u << (c & 31));
+      }
+    }
+  } else {
+    uint8_t *orow = a.out + s * (int64_t)a.num_outputs;
+    for (int i = 0; i < nout; ++i) orow[dest[i]] = (uint8_t)lds_bit(pvl, F + i);
   }
   if (err) atomicOr(a.err, err);
 }

@@ -50,6 +50,12 @@ constexpr int MAG_LDS_N = (2 * MAG_LDS_R + 1) * 4 * 8;
 // pi^a pibar^b for a, b < PI_MAX (int32 coefficients, < 2^24)
 constexpr int PI_MAX = 16;
 
+// 32-bit parameter words a lane holds for one shot: up to REG_WORDS_MAX in
+// registers (kernels specialised on 1, 2, 4 or 8 words), wider components
+// (at most LANE_WORDS_MAX words, F + n_out <= 4096 parameters) in LDS
+constexpr int REG_WORDS_MAX = 8;
+constexpr int LANE_WORDS_MAX = 128;
+
 // term header (dwords)
 enum TermHdr : int {
   TH_NPROJ = 0, TH_NPPX, TH_NCHUNK, TH_FLAGS,

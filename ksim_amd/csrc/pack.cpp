@@ -545,8 +545,9 @@ std::vector<uint32_t> pack_program(const ksim_graph_view *graphs, const int32_t 
                                       std::to_string(c) + " has P=" + std::to_string(g.P) +
                                       ", expected F+i=" + std::to_string(F + i));
       }
-      if (std::max(1, (F + ng - 1 + 31) / 32) > 8)
-        throw std::invalid_argument("more than 256 parameters per component");
+      if (std::max(1, (F + ng - 1 + 31) / 32) > LANE_WORDS_MAX)
+        throw std::invalid_argument("more than " + std::to_string(32 * LANE_WORDS_MAX) +
+                                    " parameters (F + outputs) in one component");
       gi += ng;
       fi += F;
     }

@@ -61,7 +61,7 @@ void validate_words(const uint32_t *w, size_t n) {
   need(PH_DWORDS, "program header");
   const uint32_t nc = w[PH_NCOMP], num_f = w[PH_NUMF], num_out = w[PH_NUMOUT];
   const uint32_t tot = w[PH_TOTOUT], wmax = w[PH_WMAX];
-  if (wmax < 1 || wmax > 8) bad("lane words outside 1..8");
+  if (wmax < 1 || wmax > (uint32_t)LANE_WORDS_MAX) bad("lane words outside 1..LANE_WORDS_MAX");
   need((uint64_t)PH_DWORDS + nc, "component offsets");
   uint64_t outs = 0;
   for (uint32_t c = 0; c < nc; ++c) {

@@ -88,6 +88,28 @@ def test_wide_params(tables):
     check_program(prog, tables, rows=4)
 
 
+@pytest.mark.parametrize("F,n_out", [(250, 7), (300, 3), (1000, 2)])
+def test_components_wider_than_the_register_words(tables, F, n_out):
+    # F + n_out past 256 parameters: masks of 9 to 32 words, the layout the
+    # kernels that keep parameter words in LDS walk
+    prog = synth.shape_program(n_q=6, t=4, F=F, n_out=n_out, seed=F)
+    assert check_program(prog, tables, rows=3, seed=1) > 0
+    st = {}
+    w = pack_words(prog, st)
+    assert int(w[4]) == (F + n_out + 31) // 32  # PH_WMAX
+
+
+def test_rejects_more_than_4096_parameters():
+    from ksim_amd import _lib
+    import ctypes as C
+    from ksim_amd.sampler import ProgramArgs
+    prog = synth.shape_program(n_q=2, t=0, F=4090, n_out=7, seed=1)
+    pa = ProgramArgs(prog)
+    n = C.c_int64()
+    with pytest.raises(ValueError, match="4096"):
+        _lib.check(_lib.load().ksim_program_pack(*pa.args(), None, 0, C.byref(n), None))
+
+
 def test_all_phase_pairs(tables):
     # one term per (alpha, beta), masks on both, one, or neither parity
     G = 64 * 4

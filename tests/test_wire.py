@@ -96,7 +96,7 @@ def structural_cases(w):
         v[idx] = val
         out.append((name, v))
 
-    mut("lane words", 4, 9)
+    mut("lane words", 4, 129)
     mut("component offset", PH, len(w) + 5)
     mut("graph count", c0 + 0, 0)
     mut("f_selection", fsel, int(w[1]) + 3)

@@ -175,6 +175,14 @@ int ksim_program_set_tiles(ksim_program *p, int32_t block_threads,
 /* Error word of the last launch (0 = none), for callers of the async API. */
 int ksim_program_check(ksim_program *p, void *stream);
 
+/* 1 when every graph of the program is exact, 0 when some graph carries
+ * approximate (complex64) floatfactors (pf_has_approx,
+ * src/cpp/ksim/kokkos_sim.cpp:123-129): such a graph's terms are exact in
+ * Z[w] but are cast to float64, multiplied by their factors and summed in
+ * float64 in term order, as the reference does (zx_eval.hpp:168-178), and
+ * ksim_program_eval_exact rejects it.  -1 for NULL. */
+int32_t ksim_program_exact(const ksim_program *p);
+
 /* Bytes of device memory the program holds. */
 int64_t ksim_program_device_bytes(const ksim_program *p);
 
@@ -189,7 +197,8 @@ void ksim_program_destroy(ksim_program *p);
 
 /* ---- wire format: a packed program as one validated blob ---------------- */
 
-/* Program blob layout, version 3 (little-endian): "KSIMPRG\0", u32 version,
+/* Program blob layout, version 4 (little-endian; version 3 blobs are read
+ * too): "KSIMPRG\0", u32 version,
  * u32 header bytes (128), u64 word count, u64 FNV-1a 64 of the words,
  * 8 f64 stats (ksim_program_stats order), u32 n_comp / num_f / num_outputs /
  * total outputs / lane words, 12 zero bytes, then the u32 program words.

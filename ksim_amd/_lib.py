@@ -59,6 +59,7 @@ EXPORTS = {
                                           C.POINTER(C.c_double)]),
     "ksim_program_check": (C.c_int, [vp, vp]),
     "ksim_program_device_bytes": (C.c_int64, [vp]),
+    "ksim_program_exact": (C.c_int32, [vp]),
     "ksim_program_stats": (C.c_int, [vp, f64p]),
     "ksim_program_destroy": (None, [vp]),
     "ksim_batch_create": (C.c_int, [C.POINTER(vp), C.c_int32, C.POINTER(vp), i32p, i64p,

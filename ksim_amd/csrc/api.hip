@@ -489,6 +489,7 @@ struct ksim_program {
   bool packed_full_rows = false;  // one component writes whole packed rows in order
   int term_split = 0;  // waves per 64-shot tile in plain launches; 0: chosen per launch
   bool all_narrow = false;  // every graph GF_MONO or GF_NARROW: kernels without 128-bit sums
+  bool approx = false;      // some graph has approximate floatfactors (GF_APPROX)
   // staging for the host-pointer API
   uint8_t *d_f = nullptr;
   size_t f_cap = 0;
@@ -515,7 +516,8 @@ void comp_work(const ksim_program *p, int c, double w[3]) {
     const uint32_t *h = &gh[(size_t)i * GH_DWORDS];
     const double body = (double)h[GH_NREC] + 40.0 * h[GH_NTERMS];
     w[0] += body + 60.0;
-    const bool mono = h[GH_FLAGS] & GF_MONO;
+    // single-term and approximate graphs are evaluated whole by every wave
+    const bool mono = h[GH_FLAGS] & (GF_MONO | GF_APPROX);
     w[1] += mono ? body + 60.0 : body / 2 + 80.0;
     w[2] += mono ? body + 60.0 : body / 4 + 80.0;
   }
@@ -884,8 +886,12 @@ ksim_program *upload_program(const std::vector<uint32_t> &words, const PackStats
     }
     p->all_narrow = true;
     for (const auto &gh : p->graph_hdr)
-      for (size_t i = 0; i + GH_DWORDS <= gh.size(); i += GH_DWORDS)
+      for (size_t i = 0; i + GH_DWORDS <= gh.size(); i += GH_DWORDS) {
         if (!(gh[i + GH_FLAGS] & (GF_MONO | GF_NARROW))) p->all_narrow = false;
+        if (gh[i + GH_FLAGS] & GF_APPROX) p->approx = true;
+      }
+    // approximate graphs have their float path only in the LDS-word kernels
+    if (p->approx) p->wl = 0;
     tables_for_current_device();
   } catch (...) {
     ksim_program_destroy(p);
@@ -1071,7 +1077,10 @@ static void eval_host(ksim_program *p, int32_t comp, int32_t graph, const uint8_
   if (B < 0) throw std::invalid_argument("B < 0");
   if (B == 0) return;
   const int W32 = std::max(1, (P + 31) / 32);
-  const int wl = lane_words(W32);  // 0: parameter words in LDS
+  const bool approx = p->graph_hdr[comp][(size_t)graph * GH_DWORDS + GH_FLAGS] & GF_APPROX;
+  if (approx && mode == 0)
+    throw std::runtime_error("the graph has approximate floatfactors: its value is not exact");
+  const int wl = approx ? 0 : lane_words(W32);  // 0: parameter words in LDS
   hipStream_t st = nullptr;
   ensure(p->d_params, p->params_cap, (size_t)B * P);
   if ((size_t)B * P)
@@ -1148,6 +1157,10 @@ int ksim_program_set_tiles(ksim_program *p, int32_t block_threads, int32_t term_
     p->block_threads = block_threads;
     p->term_split = term_split;
   });
+}
+
+int32_t ksim_program_exact(const ksim_program *p) {
+  return p ? (p->approx ? 0 : 1) : -1;
 }
 
 int64_t ksim_program_device_bytes(const ksim_program *p) {
@@ -1295,6 +1308,7 @@ struct BatchPlan {
   int wmax = 1;            // parameter words per lane of the widest program
   int wl = 1;              // register words of the kernel (0: words in LDS)
   bool all_narrow = true;  // every program without 128-bit graphs
+  bool approx = false;     // some program has approximate graphs (LDS-word kernel)
 };
 
 BatchPlan plan_batch(ksim_program *const *progs, int32_t n, const uint8_t *const *d_f,
@@ -1329,6 +1343,7 @@ BatchPlan plan_batch(ksim_program *const *progs, int32_t n, const uint8_t *const
     ksim_program *p = progs[k];
     plan.wmax = std::max(plan.wmax, p->wmax);
     plan.all_narrow = plan.all_narrow && p->all_narrow;
+    plan.approx = plan.approx || p->approx;
     plan.progs[k] = BatchProg{p->d_prog, d_f[k], d_out[k], seed[k], B[k], f_cols[k],
                               p->num_outputs, p->tot_out, p->wl};
     for (int c = 0; c < p->n_comp; ++c) {
@@ -1343,7 +1358,7 @@ BatchPlan plan_batch(ksim_program *const *progs, int32_t n, const uint8_t *const
       }
     }
   }
-  plan.wl = lane_words(plan.wmax);
+  plan.wl = plan.approx ? 0 : lane_words(plan.wmax);
   // heaviest items first: the dynamic cursor then balances the tail
   std::stable_sort(items.begin(), items.end(),
                    [](const Cost &x, const Cost &y) { return x.c > y.c; });
@@ -1620,6 +1635,7 @@ struct ksim_sched {
   int max_slots = 0, max_items = 0;
   std::vector<BatchProg> slots;
   std::vector<int> slot_wmax, slot_ncomp;  // parameter words per lane, components
+  std::vector<char> slot_approx;            // program with approximate graphs
   std::vector<char> slot_narrow;
   // per slot and component: per-wave cost of a 64-shot tile at split 1, 2, 4
   std::vector<std::vector<std::array<double, 3>>> slot_cost;
@@ -1644,6 +1660,7 @@ int ksim_sched_create(int32_t max_slots, int32_t max_items, ksim_sched **out) {
     q->max_items = max_items;
     q->slots.assign(max_slots, BatchProg{nullptr, nullptr, nullptr, 0, 0, 0, 0, 0, 1});
     q->slot_wmax.assign(max_slots, 0);
+    q->slot_approx.assign(max_slots, 0);
     q->slot_ncomp.assign(max_slots, 0);
     q->slot_narrow.assign(max_slots, 1);
     q->slot_cost.assign(max_slots, {});
@@ -1679,6 +1696,7 @@ int ksim_sched_set_slot(ksim_sched *q, int32_t slot, ksim_program *p, const uint
     q->slots[slot] = BatchProg{p->d_prog, d_f, d_out, seed, B, f_cols, p->num_outputs,
                                p->tot_out, p->wl};
     q->slot_wmax[slot] = p->wmax;
+    q->slot_approx[slot] = p->approx ? 1 : 0;
     q->slot_ncomp[slot] = p->n_comp;
     q->slot_narrow[slot] = p->all_narrow ? 1 : 0;
     q->slot_cost[slot].assign(p->n_comp, {0.0, 0.0, 0.0});
@@ -1693,6 +1711,7 @@ int ksim_sched_clear_slot(ksim_sched *q, int32_t slot) {
     if (slot < 0 || slot >= q->max_slots) throw std::invalid_argument("slot out of range");
     q->slots[slot] = BatchProg{nullptr, nullptr, nullptr, 0, 0, 0, 0, 0, 1};
     q->slot_wmax[slot] = 0;
+    q->slot_approx[slot] = 0;
     q->slot_ncomp[slot] = 0;
     q->slot_narrow[slot] = 1;
     q->slot_cost[slot].clear();
@@ -1715,7 +1734,7 @@ int ksim_sched_launch(ksim_sched *q, const int32_t *items, int32_t n, int32_t bl
     // the set's previous launch (possibly on another stream) must be over
     if (q->done_valid[k]) hip_check(hipEventSynchronize(q->done[k]), "hipEventSynchronize");
     int wmax = 1;
-    bool narrow = true;
+    bool narrow = true, approx = false;
     for (int i = 0; i < n; ++i) {
       const int32_t slot = items[5 * i], comp = items[5 * i + 1];
       const int32_t shot0 = items[5 * i + 2], count = items[5 * i + 3];
@@ -1730,6 +1749,7 @@ int ksim_sched_launch(ksim_sched *q, const int32_t *items, int32_t n, int32_t bl
       if (shot0 < 0 || count <= 0 || (int64_t)shot0 + count > bp.B)
         throw std::invalid_argument("item shot range outside its request");
       wmax = std::max(wmax, q->slot_wmax[slot]);
+      approx = approx || q->slot_approx[slot];
       narrow = narrow && q->slot_narrow[slot];
       q->h_items[k][i] = BatchItem{slot, comp, shot0, count, split};
     }
@@ -1758,10 +1778,10 @@ int ksim_sched_launch(ksim_sched *q, const int32_t *items, int32_t n, int32_t bl
                              hipMemcpyHostToDevice, st), "H2D items");
     DevTables t = tables_for_current_device();
     uint32_t *cursor = q->d_cursor + 2 * k;
-    const int wl = lane_words(wmax);
+    const int wl = approx ? 0 : lane_words(wmax);
     if (wl == 0) {
-      // a component wider than the register words: every item's parameter
-      // words in LDS
+      // a component wider than the register words, or a program with
+      // approximate graphs: every item's parameter words in LDS
       hipLaunchKernelGGL(k_reset_cursor, dim3(1), dim3(64), 0, st, cursor, cursor + 1);
       hip_check(hipGetLastError(), "k_reset_cursor");
       const size_t dyn = lds_words_bytes(wmax);

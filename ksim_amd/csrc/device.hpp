@@ -79,6 +79,30 @@ __device__ inline double magnitude(const Acc4 &v, int e) {
   return __builtin_ldexp(__dsqrt_rn(n2), e + k);
 }
 
+// float64 (re, im) of an exact value (a + b w + c i + d conj(w)) * 2^e by the
+// first steps of magnitude(): the same 62-bit window, re = dbl(a') +
+// dbl(s') * SQ, im = dbl(c') + dbl(t') * SQ, then scaled by 2^(e + k).  For
+// coefficients below 2^53 this is the reference's cast of one term
+// (zx_eval.hpp:168-171: dbl(b) + dbl(d) is dbl(b + d) there) whatever the
+// representation, since scaling by 2 commutes with every step.
+__device__ inline void term_float(const Acc4 &v, int e, double &re, double &im) {
+#pragma clang fp contract(off)
+  const double SQ = 0.70710678118654752440;
+  const i128_t s = v.b + v.d, t = v.b - v.d;
+  int64_t a_, c_, s_, t_;
+  int k = 0;
+  if (fits62(v.a) && fits62(v.c) && fits62(s) && fits62(t)) {
+    a_ = (int64_t)v.a; c_ = (int64_t)v.c; s_ = (int64_t)s; t_ = (int64_t)t;
+  } else {
+    const int L = max(max(bitlen128(v.a), bitlen128(v.c)), max(bitlen128(s), bitlen128(t)));
+    k = L - 62;
+    a_ = (int64_t)(v.a >> k); c_ = (int64_t)(v.c >> k);
+    s_ = (int64_t)(s >> k); t_ = (int64_t)(t >> k);
+  }
+  re = __builtin_ldexp(__dadd_rn((double)a_, __dmul_rn((double)s_, SQ)), e + k);
+  im = __builtin_ldexp(__dadd_rn((double)c_, __dmul_rn((double)t_, SQ)), e + k);
+}
+
 // ------------------------------------------------------------------- RNG
 // Per-shot xorshift64 stream of the reference: seeding at
 // src/cpp/ksim/kokkos_sim.cpp:215-217, step and 53-bit uniform at :43-46.
@@ -192,6 +216,14 @@ struct LdsWords {
     return t;
   }
 };
+
+// Graphs with approximate floatfactors (GF_APPROX) are evaluated only by
+// the kernels that keep parameter words in LDS (the host sends every
+// program holding one there), so the register kernels carry no float path.
+template <class PW>
+constexpr bool kApprox = false;
+template <>
+constexpr bool kApprox<LdsWords> = true;
 
 // -------------------------------------------------------------- one term
 // Exponents of one term for this lane's parameter words pw: the term is
@@ -401,6 +433,7 @@ struct GraphSum {
   Acc4 acc;     // 128-bit sums (graphs without GF_NARROW)
   Acc64 n;      // 64-bit sums (GF_NARROW)
   double pm;    // the float64 magnitude (GF_MONO)
+  double fr, fi;  // the float64 sum (GF_APPROX)
 };
 
 // clear the sums the graph's flags select (the others stay unset: the
@@ -423,7 +456,7 @@ template <class PW, bool WIDE = true>
 __device__ __forceinline__ void sum_terms(const cword *__restrict__ th, uint32_t nt,
                                           uint32_t gflags, int e0, const PW &pw,
                                           const LdsTables *lt, const Tables &tb, GraphSum &g,
-                                          uint32_t &err) {
+                                          uint32_t &err, const cword *__restrict__ apx = nullptr) {
   for (uint32_t t = 0; t < nt; ++t) {
     const uint32_t flags = th[TH_FLAGS];
     const int E = (int)th[TH_E];
@@ -452,6 +485,29 @@ __device__ __forceinline__ void sum_terms(const cword *__restrict__ th, uint32_t
       if (!WIDE || (gflags & GF_NARROW)) {
         if (!WIDE && !(gflags & GF_NARROW)) err |= ERR_WIDE;
         add_term_narrow(g.n, e, vanished, flags, A, Bq, lt, th, sh, err);
+      } else if (kApprox<PW> && WIDE && (gflags & GF_APPROX)) {
+        // the term's exact value at its own exponent (digit planes at their
+        // own shifts), cast to float64, times its complex64 factor, added
+        // to the float64 sum in term order (zx_eval.hpp:168-178)
+        Acc4 v;
+        v.a = v.b = v.c = v.d = 0;
+        v = add_term_digit<true>(v, e, vanished, flags, A, Bq, lt, th, 0, err);
+        const bool wide = ti.in_tab && !vanished && (unsigned)(ti.n2 + TAB_R) > 2u * TAB_R;
+        if (__builtin_amdgcn_ballot_w64(wide)) {
+#pragma nounroll
+          for (int d = 1; d < TAB_DIGITS; ++d) {
+            const int4 ed = wide ? tb.tab[d * TAB_N + ti.gi] : make_int4(0, 0, 0, 0);
+            v = add_term_digit<false>(v, ed, !wide, flags, A, Bq, lt, th, TAB_DIGIT_BITS * d, err);
+          }
+        }
+        if (!vanished) {
+          double re, im;
+          term_float(v, e0 + sh, re, im);
+          const double ar = (double)__uint_as_float(apx[2 * t]);
+          const double ai = (double)__uint_as_float(apx[2 * t + 1]);
+          g.fr = __dadd_rn(g.fr, __dsub_rn(__dmul_rn(re, ar), __dmul_rn(im, ai)));
+          g.fi = __dadd_rn(g.fi, __dadd_rn(__dmul_rn(re, ai), __dmul_rn(im, ar)));
+        }
       } else if constexpr (WIDE) {
         g.acc = add_term_digit<true>(g.acc, e, vanished, flags, A, Bq, lt, th, sh, err);
         const bool wide = ti.in_tab && !vanished && (unsigned)(ti.n2 + TAB_R) > 2u * TAB_R;
@@ -562,7 +618,10 @@ __device__ __forceinline__ double graph_prob(const cword *__restrict__ P,
   const int e0 = (int)gh[GH_E0];
   const uint32_t nt = gh[GH_NTERMS];
   uint32_t t0 = 0, t1 = nt, off = gh[GH_TERM_OFF];
-  const bool share = SPLIT && sp.L > 1 && !(gflags & GF_MONO);
+  // single-term and (in the kernels that have them) approximate graphs are
+  // evaluated whole by every wave of a group
+  constexpr uint32_t kWhole = kApprox<PW> ? (GF_MONO | GF_APPROX) : GF_MONO;
+  const bool share = SPLIT && sp.L > 1 && !(gflags & kWhole);
   if (share) {
     const int per = GH_PARTS / sp.L;
     const int p0 = __builtin_amdgcn_readfirstlane(sp.part * per), p1 = p0 + per;
@@ -574,6 +633,13 @@ __device__ __forceinline__ double graph_prob(const cword *__restrict__ P,
   }
   GraphSum g;
   graph_sum_clear(g, gflags);
+  if constexpr (kApprox<PW> && WIDE) {
+    if (gflags & GF_APPROX) {
+      g.fr = g.fi = 0.0;
+      sum_terms<PW, WIDE>(P + off, t1 - t0, gflags, e0, pw, lt, tb, g, err, P + gh[GH_APX_OFF]);
+      return __dsqrt_rn(__dadd_rn(__dmul_rn(g.fr, g.fr), __dmul_rn(g.fi, g.fi)));
+    }
+  }
   sum_terms<PW, WIDE>(P + off, t1 - t0, gflags, e0, pw, lt, tb, g, err);
   if (gflags & GF_MONO) return g.pm;
   const bool narrow = !WIDE || (gflags & GF_NARROW);

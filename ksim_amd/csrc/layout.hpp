@@ -81,7 +81,8 @@ enum GraphHdr : int {
   GH_NTERMS = 0, GH_W32, GH_E0, GH_TERM_OFF, GH_NREC, GH_FLAGS,
   GH_PART_T1, GH_PART_T2, GH_PART_T3,
   GH_PART_O1, GH_PART_O2, GH_PART_O3,
-  GH_PAD0, GH_PAD1, GH_PAD2, GH_PAD3,
+  GH_APX_OFF,  // GF_APPROX: dword offset of the terms' complex64 factors (2 per term)
+  GH_PAD1, GH_PAD2, GH_PAD3,
   GH_DWORDS
 };
 // graph flags
@@ -89,6 +90,10 @@ constexpr uint32_t GF_MONO = 1u;    // one term without pi powers or residual: i
                                     // float64 magnitude comes from a table
 constexpr uint32_t GF_NARROW = 2u;  // every partial sum fits a signed 64-bit
                                     // integer (and no term uses several digits)
+constexpr uint32_t GF_APPROX = 4u;  // approximate floatfactors: each term's exact
+                                    // value is cast to float64, multiplied by its
+                                    // complex64 factor and summed in float64, in
+                                    // term order (no term split shares it)
 
 // component header (dwords)
 enum CompHdr : int {

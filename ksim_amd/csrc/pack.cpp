@@ -306,11 +306,10 @@ static void put_mask(std::vector<uint32_t> &w, const Mask &m) {
 
 void pack_graph(const ksim_graph_view &g, std::vector<uint32_t> &out, uint32_t *ghdr,
                 PackStats &st) {
-  if (g.pf_has_approx)
-    throw std::invalid_argument(
-        "pf_has_approx: approximate (complex64) floatfactors are not exact; "
-        "the exact sampler does not accept them");
   if (g.G < 0 || g.P < 0) throw std::invalid_argument("negative graph dimensions");
+  const bool approx = g.pf_has_approx != 0;
+  if (approx && g.G > 0 && !g.pf_approx_f32)
+    throw std::invalid_argument("pf_has_approx without pf_approx factors");
   const int W32 = std::max(1, (g.P + 31) / 32);
   {
     // algorithmic work of the reference formulation (zx_eval.hpp:102-178):
@@ -325,11 +324,21 @@ void pack_graph(const ksim_graph_view &g, std::vector<uint32_t> &out, uint32_t *
     }
   }
   std::vector<TermBuild> terms;
+  std::vector<uint32_t> apx;  // approximate factors of the kept terms, (re, im) f32 bits
   terms.reserve(g.G);
   for (int t = 0; t < g.G; ++t) {
     TermBuild tb;
     build_term(g, t, W32, tb);
-    if (!tb.zero) terms.push_back(std::move(tb));
+    if (!tb.zero) {
+      terms.push_back(std::move(tb));
+      if (approx) {
+        uint32_t re, im;
+        std::memcpy(&re, &g.pf_approx_f32[2 * (size_t)t], 4);
+        std::memcpy(&im, &g.pf_approx_f32[2 * (size_t)t + 1], 4);
+        apx.push_back(re);
+        apx.push_back(im);
+      }
+    }
   }
   // ---- exponent ranges and the static alignment exponent
   struct Range { int vmin, vmax, nmin, nmax, amax, bmax, emin, emax; };
@@ -384,7 +393,11 @@ void pack_graph(const ksim_graph_view &g, std::vector<uint32_t> &out, uint32_t *
                          std::abs(r.nmax + 2 * (r.vmax >> 2))) > TAB_R;
   }
   uint32_t gflags = 0;
-  if (!terms.empty()) {
+  if (approx) {
+    // the terms are summed in float64 one by one: no exact sum to bound,
+    // each term's own value fits the 128-bit accumulator (|n'| <= TAB_W)
+    gflags |= GF_APPROX;
+  } else if (!terms.empty()) {
     const int need = xbits - e0 + ceil_log2((int64_t)terms.size()) + 1;
     st.sum_bits = std::max(st.sum_bits, need + 1);
     if (need > SHIFT_MAX)
@@ -403,7 +416,12 @@ void pack_graph(const ksim_graph_view &g, std::vector<uint32_t> &out, uint32_t *
   // the order of its terms.
   std::vector<size_t> order;
   uint32_t part_first[GH_PARTS + 1] = {0, 0, 0, 0, 0};
-  {
+  if (approx) {
+    // the float64 sum follows the reference's term order (zx_eval.hpp:178),
+    // and no term split shares it: every term in part 0
+    for (size_t k = 0; k < terms.size(); ++k) order.push_back(k);
+    for (int p = 1; p <= GH_PARTS; ++p) part_first[p] = (uint32_t)order.size();
+  } else {
     std::vector<double> wt(terms.size());
     for (size_t k = 0; k < terms.size(); ++k) {
       const TermBuild &tb = terms[k];
@@ -512,6 +530,12 @@ void pack_graph(const ksim_graph_view &g, std::vector<uint32_t> &out, uint32_t *
       ghdr[GH_PART_O1 + p - 1] = (uint32_t)out.size();
     }
   ghdr[GH_NREC] = (uint32_t)nrec;
+  ghdr[GH_APX_OFF] = 0;
+  if (approx) {
+    ghdr[GH_APX_OFF] = (uint32_t)out.size();
+    out.insert(out.end(), apx.begin(), apx.end());
+    st.approx_graphs += 1;
+  }
   st.terms += terms.size();
   st.records += nrec;
 }
@@ -633,6 +657,7 @@ std::vector<uint32_t> pack_program(const ksim_graph_view *graphs, const int32_t 
       if (j.hdr[GH_NTERMS] == 0) j.hdr[GH_W32] = (uint32_t)std::max(1, (F + i + 31) / 32);
       j.hdr[GH_TERM_OFF] += (uint32_t)w.size();  // relative -> absolute
       for (int p = 0; p < GH_PARTS - 1; ++p) j.hdr[GH_PART_O1 + p] += (uint32_t)w.size();
+      if (j.hdr[GH_FLAGS] & GF_APPROX) j.hdr[GH_APX_OFF] += (uint32_t)w.size();
       std::memcpy(&w[gh + (size_t)i * GH_DWORDS], j.hdr, sizeof(j.hdr));
       if (w.size() + j.words.size() >= (size_t)1 << 32)
         throw std::invalid_argument("program larger than 16 GiB");
@@ -645,6 +670,7 @@ std::vector<uint32_t> pack_program(const ksim_graph_view *graphs, const int32_t 
       st.wide_terms += j.st.wide_terms;
       st.sum_bits = std::max(st.sum_bits, j.st.sum_bits);
       st.ops += j.st.ops;
+      st.approx_graphs += j.st.approx_graphs;
     }
     // exact magnitude, compare and running-probability update per output
     st.ops += 32.0 * std::max(0, ng - 1);

@@ -16,6 +16,7 @@ struct PackStats {
   size_t wide_terms = 0;  // terms that can reach beyond the single-digit rows
   int sum_bits = 0;     // bits of the 128-bit sum the worst graph may use
   double ops = 0;       // algorithmic 64-bit integer operations per shot
+  size_t approx_graphs = 0;  // graphs with approximate floatfactors (GF_APPROX)
 };
 
 // Pack one graph: header into ghdr[GH_DWORDS], terms appended to out.

@@ -97,8 +97,13 @@ void validate_words(const uint32_t *w, size_t n) {
       if (W32 < 1 || W32 > cw) bad("graph mask words");
       if (W32 != std::max<uint32_t>(1u, (F + g + 31) / 32)) bad("graph width is not ceil((F + i) / 32)");
       const uint32_t nt = gh[GH_NTERMS];
-      if (gh[GH_FLAGS] & ~(GF_MONO | GF_NARROW)) bad("graph flags");
+      if (gh[GH_FLAGS] & ~(GF_MONO | GF_NARROW | GF_APPROX)) bad("graph flags");
       if ((gh[GH_FLAGS] & GF_MONO) && nt != 1) bad("single-term flag on a graph of several terms");
+      if (gh[GH_FLAGS] & GF_APPROX) {
+        if (gh[GH_FLAGS] & (GF_MONO | GF_NARROW)) bad("approximate graph with an exact-sum flag");
+        if (gh[GH_PART_T1] != nt) bad("approximate graph split into term parts");
+        need((uint64_t)gh[GH_APX_OFF] + 2ull * nt, "approximate factors");
+      }
       const uint32_t pt[GH_PARTS + 1] = {0, gh[GH_PART_T1], gh[GH_PART_T2], gh[GH_PART_T3], nt};
       for (int p = 0; p < GH_PARTS; ++p)
         if (pt[p] > pt[p + 1]) bad("term parts out of order");
@@ -134,7 +139,8 @@ std::vector<uint32_t> words_from_blob(const uint8_t *blob, size_t n, double *sta
   if (!blob) bad("NULL blob");
   if (n < WIRE_HEADER) bad("shorter than its header");
   if (std::memcmp(blob, kMagic, 8) != 0) bad("bad magic");
-  if (rd<uint32_t>(blob + 8) != WIRE_VERSION) bad("unsupported version");
+  const uint32_t version = rd<uint32_t>(blob + 8);
+  if (version != WIRE_VERSION && version != 3) bad("unsupported version");
   if (rd<uint32_t>(blob + 12) != WIRE_HEADER) bad("header size");
   const uint64_t nw = rd<uint64_t>(blob + 16);
   if (nw < PH_DWORDS || nw > (n - WIRE_HEADER) / 4 || WIRE_HEADER + nw * 4 != n)

@@ -8,10 +8,13 @@
 
 namespace ksim {
 
-// Blob layout (little-endian), version 3 (16-dword graph headers with
-// flags and term parts, graph i of a component ceil((F + i) / 32) mask
-// words wide; version 2 let a graph without terms keep the width of its
-// own parameter count, version 1 had 8-dword graph headers):
+// Blob layout (little-endian), version 4 (16-dword graph headers with
+// flags, term parts and the offset of a graph's approximate factors, graph
+// i of a component ceil((F + i) / 32) mask words wide, up to LANE_WORDS_MAX
+// words; version 3 had neither approximate factors nor components wider
+// than 8 words and is read as well; version 2 let a graph without terms keep
+// the width of its own parameter count, version 1 had 8-dword graph
+// headers):
 //   [0, 8)     magic "KSIMPRG\0"
 //   [8, 12)    u32 version
 //   [12, 16)   u32 header bytes (128)
@@ -21,7 +24,7 @@ namespace ksim {
 //   [96, 116)  u32 x 5: n_comp, num_f, num_outputs, total outputs, lane words
 //   [116, 128) zero
 //   [128, ...) u32 program words
-constexpr uint32_t WIRE_VERSION = 3;
+constexpr uint32_t WIRE_VERSION = 4;
 constexpr size_t WIRE_HEADER = 128;
 
 uint64_t fnv1a64(const void *data, size_t n);

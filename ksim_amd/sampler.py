@@ -461,6 +461,14 @@ class HipProgramSampler:
     def handle(self):
         return self._native._h
 
+    @property
+    def exact(self) -> bool:
+        """False when some graph carries approximate (complex64)
+        floatfactors: its terms are then summed in float64 like the
+        reference's (zx_eval.hpp:168-178) instead of exactly, and
+        eval_exact refuses it."""
+        return _lib.load().ksim_program_exact(self.handle) == 1
+
     def set_term_split(self, term_split: int) -> None:
         """Waves that share one 64-shot tile's terms in plain launches: 1, 2
         or 4, or 0 (default) to choose per launch (split only launches too

@@ -105,6 +105,71 @@ def htht_h_program() -> FlatProgram:
     return single_program(htht_h_graphs(), F=0)
 
 
+def rz_graphs(theta: float) -> list:
+    """H.RZ(theta).H on one qubit, measured: P(1) = sin^2(theta / 2), a
+    rotation outside Clifford+T, so its phases are approximate complex64
+    floatfactors (the reference's compiler moves any phase whose
+    denominator is not 1, 2 or 4 there, compile/pipeline.py:362-372).
+
+    graph1 (param: the trying bit m) has three terms: 1/2,
+    (1/4) (-1)^m e^{i theta} and (1/4) (-1)^m e^{-i theta}, the (-1)^m a
+    half-pi record of coefficient 4; at m = 1 the sum is
+    (1 - cos theta) / 2."""
+    g0 = constant_graph(0)
+    g1 = empty_graph(3, 1, Thp=1)
+    g1.pf_power2[:] = [-1, -2, -2]
+    g1.hp_coeffs[1:, 0] = 4
+    g1.hp_params[1:, 0, 0] = 1
+    g1.pf_approx = np.array([1.0, np.exp(1j * theta), np.exp(-1j * theta)], np.complex64)
+    g1.pf_has_approx = True
+    return [g0, g1]
+
+
+def rz_program(theta: float) -> FlatProgram:
+    return single_program(rz_graphs(theta), F=0)
+
+
+def phase_graphs(theta: float) -> list:
+    """One output with P(1) = |1 + w e^{i theta}| / 2 = |cos((theta + pi/4)
+    / 2)| (graph0 = 1): a complex sum whose magnitude depends on the sign of
+    theta and on w, so a conjugated factor, a dropped w or a swapped real
+    and imaginary part changes it.  graph1 has two terms, 1/2 and
+    (1/2) w^5 (-1)^m e^{i theta} (w^5 (-1)^m = w at m = 1, the (-1)^m a
+    half-pi record of coefficient 4 on the trying bit)."""
+    g0 = constant_graph(0)
+    g1 = empty_graph(2, 1, Thp=1)
+    g1.pf_power2[:] = [-1, -1]
+    g1.hp_coeffs[1, 0] = 4
+    g1.hp_params[1, 0, 0] = 1
+    g1.pf_phase_idx[1] = 5
+    g1.pf_approx = np.array([1.0, np.exp(1j * theta)], np.complex64)
+    g1.pf_has_approx = True
+    return [g0, g1]
+
+
+def phase_program(theta: float) -> FlatProgram:
+    return single_program(phase_graphs(theta), F=0)
+
+
+def with_approx_factors(prog: FlatProgram, seed: int, frac: float = 0.5) -> FlatProgram:
+    """``prog`` with random unit complex64 factors on the terms of about
+    ``frac`` of its graphs (pf_has_approx set on those): the shape of a
+    circuit with arbitrary-angle rotations.  Values are not probabilities."""
+    rng = np.random.default_rng(seed)
+    comps = []
+    for c in prog.components:
+        gs = []
+        for g in c.graphs:
+            g = _copy_graph(g)
+            if rng.random() < frac:
+                G = g.pf_phase_idx.shape[0]
+                g.pf_approx = np.exp(1j * rng.uniform(0, 2 * np.pi, size=G)).astype(np.complex64)
+                g.pf_has_approx = True
+            gs.append(g)
+        comps.append(FlatComponent(c.output_indices, c.f_selection, gs))
+    return FlatProgram(comps, prog.output_order, prog.num_outputs, prog.num_detectors)
+
+
 def ghz_graphs(n: int) -> list:
     """n outputs that are all equal, each value 0/1 with probability 1/2.
 

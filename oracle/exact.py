@@ -18,6 +18,9 @@ What it restates (reference files under src/):
 * :func:`magnitude` — the float64 |amplitude| the samplers compare with,
   computed from the exact value by one fixed sequence of correctly rounded
   IEEE operations (the HIP kernel runs the same sequence).
+* :func:`graph_float` / :func:`graph_magnitude` — graphs with approximate
+  (complex64) floatfactors: exact terms cast to float64, times their
+  factors, summed in term order (``zx_eval.hpp:168-178``).
 * :func:`sample_component` / :func:`sample_program` — the autoregressive
   recurrence of ``evaluate.py:69-87`` and ``src/cpp/ksim/kokkos_sim.cpp:214-230``
   driven by given uniforms, in ``sample_flat``'s column order.
@@ -183,6 +186,58 @@ def magnitude(c, e) -> float:
     return math.ldexp(math.sqrt(re * re + im * im), e + k)
 
 
+def term_float(c, e) -> tuple[float, float]:
+    """float64 (re, im) of the exact value c * 2**e by the first steps of
+    :func:`magnitude` (same 62-bit window): re = dbl(a') + dbl(s') * SQ,
+    im = dbl(c') + dbl(t') * SQ, scaled by 2**(e + k).  For coefficients
+    below 2**53 this is the reference's cast of one term
+    (zx_eval.hpp:168-171), whatever the representation."""
+    a, b, cc, d = c
+    s, t = b + d, b - d
+    L = max(abs(a).bit_length(), abs(cc).bit_length(), abs(s).bit_length(), abs(t).bit_length())
+    k = max(0, L - 62)
+    a, cc, s, t = a >> k, cc >> k, s >> k, t >> k
+    re = int_to_double(a) + int_to_double(s) * SQ
+    im = int_to_double(cc) + int_to_double(t) * SQ
+    return math.ldexp(re, e + k), math.ldexp(im, e + k)
+
+
+def approx_factor(d: dict, t: int) -> tuple[float, float]:
+    """The complex64 factor of term t as two float64 values (the float32
+    pairs of pf_approx_f32, src/python/ksim/sample/sampler.py:22,41)."""
+    f = np.asarray(d["pf_approx_f32"], dtype=np.float32).reshape(-1, 2)
+    return float(f[t, 0]), float(f[t, 1])
+
+
+def graph_float(d: dict, params) -> tuple[float, float]:
+    """float64 amplitude of a graph with approximate floatfactors
+    (pf_has_approx): each term's exact value cast by :func:`term_float`,
+    multiplied by its factor (re*ar - im*ai, re*ai + im*ar) and added in
+    term order, as zx_eval.hpp:168-178 does; zero terms add nothing."""
+    params = np.asarray(params, dtype=np.int64).reshape(-1)
+    G = int(np.asarray(d["pf_phase_idx"]).shape[0])
+    sr = si = 0.0
+    for t in range(G):
+        tv = term_value(d, t, params)
+        if tv is None:
+            continue
+        re, im = term_float(*tv)
+        ar, ai = approx_factor(d, t)
+        sr += re * ar - im * ai
+        si += re * ai + im * ar
+    return sr, si
+
+
+def graph_magnitude(d: dict, params) -> float:
+    """The float64 |amplitude| the samplers decide on: :func:`magnitude` of
+    the exact sum, or, for a graph with approximate floatfactors,
+    sqrt(re*re + im*im) of :func:`graph_float` (kokkos_sim.cpp:220,224)."""
+    if d["pf_has_approx"]:
+        sr, si = graph_float(d, params)
+        return math.sqrt(sr * sr + si * si)
+    return magnitude(*graph_value(d, params))
+
+
 def exact_abs_fraction_sq(c, e) -> tuple[Fraction, Fraction]:
     """|c*2^e|^2 = X + Y*sqrt(2) exactly, as Fractions (X, Y)."""
     a, b, cc, d = c
@@ -213,19 +268,25 @@ def sample_component(graph_dicts, f_sel: np.ndarray, uniforms: np.ndarray,
     B = f_sel.shape[0]
     n_out = len(graph_dicts) - 1
     out = np.zeros((B, max(n_out, 0)), dtype=np.uint8)
+
+    def prob(d, params, s, i):
+        if d["pf_has_approx"]:
+            v = graph_float(d, params)
+            if trace is not None:
+                trace.append((s, i, v))
+            return math.sqrt(v[0] * v[0] + v[1] * v[1])
+        v = graph_value(d, params)
+        if trace is not None:
+            trace.append((s, i, v))
+        return magnitude(*v)
+
     with np.errstate(divide="ignore", invalid="ignore"):
         for s in range(B):
             f_row = f_sel[s]
-            v0 = graph_value(graph_dicts[0], np.asarray(f_row, dtype=np.int64))
-            if trace is not None:
-                trace.append((s, 0, v0))
-            prev = magnitude(*v0)
+            prev = prob(graph_dicts[0], np.asarray(f_row, dtype=np.int64), s, 0)
             m = []
             for i in range(n_out):
-                vi = graph_value(graph_dicts[i + 1], params_row(f_row, m, i))
-                if trace is not None:
-                    trace.append((s, i + 1, vi))
-                p1 = magnitude(*vi)
+                p1 = prob(graph_dicts[i + 1], params_row(f_row, m, i), s, i + 1)
                 ratio = np.float64(p1) / np.float64(prev)
                 bit = 1 if float(uniforms[s, i]) < ratio else 0
                 m.append(bit)

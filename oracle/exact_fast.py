@@ -89,19 +89,26 @@ def graph_values(d: dict, params: np.ndarray) -> list:
     return FastGraph(d).values(params)
 
 
+def magnitudes(d: dict, rows: np.ndarray) -> np.ndarray:
+    """exact.graph_magnitude at every parameter row: exact sums from the C
+    code, graphs with approximate floatfactors by exact.graph_float."""
+    rows = np.asarray(rows, dtype=np.uint8)
+    if d["pf_has_approx"]:
+        return np.array([exact.graph_magnitude(d, r) for r in rows], dtype=np.float64)
+    return np.array([exact.magnitude(*v) for v in FastGraph(d).values(rows)], dtype=np.float64)
+
+
 def sample_component(graph_dicts, f_sel: np.ndarray, uniforms: np.ndarray) -> np.ndarray:
     """exact.sample_component, all shots of a graph at a time."""
     B = f_sel.shape[0]
     n_out = len(graph_dicts) - 1
     m = np.zeros((B, n_out), np.uint8)
     f8 = np.asarray(f_sel, dtype=np.uint8)
-    prev = np.array([exact.magnitude(*v) for v in FastGraph(graph_dicts[0]).values(f8)],
-                    dtype=np.float64).reshape(B)
+    prev = magnitudes(graph_dicts[0], f8).reshape(B)
     with np.errstate(divide="ignore", invalid="ignore"):
         for i in range(n_out):
             params = np.hstack([f8, m[:, :i], np.ones((B, 1), np.uint8)])
-            p1 = np.array([exact.magnitude(*v) for v in FastGraph(graph_dicts[i + 1]).values(params)],
-                          dtype=np.float64).reshape(B)
+            p1 = magnitudes(graph_dicts[i + 1], params).reshape(B)
             bits = np.asarray(uniforms[:, i], dtype=np.float64) < p1 / prev
             m[:, i] = bits
             prev = np.where(bits, p1, prev - p1)

@@ -56,6 +56,8 @@ struct Graph {
   std::vector<uint8_t> np_ph, np_pa, hp_c, hp_pa, ps_c, ps_pa, pf_c, pf_pa, al, al_pa, be, be_pa,
       pf_idx;
   std::vector<int32_t> np_cnt, ph_cnt, ff, p2;
+  std::vector<double> apx;  // (re, im) per term, from the complex64 factors
+  bool has_apx = false;
 };
 
 template <class T>
@@ -130,8 +132,16 @@ void eval_graph(const Graph &g, const Params &pr, double &ore, double &oim) {
     double re = (double)acc.a + ((double)acc.b + (double)acc.d) * SQ;
     double im = (double)acc.c + ((double)acc.b - (double)acc.d) * SQ;
     const double sc = std::exp2((double)(pw + g.p2[t]));
-    sre += re * sc;
-    sim += im * sc;
+    re *= sc;
+    im *= sc;
+    if (g.has_apx) {  // approximate floatfactor (zx_eval.hpp:172-177)
+      const double ar = g.apx[2 * t], ai = g.apx[2 * t + 1];
+      const double r2 = re * ar - im * ai;
+      im = re * ai + im * ar;
+      re = r2;
+    }
+    sre += re;
+    sim += im;
   }
   ore = sre;
   oim = sim;
@@ -174,6 +184,9 @@ void *kport_create(const ksim_graph_view *gv, int32_t n_graphs, int32_t F) {
     g.pf_idx = copy(v.pf_phase_idx, G);
     g.ff = copy(v.pf_floatfactor, G * 4);
     g.p2 = copy(v.pf_power2, G);
+    g.has_apx = v.pf_has_approx != 0;
+    if (g.has_apx)
+      for (size_t k = 0; k < 2 * G; ++k) g.apx.push_back((double)v.pf_approx_f32[k]);
     c->g.push_back(std::move(g));
   }
   return c;

@@ -19,6 +19,7 @@ GH_DWORDS = 16
 CH = dict(NGRAPHS=0, F=1, NOUT=2, WMAX=3, COLBASE=4, FSEL=5, DEST=6, GRAPH=7)
 PH_DWORDS = 8
 TF_RESIDUAL, TF_PI = 1, 2
+GF_APPROX = 4
 
 
 def s32(x: int) -> int:
@@ -64,11 +65,44 @@ class Packed:
 
     def eval_graph(self, c, i, bits):
         """(coeffs, exp2) of graph i of component c at parameter bits."""
+        gh = self.graph_hdr(c, i)
+        e0 = s32(gh[2])
+        acc = [0, 0, 0, 0]
+        for X, sh in self.terms(c, i, bits):
+            assert sh >= 0
+            acc = [a + (v << sh) for a, v in zip(acc, X)]
+        return tuple(acc), e0
+
+    def magnitude(self, c, i, bits):
+        """The float64 |amplitude| the kernels decide on: magnitude() of the
+        exact sum, or for a GF_APPROX graph the float64 sum in term order of
+        each term's cast times its complex64 factor (GH_APX_OFF table)."""
+        import math
+        import struct
+        from oracle import exact
+        gh = self.graph_hdr(c, i)
+        e0 = s32(gh[2])
+        if not gh[5] & GF_APPROX:
+            return exact.magnitude(*self.eval_graph(c, i, bits))
+        apx = gh[12]
+        fr = fi = 0.0
+        for t, (X, sh) in enumerate(self.terms(c, i, bits, keep_dead=True)):
+            if X is None:
+                continue
+            re, im = exact.term_float(X, e0 + sh)
+            ar = struct.unpack("<f", struct.pack("<I", self.w[apx + 2 * t]))[0]
+            ai = struct.unpack("<f", struct.pack("<I", self.w[apx + 2 * t + 1]))[0]
+            fr += re * ar - im * ai
+            fi += re * ai + im * ar
+        return math.sqrt(fr * fr + fi * fi)
+
+    def terms(self, c, i, bits, keep_dead=False):
+        """(value, shift) of every term of graph i at parameter bits: value
+        * 2^(e0 + shift); a vanished term gives zero (None with keep_dead)."""
         w = self.w
         gh = self.graph_hdr(c, i)
-        nterms, W32, e0, off = gh[0], gh[1], s32(gh[2]), gh[3]
+        nterms, W32, off = gh[0], gh[1], gh[3]
         pv = param_words(bits, W32)
-        acc = [0, 0, 0, 0]
         th = off
         for _ in range(nterms):
             h = w[th:th + TH_DWORDS]
@@ -123,9 +157,6 @@ class Packed:
             if h[TH["FLAGS"]] & TF_RESIDUAL:
                 X = zmul(X, tuple(s32(h[TH["R0"] + k]) for k in range(4)))
             if dead & 1:
-                X = (0, 0, 0, 0)
-            sh = q + E
-            assert sh >= 0
-            acc = [a + (v << sh) for a, v in zip(acc, X)]
+                X = None if keep_dead else (0, 0, 0, 0)
+            yield X, q + E
             th = th + TH_DWORDS + h[TH["SIZE"]]
-        return tuple(acc), e0

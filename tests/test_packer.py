@@ -164,11 +164,31 @@ def test_empty_graph_and_zero_terms(tables):
     check_program(prog, tables, all_rows=True)
 
 
-def test_rejects_approx():
-    prog = synth.htkh_program(1)
-    prog.components[0].graphs[1].pf_has_approx = True
-    with pytest.raises(ValueError, match="approximate"):
-        pack_words(prog)
+@pytest.mark.parametrize("prog_fn", [
+    lambda: synth.rz_program(0.7),
+    lambda: synth.phase_program(-1.3),
+    lambda: synth.with_approx_factors(synth.shape_program(n_q=6, t=6, F=8, n_out=6, seed=3), 4),
+    lambda: synth.with_approx_factors(synth.wide_exponent_program(seed=5), 5, frac=1.0),
+    lambda: synth.with_approx_factors(
+        synth.shape_program(n_q=6, t=4, F=300, n_out=3, seed=6, n_components=2), 6),
+])
+def test_approximate_floatfactors(tables, prog_fn):
+    # graphs with complex64 factors: the packed terms (kept in the
+    # reference's order, factors in the graph's table) give, by the
+    # kernels' float path, the oracle's float64 magnitude bit for bit
+    prog = prog_fn()
+    pk = Packed(pack_words(prog), tables)
+    rng = np.random.default_rng(2)
+    n = 0
+    for c, comp in enumerate(prog.components):
+        for i, g in enumerate(comp.graphs):
+            d = graph_to_dict(g)
+            P = g.np_params.shape[2] if g.np_params.ndim == 3 else 0
+            for _ in range(4):
+                bits = rng.integers(0, 2, size=P)
+                assert pk.magnitude(c, i, bits) == exact.graph_magnitude(d, bits), (c, i)
+                n += 1
+    assert n > 0
 
 
 def test_rejects_wrong_param_count():

@@ -48,12 +48,20 @@ def test_blob_holds_the_packed_words(prog, blob):
     st = {}
     w = pack_words(prog, st)
     assert blob[:8] == b"KSIMPRG\0"
-    assert struct.unpack("<II", blob[8:16]) == (3, HDR)
+    assert struct.unpack("<II", blob[8:16]) == (4, HDR)
     assert np.array_equal(words_of(blob), w)
     assert struct.unpack("<Q", blob[24:32])[0] == fnv1a64(w.tobytes())
     got = blob_stats(blob)
     for k in ("graphs", "terms", "records", "words", "ops_per_shot", "sum_bits"):
         assert got[k] == st[k], k
+
+
+def test_version_3_blobs_still_load(prog, blob):
+    # a version-3 blob (no approximate factors, at most 8 words) is a valid
+    # version-4 blob with the older number
+    b = bytearray(blob)
+    b[8:12] = struct.pack("<I", 3)
+    assert blob_stats(bytes(b))["words"] == blob_stats(blob)["words"]
 
 
 def test_blob_of_every_golden_program(golden_files):
@@ -125,6 +133,29 @@ def test_rejects_structural_corruption(blob):
         with pytest.raises(ValueError):
             blob_stats(with_words(blob, v))
         # the unchanged words still pass (the re-sealing itself is sound)
+    blob_stats(with_words(blob, w))
+
+
+def test_rejects_approximate_graph_corruption():
+    # approximate factors beyond the words, an exact-sum flag on an
+    # approximate graph, and an approximate graph split into term parts
+    prog = synth.with_approx_factors(synth.shape_program(n_q=6, t=6, F=5, n_out=4, seed=8), 1,
+                                     frac=1.0)
+    blob = program_blob(prog)
+    w = words_of(blob)
+    PH, GH = 8, 16
+    c0 = int(w[PH])
+    g1 = int(w[c0 + 7]) + GH
+    assert int(w[g1 + 5]) == 4  # GF_APPROX
+    assert int(w[g1 + 0]) >= 2
+    cases = []
+    for idx, val in ((g1 + 12, len(w) - 1), (g1 + 5, 4 | 2), (g1 + 6, 1)):
+        v = w.copy()
+        v[idx] = val
+        cases.append(v)
+    for v in cases:
+        with pytest.raises(ValueError):
+            blob_stats(with_words(blob, v))
     blob_stats(with_words(blob, w))
 
 

@@ -106,6 +106,13 @@ def make_workload(name: str):
     if name == "config3":
         return dict(prog=synth.config3_program(), B=1_000_000, F=24,
                     desc="30q brickwork encoder, 12 T, 30 outputs, F=24, plain launch")
+    if name.startswith("shape:"):
+        # shape:Q,T,F,NOUT,SHOTS[,SEED] -- one shape-model program, plain launch
+        v = [int(x) for x in name.split(":")[1].split(",")]
+        q, t, F, n_out, shots = v[:5]
+        seed = v[5] if len(v) > 5 else 900 + t
+        return dict(prog=synth.shape_program(n_q=q, t=t, F=F, n_out=n_out, seed=seed), B=shots,
+                    F=F, desc=f"shape model q={q} t={t} F={F} outputs={n_out}, plain launch")
     raise ValueError(name)
 
 
@@ -233,11 +240,12 @@ def host_path(pl, reps=3):
     return res
 
 
-def plain_result(name, steps, warmup, with_cpu=True):
+def plain_result(name, steps, warmup, with_cpu=True, term_split=0, with_host=True):
     import torch
     w = make_workload(name)
     log(f"[bench] {name}: building program")
     pl = Plain(w["prog"], w["B"], w["F"])
+    pl.s.set_term_split(term_split)
     wall, dev = time_steps(pl.step, steps, warmup)
     pl.check()
     ms = wall / steps * 1e3
@@ -248,7 +256,8 @@ def plain_result(name, steps, warmup, with_cpu=True):
                shots_per_s=shots_s, us_per_shot=1e6 / shots_s,
                kernel_ms=kern * 1e3, ops_per_shot=pl.ops, stats=pl.s.stats(),
                roofline=valu_roofline(achieved, pl.bytes_per_shot * w["B"] / kern, name))
-    res["host_path"] = host_path(pl)
+    if with_host:
+        res["host_path"] = host_path(pl)
     if with_cpu:
         # about 10-20 s of CPU work
         probe = cpu_baseline_port(w["prog"], w["F"], 200)
@@ -538,11 +547,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="config4",
                     help="config4 (batched, the headline) | config1 | config2 | config3 | "
-                         "config5 (serving)")
+                         "config5 (serving) | shape:Q,T,F,NOUT,SHOTS[,SEED] (one plain launch)")
+    ap.add_argument("--term-split", type=int, default=0,
+                    help="plain workloads: waves per tile (0 = automatic, 1, 2, 4)")
     ap.add_argument("--quick", action="store_true",
                     help="config4: the headline line only (no config 1-3 rows, T-count "
                          "sweep or serving run)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-host", action="store_true",
+                    help="plain workloads: skip the host-array (PCIe) path timing")
     ap.add_argument("--no-tune", action="store_true",
                     help="config4: default launch shape instead of the autotuner's")
     ap.add_argument("--batch-shape", default=None,
@@ -586,7 +599,8 @@ def main():
     if args.workload == "config5":
         print(json.dumps({"metric": "config5 serving", **serving_result()}), flush=True)
         return
-    res = plain_result(args.workload, args.steps, args.warmup, with_cpu=not args.no_cpu)
+    res = plain_result(args.workload, args.steps, args.warmup, with_cpu=not args.no_cpu,
+                       term_split=args.term_split, with_host=not args.no_host)
     if args.workload == "config2":
         w2 = make_workload("config2")
         res["noise"] = noise_result(w2["prog"], w2["B"], args.steps, args.warmup,

@@ -718,22 +718,19 @@ int plain_split(const ksim_program *p, int64_t B, bool nz) {
   if (w[0] <= 0) return fill;
   // Price the launched grid: L = 1 runs one block per 4 tiles, L > 1 one
   // block per group of 4/L tiles, at most kSplitBlocks blocks per component
-  // that loop over the groups.  Every block puts one wave on each SIMD of
-  // its CU for `iters` group iterations, so a CU's SIMDs each take b waves
-  // of iters x w[k] work, in rounds of o resident waves.
+  // that loop over the groups.  Each block iteration (a unit: one group of
+  // tiles of one component) puts one wave on each SIMD of its CU for w[k]
+  // of work, so the SIMDs each take m = units / CUs waves, in rounds of o
+  // resident waves, however the units are dealt to looping blocks (a block
+  // that finishes its units frees its slot for a waiting one).
   const int cus = simds / 4;
   auto cost = [&](int k) {
     const int o = resident_waves(p->wl, p->all_narrow, nz, Ls[k] > 1, p->wmax);
     const int64_t per = 4 / Ls[k];  // tiles per block iteration
-    const int64_t groups = (tiles + per - 1) / per;
-    // the LDS-word kernel loops over tile groups at every L
-    const bool looping = Ls[k] > 1 || p->wl == 0;
-    const int64_t blocks = looping ? std::min<int64_t>(groups, kSplitBlocks) : groups;
-    const int64_t iters = (groups + blocks - 1) / blocks;
-    const int64_t b = (blocks * p->n_comp + cus - 1) / cus;  // waves per SIMD
-    const int64_t full = b / o, rem = b % o;
-    return ((double)(full * o) + (rem ? std::max((double)rem, kLatencyWaves) : 0.0)) *
-           (double)iters * w[k];
+    const int64_t units = (tiles + per - 1) / per * p->n_comp;
+    const int64_t m = (units + cus - 1) / cus;  // waves per SIMD
+    const int64_t full = m / o, rem = m % o;
+    return ((double)(full * o) + (rem ? std::max((double)rem, kLatencyWaves) : 0.0)) * w[k];
   };
   const int kf = fill == 1 ? 0 : fill == 2 ? 1 : 2;
   const double cf = cost(kf);

@@ -55,9 +55,11 @@ HBM_PEAK = 8.0e12
 # HBM bytes per dispatch of the dominant kernel, measured by two rocprofv3
 # --pmc passes (FETCH_SIZE, WRITE_SIZE, each scaled by the same counter on a
 # copy kernel of known size) and recorded in these summaries
-PROFILE_OF = {"config2": "k_sample_config2_plain_1M.md",
+PROFILE_OF = {"config1": "k_sample_config1_plain_100k.md",
+              "config2": "k_sample_config2_plain_1M.md",
               "config3": "k_sample_config3_plain_1M.md",
-              "config4": "k_batch_config4_64prog.md"}
+              "config4": "k_batch_config4_64prog.md",
+              "shape:85,40,80,85,200000,7": "k_sample_split_t40q85_200k.md"}
 
 
 def profiled(workload, key):

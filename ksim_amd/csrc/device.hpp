@@ -225,13 +225,47 @@ constexpr bool kApprox = false;
 template <>
 constexpr bool kApprox<LdsWords> = true;
 
+// ------------------------------------------------------ record read-ahead
+// The records of a GF_STREAM graph are pulled into L2 ahead of the scalar
+// loads that walk them: before each exponent chunk the wave touches the
+// 128-byte lines from the furthest line it touched so far (or the chunk,
+// if that is further) up to STREAM_AHEAD dwords past the chunk's end, one
+// 4-byte vector load per line, at most 64 lines per touch.  The scalar
+// loads then find the lines in L2 instead of waiting on HBM one round trip
+// at a time.  The program ends STREAM_PAD >= STREAM_AHEAD dwords past its
+// last record, so every touched line is inside the program words.
+struct Stream {
+  const cword *pf;  // touched up to here; nullptr: the graph is not streamed
+};
+
+// Touch the lines of [max(ps.pf, from), want); the returned value is
+// discarded by stream_done once the chunk is walked, so the load is waited
+// for only then.
+__device__ __forceinline__ uint32_t stream_touch(Stream &ps, const cword *from,
+                                                 const cword *want) {
+  if (ps.pf == nullptr) return 0;
+  const cword *lo = ps.pf > from ? ps.pf : from;
+  const cword *base = (const cword *)((uintptr_t)lo & ~(uintptr_t)127);
+  if (base >= want) return 0;
+  const cword *a = base + 32 * (threadIdx.x & 63);
+  uint32_t v = 0;
+  if (a < want) v = *a;
+  const cword *end = base + 64 * 32;
+  ps.pf = end < want ? end : want;
+  return v;
+}
+
+__device__ __forceinline__ void stream_done(uint32_t v) { asm volatile("" ::"v"(v)); }
+
 // -------------------------------------------------------------- one term
 // Exponents of one term for this lane's parameter words pw: the term is
 // w^S (1+w)^V (1+sqrt2)^N pi^A pibar^B [R] 2^E, or zero when bit 0 of
-// `dead` is set.  th points at the term header.
+// `dead` is set.  th points at the term header.  LONG kernels read the
+// records of streamed graphs ahead (ps).
 template <class PW, bool LONG = false>
 __device__ __forceinline__ void term_exps(const cword *__restrict__ th, const PW &pw, int &S,
-                                          int &V, int &N, int &A, int &Bq, uint32_t &dead) {
+                                          int &V, int &N, int &A, int &Bq, uint32_t &dead,
+                                          Stream &ps) {
   const int W32 = pw.width();
   const uint32_t nproj = th[TH_NPROJ], nppx = th[TH_NPPX], nchunk = th[TH_NCHUNK];
   S = (int)th[TH_S]; V = (int)th[TH_V]; N = (int)th[TH_N];
@@ -266,6 +300,10 @@ __device__ __forceinline__ void term_exps(const cword *__restrict__ th, const PW
   for (uint32_t ch = 0; ch < nchunk; ++ch) {
     const uint32_t nl = rp[0], npr = rp[1];
     uint32_t x = rp[2];
+    uint32_t touched = 0;
+    if constexpr (LONG)
+      touched = stream_touch(ps, rp, rp + CHUNK_HDR + nl * (W32 + 1) + npr * (2 * W32 + 3) +
+                                         STREAM_AHEAD);
     rp += CHUNK_HDR;
     // LIN records, unrolled so each scalar load fetches several records
     if constexpr (LONG) {
@@ -297,6 +335,7 @@ __device__ __forceinline__ void term_exps(const cword *__restrict__ th, const PW
         x += (ma & rp[2 * W32]) + (mb & rp[2 * W32 + 1]) + (ma & mb & rp[2 * W32 + 2]);
       }
     }
+    if constexpr (LONG) stream_done(touched);
     V += (int)((x >> PK_OV) & ((1u << PK_WV) - 1));
     N += (int)((x >> PK_ON) & ((1u << PK_WN) - 1));
     A += (int)((x >> PK_OA) & ((1u << PK_WA) - 1));
@@ -457,12 +496,13 @@ __device__ __forceinline__ void sum_terms(const cword *__restrict__ th, uint32_t
                                           uint32_t gflags, int e0, const PW &pw,
                                           const LdsTables *lt, const Tables &tb, GraphSum &g,
                                           uint32_t &err, const cword *__restrict__ apx = nullptr) {
+  Stream ps{(gflags & GF_STREAM) ? th : nullptr};
   for (uint32_t t = 0; t < nt; ++t) {
     const uint32_t flags = th[TH_FLAGS];
     const int E = (int)th[TH_E];
     int S, V, N, A, Bq;
     uint32_t dead;
-    term_exps<PW, WIDE>(th, pw, S, V, N, A, Bq, dead);
+    term_exps<PW, WIDE>(th, pw, S, V, N, A, Bq, dead, ps);
     const TermIdx ti = term_index(S, V, N);
     const bool vanished = dead & 1u;
     const int sh = ti.q + E;

@@ -94,6 +94,16 @@ constexpr uint32_t GF_APPROX = 4u;  // approximate floatfactors: each term's exa
                                     // value is cast to float64, multiplied by its
                                     // complex64 factor and summed in float64, in
                                     // term order (no term split shares it)
+constexpr uint32_t GF_STREAM = 8u;  // the 128-bit kernels read the graph's records
+                                    // ahead into L2; the program then ends in
+                                    // STREAM_PAD spare dwords
+
+// Record read-ahead of GF_STREAM graphs (the packer sets the flag on every
+// graph of several terms): each touch reaches STREAM_AHEAD dwords past the
+// exponent chunk about to be walked, so such a program ends in
+// STREAM_PAD >= STREAM_AHEAD dwords past its last record.
+constexpr int STREAM_AHEAD = 256;
+constexpr int STREAM_PAD = 256;
 
 // component header (dwords)
 enum CompHdr : int {

@@ -97,7 +97,7 @@ void validate_words(const uint32_t *w, size_t n) {
       if (W32 < 1 || W32 > cw) bad("graph mask words");
       if (W32 != std::max<uint32_t>(1u, (F + g + 31) / 32)) bad("graph width is not ceil((F + i) / 32)");
       const uint32_t nt = gh[GH_NTERMS];
-      if (gh[GH_FLAGS] & ~(GF_MONO | GF_NARROW | GF_APPROX)) bad("graph flags");
+      if (gh[GH_FLAGS] & ~(GF_MONO | GF_NARROW | GF_APPROX | GF_STREAM)) bad("graph flags");
       if ((gh[GH_FLAGS] & GF_MONO) && nt != 1) bad("single-term flag on a graph of several terms");
       if (gh[GH_FLAGS] & GF_APPROX) {
         if (gh[GH_FLAGS] & (GF_MONO | GF_NARROW)) bad("approximate graph with an exact-sum flag");
@@ -130,6 +130,8 @@ void validate_words(const uint32_t *w, size_t n) {
         need(to + TH_DWORDS + sz, "term records");
         to += TH_DWORDS + sz;
       }
+      // the record read-ahead reaches STREAM_AHEAD dwords past the last record
+      if (gh[GH_FLAGS] & GF_STREAM) need(to + STREAM_AHEAD, "record read-ahead padding");
     }
   }
   if (outs != tot) bad("total outputs");

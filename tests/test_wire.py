@@ -116,7 +116,7 @@ def structural_cases(w):
     mut("term size", t0 + 14, int(w[t0 + 14]) + 1)
     mut("term flags", t0 + 3, 8)
     mut("chunk count", t0 + 2, int(w[t0 + 2]) + 1000)
-    mut("graph flags", g1 + 5, 8)
+    mut("graph flags", g1 + 5, 16)
     mut("term part order", g1 + 6, int(w[g1 + 0]) + 1)
     mut("term part offset", g1 + 9, int(w[g1 + 9]) + 1)
     ng = int(w[c0 + 0])
@@ -146,7 +146,7 @@ def test_rejects_approximate_graph_corruption():
     PH, GH = 8, 16
     c0 = int(w[PH])
     g1 = int(w[c0 + 7]) + GH
-    assert int(w[g1 + 5]) == 4  # GF_APPROX
+    assert int(w[g1 + 5]) & 4  # GF_APPROX
     assert int(w[g1 + 0]) >= 2
     cases = []
     for idx, val in ((g1 + 12, len(w) - 1), (g1 + 5, 4 | 2), (g1 + 6, 1)):
@@ -157,6 +157,30 @@ def test_rejects_approximate_graph_corruption():
         with pytest.raises(ValueError):
             blob_stats(with_words(blob, v))
     blob_stats(with_words(blob, w))
+
+
+def test_stream_flag_needs_its_padding():
+    # the packer flags every multi-term graph GF_STREAM (record read-ahead)
+    # and ends the words in 256 zero dwords; a flagged graph without that
+    # padding past its last record is rejected
+    prog = synth.shape_program(n_q=6, t=6, F=5, n_out=4, seed=8)
+    blob = program_blob(prog)
+    w = words_of(blob)
+    PH, GH = 8, 16
+    c0 = int(w[PH])
+    g0, ng = int(w[c0 + 7]), int(w[c0])
+    for i in range(ng):
+        assert bool(int(w[g0 + GH * i + 5]) & 8) == (int(w[g0 + GH * i]) > 1), i
+    assert not w[-256:].any()
+    last = g0 + GH * (ng - 1)
+    assert int(w[last]) >= 2 and int(w[last + 5]) & 8
+    with pytest.raises(ValueError, match="read-ahead"):
+        blob_stats(with_words(blob, w[:-1]))
+    # without the flags the padding is not needed
+    v = w[:-256].copy()
+    for i in range(ng):
+        v[g0 + GH * i + 5] &= ~np.uint32(8)
+    blob_stats(with_words(blob, v))
 
 
 def test_unpack_outputs_and_split():

@@ -442,47 +442,76 @@ def tcount_sweep(ts=(0, 4, 8, 12, 16, 20, 24, 32, 40), shots=100_000, n_q=30):
     return rows
 
 
-def serving_result(n_requests=48, n_pool=8, load=0.7, t_max=40, q_max=85, seed=5):
+def sparse_f(B, F, p, seed):
+    """A (B, F) uint8 matrix of independent Bernoulli(p) bits for small p,
+    drawn by placing Binomial(B F, p) ones at uniform positions (a position
+    drawn twice holds one 1, a relative deficit of about B F p / 2 ones in
+    B F), so a 1M x 80 request does not build a float64 matrix."""
+    rng = np.random.default_rng(seed)
+    f = np.zeros((B, F), np.uint8)
+    if B * F:
+        f.flat[rng.integers(0, B * F, size=int(rng.binomial(B * F, p)))] = 1
+    return f
+
+
+def serving_result(n_requests=200, n_pool=8, load=0.7, t_max=40, q_max=85, seed=5,
+                   rate=None, fill_long=True):
     """config5: Poisson request stream; programs from the config-5 generator
-    (t <= 40, q <= 85) drawn from a pool of resident programs, shots
-    log-uniform in [100, 1M]; arrival rate = ``load`` x the saturation rate
-    measured on the same mix."""
+    (t <= 40, q <= 85) drawn from a pool of resident programs whose first
+    program is the generator's largest shape (t = 40, q = 85, F = 80, 85
+    outputs), shots log-uniform in [100, 1M]; arrival rate = ``load`` x the
+    saturation rate measured on the same stream (or ``rate`` requests/s).
+    The latency of the
+    heaviest requests is bounded below by their work at the saturation rate
+    (``heaviest``)."""
     import torch
     from ksim_amd import synth
     from ksim_amd.sampler import HipProgramSampler
     from ksim_amd.serve import Request, ServingLoop, poisson_arrivals
     rng = np.random.default_rng(seed)
     specs = [synth.config5_spec(rng, t_max=t_max, q_max=q_max) for _ in range(n_pool)]
-    log(f"[bench] config5: building {n_pool} programs (t = {[s['t'] for s in specs]})")
+    specs[0].update(n_q=q_max, t=t_max, F=2 * t_max, n_out=q_max)
+    log(f"[bench] config5: building {n_pool} programs (t = {[s['t'] for s in specs]}, "
+        f"q = {[s['n_q'] for s in specs]})")
     pool = [HipProgramSampler(synth.program_from_spec(sp)) for sp in specs]
     reqs = []
     for i in range(n_requests):
         k = int(rng.integers(n_pool))
         shots = int(round(math.exp(rng.uniform(math.log(100), math.log(1_000_000)))))
-        f = synth.random_f(shots, specs[k]["F"], 1e-3, 10_000 + i)
+        f = sparse_f(shots, specs[k]["F"], 1e-3, 10_000 + i)
         reqs.append(Request(key=k, prog=pool[k], f=f, seed=20_000 + i))
-    total_ops = sum(pool[r.key].ops_per_shot * r.shots for r in reqs)
+    work = np.array([pool[r.key].ops_per_shot * r.shots for r in reqs])
+    total_ops = float(work.sum())
     # saturation: the whole stream at once (no arrival gaps)
-    loop = ServingLoop(hbm_budget=96 << 30, max_slots=64, step_ms=30.0)
-    warm = [Request(key=r.key, prog=r.prog, f=r.f[:64], seed=r.seed) for r in reqs[:4]]
+    loop = ServingLoop(hbm_budget=96 << 30, max_slots=64, step_ms=30.0, fill_long=fill_long)
+    warm = [Request(key=r.key, prog=r.prog, f=r.f[:64], seed=r.seed) for r in reqs[:8]]
     loop.run(warm, realtime=False, progress=True, max_wall_s=60)
     log(f"[bench] config5: saturation run over {n_requests} requests, {total_ops:.3g} ops")
     sat = loop.run([Request(key=r.key, prog=r.prog, f=r.f, seed=r.seed) for r in reqs],
-                   realtime=False, progress=True, max_wall_s=120)
+                   realtime=False, progress=True, max_wall_s=240)
     rate_ops = total_ops / sat["wall_s"]
-    lam = load * rate_ops / (total_ops / n_requests)
+    lam = rate if rate else load * rate_ops / (total_ops / n_requests)
     for r, t in zip(reqs, poisson_arrivals(n_requests, lam, seed)):
         r.arrival = float(t)
     log(f"[bench] config5: saturation {sat['wall_s']:.2f}s for {n_requests} requests; "
         f"Poisson rate {lam:.2f}/s")
-    st = loop.run(reqs, realtime=True, progress=True, max_wall_s=180)
+    st = loop.run(reqs, realtime=True, progress=True, max_wall_s=300)
     loop.close()
+    lat = np.array([r.latency for r in reqs]) * 1e3
+    top = np.argsort(work)[::-1][:3]
     return dict(requests=n_requests, pool=n_pool, load=load, arrival_rate_per_s=lam,
                 pool_t=[sp["t"] for sp in specs], pool_q=[sp["n_q"] for sp in specs],
-                shots=st["shots"], p50_ms=st["p50_ms"], p99_ms=st["p99_ms"],
-                mean_ms=st["mean_ms"], shots_per_s=st["shots_per_s"], wall_s=st["wall_s"],
-                saturation_shots_per_s=sat["shots_per_s"], steps=st["steps"],
-                algorithmic_ops_per_s=rate_ops)
+                pool_ops_per_shot=[p.ops_per_shot for p in pool],
+                shots=st["shots"], p50_ms=st["p50_ms"], p90_ms=float(np.percentile(lat, 90)),
+                p99_ms=st["p99_ms"], max_ms=float(lat.max()), mean_ms=st["mean_ms"],
+                shots_per_s=st["shots_per_s"], wall_s=st["wall_s"],
+                saturation_shots_per_s=sat["shots_per_s"], saturation_wall_s=sat["wall_s"],
+                steps=st["steps"], algorithmic_ops_per_s=rate_ops,
+                heaviest=[dict(program=int(reqs[i].key), t=specs[reqs[i].key]["t"],
+                               q=specs[reqs[i].key]["n_q"], shots=reqs[i].shots,
+                               latency_ms=float(lat[i]),
+                               work_at_saturation_ms=float(work[i] / rate_ops * 1e3))
+                          for i in top])
 
 
 METRIC = "shots/s & \u00b5s/shot vs T-count, plain vs batched (exact); p50/p99 latency"
@@ -563,6 +592,11 @@ def main():
     ap.add_argument("--batch-shape", default=None,
                     help="config4: launch shape WAVE_SHOTS,SPLIT_SHARE|none,BLOCKS_PER_CU "
                          "instead of the autotuner's (e.g. 64,none,5)")
+    ap.add_argument("--serve-rate", type=float, default=None,
+                    help="config5: Poisson arrival rate (requests/s) instead of 0.7 x the "
+                         "measured saturation rate")
+    ap.add_argument("--serve-no-fill", action="store_true",
+                    help="config5: steps sized by step_ms only (ServingLoop fill_long=False)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="launch one elementwise int32 copy kernel of this size first, so "
                          "counter runs can scale FETCH_SIZE / WRITE_SIZE against known bytes")
@@ -599,7 +633,9 @@ def main():
         print(json.dumps(line), flush=True)
         return
     if args.workload == "config5":
-        print(json.dumps({"metric": "config5 serving", **serving_result()}), flush=True)
+        print(json.dumps({"metric": "config5 serving",
+                          **serving_result(rate=args.serve_rate,
+                                           fill_long=not args.serve_no_fill)}), flush=True)
         return
     res = plain_result(args.workload, args.steps, args.warmup, with_cpu=not args.no_cpu,
                        term_split=args.term_split, with_host=not args.no_host)

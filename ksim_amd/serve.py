@@ -82,11 +82,13 @@ class ServingLoop:
     tile_shots      shots per wave group per work item (a multiple of 64);
                     an item of term split L covers tile_shots * 4 / L shots
     blocks          persistent blocks per launch (default: 6 per CU)
+    fill_long       a step that holds long items also holds the work of
+                    blocks / streams of the longest (see run)
     """
 
     def __init__(self, hbm_budget: int = 32 << 30, max_slots: int = 64, step_ms: float = 20.0,
                  tile_shots: int = 64, blocks: int | None = None, max_items: int = 1 << 16,
-                 rate: float = 2.5e13, streams: int = 3):
+                 rate: float = 2.5e13, streams: int = 3, fill_long: bool = True):
         import torch
         from .batch import device_info
         torch.cuda.init()  # stream creation alone does not initialise torch's device state
@@ -116,6 +118,7 @@ class ServingLoop:
         # algorithmic int64 ops per second of the whole device (sets the work
         # put into one step)
         self.rate = float(rate)
+        self.fill_long = bool(fill_long)
 
     def _split(self, sampler, comp: int) -> int:
         """Term split of a request's tiles: a 64-shot tile should take each
@@ -309,13 +312,21 @@ class ServingLoop:
                 # ---- launch a step if a stream is free
                 if len(inflight) < len(self.streams) and any(a.tiles for a in active):
                     budget = self.rate * self.step_s
-                    items, taken, est = [], [], 0.0
+                    items, taken, est, big = [], [], 0.0, 0.0
                     # round-robin over active requests, one item each per pass,
-                    # until the step holds the estimated work of step_ms and at
+                    # until the step holds the estimated work of step_ms, at
                     # least one item per block of the persistent grid (an item
-                    # is one block's work: fewer items leave SIMDs idle)
+                    # is one block's work: fewer items leave SIMDs idle) and,
+                    # when it holds long items, the work of a 1/streams share
+                    # of the grid's blocks each running one of the longest:
+                    # the steps in flight then keep every block busy while
+                    # long items run (a step of a few long items among short
+                    # ones holds the stream until they end, with most of its
+                    # blocks idle)
+                    share = self.blocks / len(self.streams) if self.fill_long else 0.0
                     def short():
-                        return est < budget or len(items) < self.min_items
+                        return (est < max(budget, big * share)
+                                or len(items) < self.min_items)
                     while short() and len(items) < self.max_items:
                         progressed = False
                         for a in active:
@@ -325,7 +336,9 @@ class ServingLoop:
                             items.append((a.slot, c, s0, cnt, L))
                             taken.append(a)
                             a.outstanding += 1
-                            est += a.sampler.ops_per_shot * cnt / max(a.sampler.n_comp, 1)
+                            w = a.sampler.ops_per_shot * cnt / max(a.sampler.n_comp, 1)
+                            est += w
+                            big = max(big, w)
                             progressed = True
                             if not short() or len(items) >= self.max_items:
                                 break

@@ -5,16 +5,21 @@
 // src/cpp/ksim/kokkos_sim.cpp:49-57,132-148), stages per-call inputs, launches
 // the gfx950 kernels and reports device-side range errors.
 #include <hip/hip_runtime.h>
+#include <sys/mman.h>
+#include <unistd.h>
 
 #include <algorithm>
 #include <array>
+#include <condition_variable>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/ksim_hip.h"
@@ -475,6 +480,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WIDE ? K
 }  // namespace ksim
 
 // ------------------------------------------------------------------ objects
+namespace ksim {
+struct HostPipe;
+void destroy_pipe(HostPipe *h);
+}  // namespace ksim
+
 struct ksim_program {
   int device = 0;
   uint32_t *d_prog = nullptr;
@@ -501,6 +511,7 @@ struct ksim_program {
   size_t params_cap = 0;
   void *d_res = nullptr;
   size_t res_cap = 0;
+  HostPipe *pipe = nullptr;  // pinned chunk pipeline of the host-pointer API
 };
 
 namespace {
@@ -1001,6 +1012,242 @@ int ksim_program_check(ksim_program *p, void *stream) {
   });
 }
 
+}  // extern "C"
+
+namespace ksim {
+
+// Copies between caller memory and pinned staging, split over a few
+// persistent threads in pieces of at least 64 KiB: one thread's memcpy
+// moves a fraction of what the host memory and the PCIe link can, and a
+// fresh output array takes a page fault per 4 KiB on first touch, which
+// threads take in parallel (a 29 MB copy into a new numpy array: about
+// 8 ms on one thread, 2 ms on eight).
+class CopyPool {
+ public:
+  static CopyPool &get() {
+    static CopyPool pool;
+    return pool;
+  }
+  void copy(void *dst, const void *src, size_t n) {
+    const size_t parts = std::min<size_t>(workers_.size() + 1, std::max<size_t>(1, n >> 16));
+    if (parts <= 1) {
+      std::memcpy(dst, src, n);
+      return;
+    }
+    const size_t piece = (n + parts - 1) / parts;
+    int mine = 0;
+    {
+      std::lock_guard<std::mutex> lk(m_);
+      for (size_t i = 1; i < parts && i * piece < n; ++i) {
+        jobs_.push_back(Job{(char *)dst + i * piece, (const char *)src + i * piece,
+                            std::min(piece, n - i * piece), &mine});
+        ++mine;
+      }
+    }
+    cv_.notify_all();
+    std::memcpy(dst, src, std::min(piece, n));
+    std::unique_lock<std::mutex> lk(m_);
+    done_.wait(lk, [&] { return mine == 0; });
+  }
+
+ private:
+  struct Job {
+    char *d;
+    const char *s;
+    size_t n;
+    int *left;  // the caller's count of pieces not yet copied
+  };
+  CopyPool() {
+    const unsigned hw = std::thread::hardware_concurrency();
+    const unsigned nt = std::min(7u, hw > 1 ? hw - 1 : 0u);
+    for (unsigned i = 0; i < nt; ++i) workers_.emplace_back([this] { loop(); });
+  }
+  ~CopyPool() {
+    {
+      std::lock_guard<std::mutex> lk(m_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto &t : workers_) t.join();
+  }
+  void loop() {
+    for (;;) {
+      Job j;
+      {
+        std::unique_lock<std::mutex> lk(m_);
+        cv_.wait(lk, [&] { return stop_ || !jobs_.empty(); });
+        if (jobs_.empty()) return;
+        j = jobs_.front();
+        jobs_.pop_front();
+      }
+      std::memcpy(j.d, j.s, j.n);
+      {
+        std::lock_guard<std::mutex> lk(m_);
+        --*j.left;
+      }
+      done_.notify_all();
+    }
+  }
+  std::mutex m_;
+  std::condition_variable cv_, done_;
+  std::deque<Job> jobs_;
+  bool stop_ = false;
+  std::vector<std::thread> workers_;
+};
+
+// Host-pointer sampling of large inputs as a pipeline of chunks of whole
+// 64-shot tiles: chunk k's f (and U) rows are copied into a pinned slot and
+// sent on the upload stream while chunk k-1 runs on the compute stream and
+// chunk k-2's rows come back on the download stream.  Two slots of each.
+struct HostPipe {
+  static constexpr int S = 2;
+  hipStream_t up = nullptr, run = nullptr, down = nullptr;
+  hipEvent_t sent[S] = {}, ran[S] = {}, back[S] = {};
+  uint8_t *pin_f[S] = {}, *pin_o[S] = {};
+  double *pin_u[S] = {};
+  size_t cap_f = 0, cap_o = 0, cap_u = 0;
+};
+
+void destroy_pipe(HostPipe *h) {
+  if (!h) return;
+  for (int s = 0; s < HostPipe::S; ++s) {
+    if (h->sent[s]) (void)hipEventDestroy(h->sent[s]);
+    if (h->ran[s]) (void)hipEventDestroy(h->ran[s]);
+    if (h->back[s]) (void)hipEventDestroy(h->back[s]);
+    if (h->pin_f[s]) (void)hipHostFree(h->pin_f[s]);
+    if (h->pin_o[s]) (void)hipHostFree(h->pin_o[s]);
+    if (h->pin_u[s]) (void)hipHostFree(h->pin_u[s]);
+  }
+  if (h->up) (void)hipStreamDestroy(h->up);
+  if (h->run) (void)hipStreamDestroy(h->run);
+  if (h->down) (void)hipStreamDestroy(h->down);
+  delete h;
+}
+
+}  // namespace ksim
+
+namespace {
+
+// bytes of f and U rows per pipeline chunk, and the input size from which
+// the host path pipelines
+constexpr size_t kChunkBytes = 8u << 20;
+constexpr size_t kPipeMinBytes = 2 * kChunkBytes;
+
+HostPipe *pipe_for(ksim_program *p, size_t fb, size_t ob, size_t ub) {
+  if (!p->pipe) {
+    std::unique_ptr<HostPipe> h(new HostPipe);
+    for (hipStream_t *q : {&h->up, &h->run, &h->down})
+      hip_check(hipStreamCreateWithFlags(q, hipStreamNonBlocking), "hipStreamCreate");
+    for (int s = 0; s < HostPipe::S; ++s)
+      for (hipEvent_t *e : {&h->sent[s], &h->ran[s], &h->back[s]})
+        hip_check(hipEventCreateWithFlags(e, hipEventDisableTiming), "hipEventCreate");
+    p->pipe = h.release();
+  }
+  HostPipe *h = p->pipe;
+  auto grow = [](auto **slot, size_t &cap, size_t need) {
+    if (need <= cap) return;
+    for (int s = 0; s < HostPipe::S; ++s) {
+      if (slot[s]) hip_check(hipHostFree(slot[s]), "hipHostFree");
+      slot[s] = nullptr;
+    }
+    cap = 0;
+    for (int s = 0; s < HostPipe::S; ++s)
+      hip_check(hipHostMalloc((void **)&slot[s], need, hipHostMallocDefault), "hipHostMalloc");
+    cap = need;
+  };
+  grow(h->pin_f, h->cap_f, std::max<size_t>(fb, 64));
+  grow(h->pin_o, h->cap_o, std::max<size_t>(ob, 64));
+  if (ub) grow(h->pin_u, h->cap_u, ub);
+  return h;
+}
+
+// Map the pages of a caller's output array writable ahead of the copies
+// into it, on a thread of its own: a fresh array faults on every 4 KiB page
+// at first touch, which otherwise stalls the copies on the pipeline's
+// critical path (MADV_POPULATE_WRITE leaves the contents as they are, so
+// it may run beside the copies; kernels without it skip this).
+class Populate {
+ public:
+  Populate(void *p, size_t n) {
+    const uintptr_t pg = (uintptr_t)sysconf(_SC_PAGESIZE);
+    const uintptr_t a = ((uintptr_t)p + pg - 1) & ~(pg - 1);
+    const uintptr_t b = ((uintptr_t)p + n) & ~(pg - 1);
+    if (b > a) th_ = std::thread([a, b] {
+        constexpr int kPopulateWrite = 23;  // MADV_POPULATE_WRITE (Linux 5.14)
+        for (uintptr_t x = a; x < b; x += (4u << 20))
+          if (madvise((void *)x, std::min<uintptr_t>(4u << 20, b - x), kPopulateWrite) != 0)
+            return;
+      });
+  }
+  ~Populate() {
+    if (th_.joinable()) th_.join();
+  }
+
+ private:
+  std::thread th_;
+};
+
+// The chunked host path (same results as one launch: every shot draws the
+// stream keyed by its index in the whole call, `shot_base`).
+void sample_host_pipelined(ksim_program *p, const uint8_t *f, int64_t B, int32_t f_cols,
+                           uint64_t seed, const double *U, uint8_t *out, int out_cols,
+                           bool packed) {
+  const size_t frow = (size_t)f_cols, urow = U ? (size_t)p->tot_out * sizeof(double) : 0;
+  const size_t orow = packed ? (size_t)p->out_words * 4 : (size_t)out_cols;
+  const int64_t rows = std::max<int64_t>(64, (int64_t)(kChunkBytes / std::max<size_t>(frow + urow, 1)) & ~63);
+  const int64_t nch = (B + rows - 1) / rows;
+  ensure(p->d_f, p->f_cap, (size_t)B * frow);
+  ensure(p->d_out, p->out_cap, (size_t)B * orow);
+  if (U) ensure(p->d_U, p->U_cap, (size_t)B * urow);
+  HostPipe *h = pipe_for(p, (size_t)rows * frow, (size_t)rows * orow, (size_t)rows * urow);
+  CopyPool &cp = CopyPool::get();
+  Populate pop(out, (size_t)B * orow);
+  const bool fill = !packed && (out_cols != p->num_outputs || p->tot_out < p->num_outputs);
+  auto take_back = [&](int64_t k) {  // chunk k's rows to the caller
+    const int s = (int)(k % HostPipe::S);
+    const int64_t r0 = k * rows, n = std::min(rows, B - r0);
+    hip_check(hipEventSynchronize(h->back[s]), "hipEventSynchronize");
+    cp.copy(out + (size_t)r0 * orow, h->pin_o[s], (size_t)n * orow);
+  };
+  reset_err_async(p, h->run);
+  for (int64_t k = 0; k < nch; ++k) {
+    const int s = (int)(k % HostPipe::S);
+    const int64_t r0 = k * rows, n = std::min(rows, B - r0);
+    // the slot's previous upload is over before its pinned rows are rewritten
+    if (k >= HostPipe::S) hip_check(hipEventSynchronize(h->sent[s]), "hipEventSynchronize");
+    if (frow) {
+      cp.copy(h->pin_f[s], f + (size_t)r0 * frow, (size_t)n * frow);
+      hip_check(hipMemcpyAsync(p->d_f + (size_t)r0 * frow, h->pin_f[s], (size_t)n * frow,
+                               hipMemcpyHostToDevice, h->up), "H2D f");
+    }
+    if (urow) {
+      cp.copy(h->pin_u[s], (const uint8_t *)U + (size_t)r0 * urow, (size_t)n * urow);
+      hip_check(hipMemcpyAsync((uint8_t *)p->d_U + (size_t)r0 * urow, h->pin_u[s],
+                               (size_t)n * urow, hipMemcpyHostToDevice, h->up), "H2D U");
+    }
+    hip_check(hipEventRecord(h->sent[s], h->up), "hipEventRecord");
+    hip_check(hipStreamWaitEvent(h->run, h->sent[s], 0), "hipStreamWaitEvent");
+    uint8_t *d_out = p->d_out + (size_t)r0 * orow;
+    if (fill) fill_u8_async(d_out, 0, n * (int64_t)orow, h->run);
+    launch_sample(p, p->d_f + (size_t)r0 * frow, n, f_cols, r0, seed,
+                  U ? p->d_U + (size_t)r0 * p->tot_out : nullptr, d_out, h->run,
+                  packed ? p->out_words : 0);
+    hip_check(hipEventRecord(h->ran[s], h->run), "hipEventRecord");
+    // the slot's previous rows reach the caller before it is reused
+    if (k >= HostPipe::S) take_back(k - HostPipe::S);
+    hip_check(hipStreamWaitEvent(h->down, h->ran[s], 0), "hipStreamWaitEvent");
+    hip_check(hipMemcpyAsync(h->pin_o[s], d_out, (size_t)n * orow, hipMemcpyDeviceToHost,
+                             h->down), "D2H out");
+    hip_check(hipEventRecord(h->back[s], h->down), "hipEventRecord");
+  }
+  for (int64_t k = std::max<int64_t>(0, nch - HostPipe::S); k < nch; ++k) take_back(k);
+  check_err_sync(p, h->run);
+}
+
+}  // namespace
+
+extern "C" {
+
 static void sample_host(ksim_program *p, const uint8_t *f, int64_t B, int32_t f_cols,
                         uint64_t seed, const double *U, uint8_t *out, int out_cols,
                         bool packed = false) {
@@ -1011,6 +1258,11 @@ static void sample_host(ksim_program *p, const uint8_t *f, int64_t B, int32_t f_
   hipStream_t st = nullptr;
   const size_t fb = (size_t)B * f_cols;
   const size_t ob = packed ? (size_t)B * p->out_words * 4 : (size_t)B * out_cols;
+  const size_t ub = U ? (size_t)B * p->tot_out * sizeof(double) : 0;
+  if (fb + ub >= kPipeMinBytes && B >= 256) {
+    sample_host_pipelined(p, f, B, f_cols, seed, U, out, out_cols, packed);
+    return;
+  }
   ensure(p->d_f, p->f_cap, fb);
   ensure(p->d_out, p->out_cap, ob);
   if (fb) hip_check(hipMemcpyAsync(p->d_f, f, fb, hipMemcpyHostToDevice, st), "H2D f");
@@ -1187,6 +1439,7 @@ void ksim_program_destroy(ksim_program *p) {
   if (p->d_U) (void)hipFree(p->d_U);
   if (p->d_params) (void)hipFree(p->d_params);
   if (p->d_res) (void)hipFree(p->d_res);
+  destroy_pipe(p->pipe);
   delete p;
 }
 

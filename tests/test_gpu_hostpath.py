@@ -71,3 +71,18 @@ def test_pipelined_component_sample():
     idx = pick(B, 6, 3)
     want = exact.sample_component(gds, f[idx], exact.kokkos_uniforms_at(77, idx, 5))
     assert np.array_equal(got[idx], want)
+
+
+def test_pipelined_fed_uniforms_without_f_columns():
+    # config-1 shape (F = 0): only the fed uniforms (1M x 5 float64, 40 MB)
+    # make the call large enough for the pipeline
+    prog = synth.config1_program()
+    s = HipProgramSampler(prog)
+    B = 1_000_000
+    f = np.zeros((B, 0), np.uint8)
+    U = np.random.default_rng(8).random((B, s.tot_out))
+    got = s.sample_uniforms(f, U)
+    assert np.array_equal(got, resident(s, f, 0, U))
+    idx = pick(B, 8, 4)
+    from oracle import exact
+    assert np.array_equal(got[idx], exact.sample_program(prog, f[idx], U[idx]))

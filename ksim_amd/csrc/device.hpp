@@ -298,13 +298,37 @@ __device__ __forceinline__ void term_exps(const cword *__restrict__ th, const PW
   S += (int)((ex & 1u) << 2);
   // LIN / PAIR chunks: packed exponent adds
   for (uint32_t ch = 0; ch < nchunk; ++ch) {
-    const uint32_t nl = rp[0], npr = rp[1];
+    const uint32_t nl = rp[0], npr = rp[1], npm = rp[3];
     uint32_t x = rp[2];
     uint32_t touched = 0;
     if constexpr (LONG)
-      touched = stream_touch(ps, rp, rp + CHUNK_HDR + nl * (W32 + 1) + npr * (2 * W32 + 3) +
+      touched = stream_touch(ps, rp, rp + CHUNK_HDR + (npm + PM_GROUP - 1) / PM_GROUP * PM_HDR +
+                                         npm * W32 + nl * (W32 + 1) + npr * (2 * W32 + 3) +
                                          STREAM_AHEAD);
     rp += CHUNK_HDR;
+    // sign records (S += 2s, N += t, s, t in {-1, 0, 1}): the parities of a
+    // group go into one word, record j of n at bit 32 - n + j (each funnel
+    // shift takes bit 0 of the popcount in at the top), then two popcounts
+    // over the group's sign masks give the S and N steps; the chunk base
+    // holds the -2^30 |S-| - 2^9 |N-| of the parity-0 records counted there
+    for (uint32_t g0 = 0; g0 < npm; g0 += PM_GROUP) {
+      const uint32_t n = min(npm - g0, (uint32_t)PM_GROUP);
+      const uint32_t sp = rp[0], sm = rp[1], tp = rp[2], tm = rp[3];
+      rp += PM_HDR;
+      uint32_t pk = 0;
+      if constexpr (LONG) {
+#pragma unroll 8
+        for (uint32_t k = 0; k < n; ++k, rp += W32)
+          pk = __builtin_amdgcn_alignbit(__popc(pw.mxor(rp)), pk, 1);
+      } else {
+#pragma unroll 4
+        for (uint32_t k = 0; k < n; ++k, rp += W32)
+          pk = __builtin_amdgcn_alignbit(__popc(pw.mxor(rp)), pk, 1);
+      }
+      const uint32_t cs = __popc((pk & sp) | (~pk & sm));
+      const uint32_t cn = __popc((pk & tp) | (~pk & tm));
+      x += (cs << 30) + (cn << 9);
+    }
     // LIN records, unrolled so each scalar load fetches several records
     if constexpr (LONG) {
 #pragma unroll 8

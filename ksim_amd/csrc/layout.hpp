@@ -68,7 +68,23 @@ enum TermHdr : int {
 constexpr uint32_t TF_RESIDUAL = 1u;  // multiply by the residual (R0..R3)
 constexpr uint32_t TF_PI = 2u;        // A or B may be non-zero
 
-constexpr int CHUNK_HDR = 4;  // n_lin, n_pair, base, pad
+// A chunk is [n_lin, n_pair, base, n_pm] followed by its n_pm sign records,
+// n_lin LIN records (mask, delta) and n_pair PAIR records; n_pm + n_lin +
+// n_pair <= CHUNK_MAX.  Sign records are the LIN records whose packed delta
+// is, mod 2^32, s 2^30 + t 2^9 with s, t in {-1, 0, 1} (S += 2s mod 8,
+// N += t: every node-phase factor 1 + w^{4p + phi} with phi odd or 2 mod 4).
+// They carry no delta: they come in groups of at most PM_GROUP, each a
+// PM_HDR-dword header [S+, S-, N+, N-] and its records' masks.  The kernel
+// shifts the group's parities into one word (record j of n at bit 32 - n + j)
+// and adds 2^30 popcount((p & S+) | (~p & S-)) + 2^9 popcount((p & N+) |
+// (~p & N-)); the base holds -2^30 |S-| - 2^9 |N-|.  Blobs of version 3 and 4
+// have n_pm = 0 in every chunk.
+constexpr int CHUNK_HDR = 4;  // n_lin, n_pair, base, n_pm
+constexpr int PM_GROUP = 32;
+constexpr int PM_HDR = 4;     // S+, S-, N+, N- masks over the group's records
+inline uint64_t pm_dwords(uint64_t n_pm, uint64_t W32) {
+  return (n_pm + PM_GROUP - 1) / PM_GROUP * PM_HDR + n_pm * W32;
+}
 
 // graph header (dwords).  The terms of a graph are stored as GH_PARTS parts
 // of about equal record work (the packer deals them out heaviest first), so

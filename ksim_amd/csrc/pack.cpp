@@ -73,6 +73,15 @@ inline uint32_t pack_exp(const FacVal &e) {
          (uint32_t)e.b << PK_OB | (uint32_t)(e.s & 7) << PK_OS;
 }
 
+// A LIN delta that is, mod 2^32, s 2^30 + t 2^9 with s, t in {-1, 0, 1}
+// (S += 2s mod 8 and N += t in the packed word): a sign record.
+bool sign_delta(uint32_t d, int &s, int &t) {
+  for (s = -1; s <= 1; ++s)
+    for (t = -1; t <= 1; ++t)
+      if ((s || t) && d == (uint32_t)s * (1u << 30) + (uint32_t)t * (1u << 9)) return true;
+  return false;
+}
+
 struct Mask {
   std::vector<uint32_t> w;
   bool zero() const {
@@ -473,20 +482,50 @@ void pack_graph(const ksim_graph_view &g, std::vector<uint32_t> &out, uint32_t *
     for (const PpxRec &p : tb.ppx) {
       put_mask(rec, p.ms); put_mask(rec, p.mf); rec.push_back(p.sc); rec.push_back(p.fc);
     }
-    // chunks of at most CHUNK_MAX LIN/PAIR records
-    size_t li = 0, pi = 0;
+    // chunks of at most CHUNK_MAX sign, LIN and PAIR records
+    std::vector<const LinRec *> pm, gen;
+    std::vector<int> pm_s, pm_n;
+    for (const LinRec &l : tb.lin) {
+      int sS, sN;
+      if (sign_delta(pack_exp(l.e1) - pack_exp(l.e0), sS, sN)) {
+        pm.push_back(&l); pm_s.push_back(sS); pm_n.push_back(sN);
+      } else {
+        gen.push_back(&l);
+      }
+    }
+    size_t mi = 0, li = 0, pi = 0;
     int nchunk = 0;
     const int nlp = (int)(tb.lin.size() + tb.pair.size());
-    while (li < tb.lin.size() || pi < tb.pair.size()) {
-      int nl = (int)std::min<size_t>(tb.lin.size() - li, CHUNK_MAX);
-      int np = (int)std::min<size_t>(tb.pair.size() - pi, (size_t)(CHUNK_MAX - nl));
+    while (mi < pm.size() || li < gen.size() || pi < tb.pair.size()) {
+      int nm = (int)std::min<size_t>(pm.size() - mi, CHUNK_MAX);
+      int nl = (int)std::min<size_t>(gen.size() - li, (size_t)(CHUNK_MAX - nm));
+      int np = (int)std::min<size_t>(tb.pair.size() - pi, (size_t)(CHUNK_MAX - nm - nl));
       uint32_t base = 0;
-      for (int q = 0; q < nl; ++q) base += pack_exp(tb.lin[li + q].e0);
+      for (int q = 0; q < nm; ++q) base += pack_exp(pm[mi + q]->e0);
+      for (int q = 0; q < nl; ++q) base += pack_exp(gen[li + q]->e0);
       for (int q = 0; q < np; ++q) base += pack_exp(tb.pair[pi + q].e00);
-      rec.push_back((uint32_t)nl); rec.push_back((uint32_t)np); rec.push_back(base);
-      rec.push_back(0u);
+      const size_t chdr = rec.size();
+      rec.push_back((uint32_t)nl); rec.push_back((uint32_t)np); rec.push_back(0u);
+      rec.push_back((uint32_t)nm);
+      for (int g0 = 0; g0 < nm; g0 += PM_GROUP) {
+        const int n = std::min(nm - g0, PM_GROUP);
+        uint32_t sp = 0, sm = 0, np_ = 0, nn = 0;
+        for (int j = 0; j < n; ++j) {
+          const uint32_t bit = 1u << (PM_GROUP - n + j);
+          const int sS = pm_s[mi + g0 + j], sN = pm_n[mi + g0 + j];
+          if (sS > 0) sp |= bit;
+          if (sS < 0) sm |= bit;
+          if (sN > 0) np_ |= bit;
+          if (sN < 0) nn |= bit;
+        }
+        // the popcounts over (p & X+) | (~p & X-) count X- records of parity 0
+        base -= ((uint32_t)__builtin_popcount(sm) << 30) + ((uint32_t)__builtin_popcount(nn) << 9);
+        rec.push_back(sp); rec.push_back(sm); rec.push_back(np_); rec.push_back(nn);
+        for (int j = 0; j < n; ++j) put_mask(rec, pm[mi + g0 + j]->m);
+      }
+      rec[chdr + 2] = base;
       for (int q = 0; q < nl; ++q) {
-        const LinRec &l = tb.lin[li + q];
+        const LinRec &l = *gen[li + q];
         put_mask(rec, l.m);
         rec.push_back(pack_exp(l.e1) - pack_exp(l.e0));
       }
@@ -497,7 +536,7 @@ void pack_graph(const ksim_graph_view &g, std::vector<uint32_t> &out, uint32_t *
                        p11 = pack_exp(p.e11);
         rec.push_back(p10 - p00); rec.push_back(p01 - p00); rec.push_back(p11 - p10 - p01 + p00);
       }
-      li += nl; pi += np; ++nchunk;
+      mi += nm; li += nl; pi += np; ++nchunk;
     }
     uint32_t *h = &out[hdr];
     h[TH_NPROJ] = (uint32_t)tb.proj.size();

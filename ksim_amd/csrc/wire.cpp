@@ -120,9 +120,9 @@ void validate_words(const uint32_t *w, size_t n) {
         uint64_t rp = to + TH_DWORDS + sz;
         for (uint32_t k = 0; k < th[TH_NCHUNK]; ++k) {
           need(rp + CHUNK_HDR, "record chunk");
-          const uint64_t nl = w[rp], np = w[rp + 1];
-          if (nl + np > (uint64_t)CHUNK_MAX) bad("record chunk longer than its limit");
-          const uint64_t cs = CHUNK_HDR + nl * (W32 + 1) + np * (2 * W32 + 3);
+          const uint64_t nl = w[rp], np = w[rp + 1], nm = w[rp + 3];
+          if (nl + np + nm > (uint64_t)CHUNK_MAX) bad("record chunk longer than its limit");
+          const uint64_t cs = CHUNK_HDR + pm_dwords(nm, W32) + nl * (W32 + 1) + np * (2 * W32 + 3);
           sz += cs;
           rp += cs;
         }
@@ -142,7 +142,7 @@ std::vector<uint32_t> words_from_blob(const uint8_t *blob, size_t n, double *sta
   if (n < WIRE_HEADER) bad("shorter than its header");
   if (std::memcmp(blob, kMagic, 8) != 0) bad("bad magic");
   const uint32_t version = rd<uint32_t>(blob + 8);
-  if (version != WIRE_VERSION && version != 3) bad("unsupported version");
+  if (version != WIRE_VERSION && version != 4 && version != 3) bad("unsupported version");
   if (rd<uint32_t>(blob + 12) != WIRE_HEADER) bad("header size");
   const uint64_t nw = rd<uint64_t>(blob + 16);
   if (nw < PH_DWORDS || nw > (n - WIRE_HEADER) / 4 || WIRE_HEADER + nw * 4 != n)

@@ -15,6 +15,7 @@ PK_OV, PK_WV, PK_ON, PK_WN, PK_OA, PK_WA, PK_OB, PK_WB, PK_OS = 0, 9, 9, 8, 17, 
 TH = dict(NPROJ=0, NPPX=1, NCHUNK=2, FLAGS=3, S=4, V=5, N=6, A=7, B=8, E=9, R0=10, SIZE=14)
 TH_DWORDS = 16
 CHUNK_HDR = 4
+PM_GROUP, PM_HDR = 32, 4
 GH_DWORDS = 16
 CH = dict(NGRAPHS=0, F=1, NOUT=2, WMAX=3, COLBASE=4, FSEL=5, DEST=6, GRAPH=7)
 PH_DWORDS = 8
@@ -121,8 +122,21 @@ class Packed:
                 rp += 2 * W32 + 2
             S += (ex & 1) << 2
             for _ in range(h[TH["NCHUNK"]]):
-                nl, npr, x = w[rp], w[rp + 1], w[rp + 2]
+                nl, npr, x, npm = w[rp], w[rp + 1], w[rp + 2], w[rp + 3]
                 rp += CHUNK_HDR
+                for g0 in range(0, npm, PM_GROUP):
+                    n = min(npm - g0, PM_GROUP)
+                    sp, sm, tp, tm = w[rp:rp + PM_HDR]
+                    rp += PM_HDR
+                    pk = 0
+                    for j in range(n):
+                        if popc_and(pv, w[rp:rp + W32]) & 1:
+                            pk |= 1 << (PM_GROUP - n + j)
+                        rp += W32
+                    npk = ~pk & 0xFFFFFFFF
+                    cs = bin((pk & sp) | (npk & sm)).count("1")
+                    cn = bin((pk & tp) | (npk & tm)).count("1")
+                    x = (x + (cs << 30) + (cn << 9)) & 0xFFFFFFFF
                 for _ in range(nl):
                     if popc_and(pv, w[rp:rp + W32]) & 1:
                         x = (x + w[rp + W32]) & 0xFFFFFFFF

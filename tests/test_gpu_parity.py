@@ -279,6 +279,30 @@ def test_program_blob_round_trip():
         HipProgramSampler.from_bytes(blob[:-4])
 
 
+@pytest.mark.parametrize("name,make", [
+    ("blob_v4_two_comp.bin",
+     lambda: synth.shape_program(n_q=6, t=6, F=8, n_out=4, seed=3, n_components=2)),
+    ("blob_v4_wide.bin", synth.wide_exponent_program)])
+def test_version_4_blobs_sample_as_the_oracle(name, make):
+    # blobs of the version-4 packer hold every LIN record with its own delta
+    # (no sign-record groups): uploaded as they are, they give the oracle's
+    # shot stream and the same rows as the current packing of the program
+    import os
+    with open(os.path.join(os.path.dirname(__file__), "golden", name), "rb") as fh:
+        blob = fh.read()
+    prog = make()
+    old = HipProgramSampler.from_bytes(blob)
+    new = HipProgramSampler(prog)
+    assert old.to_bytes() != new.to_bytes()
+    B = 3000
+    f = synth.random_f(B, old.min_num_f, 0.5, 8)
+    U = np.random.default_rng(9).random((B, old.tot_out))
+    got = old.sample_uniforms(f, U)
+    assert np.array_equal(got, new.sample_uniforms(f, U))
+    assert np.array_equal(got[:300], exact.sample_program(prog, f[:300], U[:300]))
+    assert np.array_equal(old.sample(f, 5), new.sample(f, 5))
+
+
 @pytest.mark.parametrize("which", ["single", "multi", "permuted", "wide"])
 def test_packed_rows_equal_byte_rows(which):
     from ksim_amd.sampler import unpack_outputs

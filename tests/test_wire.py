@@ -1,6 +1,7 @@
 """Host-only: the program wire format.  A blob holds exactly the packed
 words, round-trips through validation, and every structural corruption a
 kernel could follow out of bounds is rejected before upload."""
+import os
 import struct
 
 import numpy as np
@@ -10,6 +11,7 @@ from ksim_amd import synth
 from ksim_amd.sampler import blob_stats, pack_words, program_blob
 
 HDR = 128
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 def fnv1a64(b: bytes) -> int:
@@ -34,6 +36,16 @@ def with_words(blob: bytes, w: np.ndarray) -> bytes:
     return bytes(head) + body
 
 
+# blobs written by the version-4 packer (this repository before sign-record
+# groups) for these programs: program_blob(make()) at that version
+LEGACY = {
+    "two_comp": (lambda: synth.shape_program(n_q=6, t=6, F=8, n_out=4, seed=3, n_components=2),
+                 open(os.path.join(GOLDEN, "blob_v4_two_comp.bin"), "rb").read()),
+    "wide": (synth.wide_exponent_program,
+             open(os.path.join(GOLDEN, "blob_v4_wide.bin"), "rb").read()),
+}
+
+
 @pytest.fixture(scope="module")
 def prog():
     return synth.shape_program(n_q=6, t=6, F=8, n_out=4, seed=3, n_components=2)
@@ -48,7 +60,7 @@ def test_blob_holds_the_packed_words(prog, blob):
     st = {}
     w = pack_words(prog, st)
     assert blob[:8] == b"KSIMPRG\0"
-    assert struct.unpack("<II", blob[8:16]) == (4, HDR)
+    assert struct.unpack("<II", blob[8:16]) == (5, HDR)
     assert np.array_equal(words_of(blob), w)
     assert struct.unpack("<Q", blob[24:32])[0] == fnv1a64(w.tobytes())
     got = blob_stats(blob)
@@ -57,11 +69,61 @@ def test_blob_holds_the_packed_words(prog, blob):
 
 
 def test_version_3_blobs_still_load(prog, blob):
-    # a version-3 blob (no approximate factors, at most 8 words) is a valid
-    # version-4 blob with the older number
-    b = bytearray(blob)
+    # version 3 differs from version 4 only in what it could not hold
+    # (approximate factors, more than 8 words): its number is accepted
+    b = bytearray(LEGACY["two_comp"][1])
     b[8:12] = struct.pack("<I", 3)
-    assert blob_stats(bytes(b))["words"] == blob_stats(blob)["words"]
+    assert blob_stats(bytes(b))["words"] == blob_stats(LEGACY["two_comp"][1])["words"]
+
+
+@pytest.mark.parametrize("name", list(LEGACY))
+def test_version_4_blobs_evaluate_as_the_oracle(name):
+    # blobs written by the version-4 packer: every LIN record with its own
+    # delta, no sign-record groups.  They validate, and their words evaluate
+    # (CPU interpreter of the device format) to the oracle's exact values
+    from oracle import exact
+    from tests.devfmt import Packed
+    from ksim_amd.program import graph_to_dict
+    from ksim_amd.sampler import lookup_tables
+    make, b = LEGACY[name]
+    assert struct.unpack("<I", b[8:12])[0] == 4
+    words = words_of(b)
+    blob_stats(b)
+    assert not np.array_equal(words, pack_words(make())), "the current packer writes sign records"
+    pk = Packed(words, lookup_tables())
+    prog = make()
+    rng = np.random.default_rng(5)
+    for c, comp in enumerate(prog.components):
+        for i, g in enumerate(comp.graphs):
+            d = graph_to_dict(g)
+            P = g.np_params.shape[2] if g.np_params.ndim == 3 else 0
+            for _ in range(4):
+                bits = rng.integers(0, 2, size=P)
+                coeffs, e0 = pk.eval_graph(c, i, bits)
+                assert exact.canonical(coeffs, e0) == exact.graph_value(d, bits), (name, c, i)
+
+
+def test_sign_records_written_and_counted():
+    # node phases 1 + w^phi (phi odd or 2 mod 4) become sign records: the
+    # packed words of a program with such factors are smaller than one
+    # (mask, delta) pair per LIN record would make them
+    from tests.devfmt import CH, GH_DWORDS, PH_DWORDS, TH, TH_DWORDS
+    w = [int(x) for x in pack_words(synth.shape_program(n_q=12, t=8, F=16, n_out=4, seed=2))]
+    off = w[PH_DWORDS]
+    n_pm = 0
+    for i in range(w[off + CH["NGRAPHS"]]):
+        gh = w[w[off + CH["GRAPH"]] + i * GH_DWORDS:][:GH_DWORDS]
+        W32, th = gh[1], gh[3]
+        for _ in range(gh[0]):
+            h = w[th:th + TH_DWORDS]
+            rp = th + TH_DWORDS + h[TH["NPROJ"]] * (W32 + 1) + h[TH["NPPX"]] * (2 * W32 + 2)
+            for _ in range(h[TH["NCHUNK"]]):
+                nl, npr, npm = w[rp], w[rp + 1], w[rp + 3]
+                assert nl + npr + npm <= 63
+                n_pm += npm
+                rp += 4 + (npm + 31) // 32 * 4 + npm * W32 + nl * (W32 + 1) + npr * (2 * W32 + 3)
+            th += TH_DWORDS + h[TH["SIZE"]]
+    assert n_pm > 0
 
 
 def test_blob_of_every_golden_program(golden_files):

@@ -390,25 +390,29 @@ __device__ __forceinline__ TermIdx term_index(int S, int V, int N) {
 }
 
 // the first digit plane is read from global memory: its central rows stay
-// resident in the CU's vector L1 cache (an LDS copy measured no faster)
+// resident in the CU's vector L1 cache (an LDS copy measured no faster).
+// A vanished term, or a row beyond the table (an error), reads the zero
+// entry, so every later step may add the lane's value unconditionally.
 __device__ __forceinline__ int4 table_digit0(const TermIdx &ti, bool vanished, const LdsTables *lt,
                                              const Tables &tb, uint32_t &err) {
-  if (ti.in_tab) return tb.tab[ti.gi];
-  if (!vanished) err |= ERR_TAB_RANGE;
-  return make_int4(0, 0, 0, 0);
+  if (!ti.in_tab && !vanished) err |= ERR_TAB_RANGE;
+  return tb.tab[ti.in_tab && !vanished ? ti.gi : TAB_ZERO];
 }
 
 // e * pi^A pibar^B for a table digit e (|e| < 2^30; pi powers < 2^24): four
-// signed 32 x 32 -> 64-bit products per coefficient, exact in int64
+// signed 32 x 32 -> 64-bit products per coefficient, exact in int64.  A term
+// that may carry pi powers (TF_PI, the same for the whole wave) always takes
+// the product, pi^0 pibar^0 = 1 included: most waves hold some lane with a
+// power, and a branch per lane costs more than the products it skips.
+// Exponents of PI_MAX or more break the packer's bound: they set the error
+// bit (the host then fails the call) and read a wrapped table row.
 __device__ __forceinline__ void times_pi(int4 e, uint32_t flags, int A, int Bq,
                                          const LdsTables *lt, int64_t &x0, int64_t &x1,
                                          int64_t &x2, int64_t &x3, uint32_t &err) {
-  if ((flags & TF_PI) && (A | Bq)) {
-    if (A >= PI_MAX || Bq >= PI_MAX) {
-      err |= ERR_PI_RANGE;
-      e = make_int4(0, 0, 0, 0);
-    }
-    const int4 p = lt->pi[min(A, PI_MAX - 1) * PI_MAX + min(Bq, PI_MAX - 1)];
+  static_assert((PI_MAX & (PI_MAX - 1)) == 0, "pi table side must be a power of two");
+  if (flags & TF_PI) {
+    if ((uint32_t)(A | Bq) >= (uint32_t)PI_MAX) err |= ERR_PI_RANGE;
+    const int4 p = lt->pi[(A & (PI_MAX - 1)) * PI_MAX + (Bq & (PI_MAX - 1))];
     x0 = (int64_t)e.x * p.x + (int64_t)e.y * p.w - (int64_t)e.z * p.z + (int64_t)e.w * p.y;
     x1 = (int64_t)e.x * p.y + (int64_t)e.y * p.x + (int64_t)e.z * p.w + (int64_t)e.w * p.z;
     x2 = (int64_t)e.x * p.z + (int64_t)e.y * p.y + (int64_t)e.z * p.x - (int64_t)e.w * p.w;
@@ -426,10 +430,26 @@ __device__ __forceinline__ Acc4 add_term_digit(Acc4 acc, int4 e, bool zero, uint
                                                int Bq, const LdsTables *lt,
                                                const cword *__restrict__ th, int sh,
                                                uint32_t &err) {
-  const int32_t keep = (int32_t)zero - 1;
-  e.x &= keep; e.y &= keep; e.z &= keep; e.w &= keep;
+  // e is zero on the lanes that add nothing (table_digit0, or the caller's
+  // select of a further digit)
+  const uint64_t live = __builtin_amdgcn_ballot_w64(!zero);
+  if (live == 0) return acc;
   int64_t x0, x1, x2, x3;
   times_pi(e, flags, A, Bq, lt, x0, x1, x2, x3, err);
+  // add X * 2^sh into the four 128-bit sums.  The shift is almost always the
+  // same on every lane that adds something (it varies only through
+  // phase-pair records), so the common path adds 32-bit limbs at a
+  // wave-uniform offset, its range checked once for the wave.
+  const int sh0 = __builtin_amdgcn_readlane(sh, __builtin_ctzll(live));
+  if (Uniform && !(flags & TF_RESIDUAL) && (uint32_t)sh0 <= (uint32_t)SHIFT_MAX &&
+      __builtin_amdgcn_ballot_w64(!zero && sh != sh0) == 0) {
+    const int rr = sh0 & 31, L = sh0 >> 5;
+    add_shifted_uniform(acc.a, x0, rr, L);
+    add_shifted_uniform(acc.b, x1, rr, L);
+    add_shifted_uniform(acc.c, x2, rr, L);
+    add_shifted_uniform(acc.d, x3, rr, L);
+    return acc;
+  }
   if (sh < 0 || sh > SHIFT_MAX) {
     if (x0 | x1 | x2 | x3) err |= ERR_SHIFT;
     x0 = x1 = x2 = x3 = 0;
@@ -440,18 +460,7 @@ __device__ __forceinline__ Acc4 add_term_digit(Acc4 acc, int4 e, bool zero, uint
     acc = add_residual_term(acc, x0, x1, x2, x3, (int32_t)th[TH_R0], (int32_t)th[TH_R1],
                             (int32_t)th[TH_R2], (int32_t)th[TH_R3], sh);
   } else {
-    // add X * 2^sh into the four 128-bit sums.  The shift is almost always
-    // the same on every lane (it varies only through phase-pair records),
-    // so the common path adds 32-bit limbs at a wave-uniform offset.
-    const int sh0 = __builtin_amdgcn_readfirstlane(sh);
-    if (zero) sh = sh0;  // a zero addend adds nothing at any shift
-    if (Uniform && __builtin_amdgcn_ballot_w64(sh != sh0) == 0) {
-      const int rr = sh0 & 31, L = sh0 >> 5;
-      add_shifted_uniform(acc.a, x0, rr, L);
-      add_shifted_uniform(acc.b, x1, rr, L);
-      add_shifted_uniform(acc.c, x2, rr, L);
-      add_shifted_uniform(acc.d, x3, rr, L);
-    } else {
+    {
       acc.a += (i128_t)((u128_t)(i128_t)x0 << sh);
       acc.b += (i128_t)((u128_t)(i128_t)x1 << sh);
       acc.c += (i128_t)((u128_t)(i128_t)x2 << sh);
@@ -472,8 +481,7 @@ __device__ __forceinline__ void add_term_narrow(Acc64 &n, int4 e, bool zero, uin
                                                 int A, int Bq, const LdsTables *lt,
                                                 const cword *__restrict__ th, int sh,
                                                 uint32_t &err) {
-  const int32_t keep = (int32_t)zero - 1;
-  e.x &= keep; e.y &= keep; e.z &= keep; e.w &= keep;
+  // e is zero on the lanes whose term vanished (table_digit0)
   int64_t x0, x1, x2, x3;
   times_pi(e, flags, A, Bq, lt, x0, x1, x2, x3, err);
   if (flags & TF_RESIDUAL)

@@ -42,6 +42,10 @@ constexpr int TAB_DIGITS = 3;
 constexpr int TAB_DIGIT_BITS = 31;
 constexpr int TAB_N = (2 * TAB_W + 1) * 4 * 8;  // entries per digit plane
 inline int tab_index(int n, int r, int s) { return ((n + TAB_W) * 4 + r) * 8 + s; }
+// An all-zero table entry: plane 1 on the single-digit row n = 0.  Lanes whose
+// term vanished read it, so their addend is zero without a select per
+// coefficient.
+constexpr int TAB_ZERO = TAB_N + ((0 + TAB_W) * 4 + 0) * 8 + 0;
 // float64 magnitudes |w^s (1+w)^r (1+sqrt2)^n| of the table entries (by the
 // sampler's magnitude sequence), same indexing; rows |n| <= MAG_LDS_R in LDS
 constexpr int MAG_LDS_R = 12;

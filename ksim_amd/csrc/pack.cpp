@@ -168,6 +168,8 @@ const int32_t *materialise_table() {
               throw std::logic_error("central table row is not a single digit");
           }
         }
+    for (int k = 0; k < 4; ++k)
+      if (tab[(size_t)TAB_ZERO * 4 + k] != 0) throw std::logic_error("zero table entry is not zero");
   });
   return tab.data();
 }

@@ -393,26 +393,49 @@ __device__ __forceinline__ TermIdx term_index(int S, int V, int N) {
 // resident in the CU's vector L1 cache (an LDS copy measured no faster).
 // A vanished term, or a row beyond the table (an error), reads the zero
 // entry, so every later step may add the lane's value unconditionally.
+template <bool ZeroRow>
 __device__ __forceinline__ int4 table_digit0(const TermIdx &ti, bool vanished, const LdsTables *lt,
                                              const Tables &tb, uint32_t &err) {
-  if (!ti.in_tab && !vanished) err |= ERR_TAB_RANGE;
-  return tb.tab[ti.in_tab && !vanished ? ti.gi : TAB_ZERO];
+  if constexpr (ZeroRow) {
+    if (!ti.in_tab && !vanished) err |= ERR_TAB_RANGE;
+    return tb.tab[ti.in_tab && !vanished ? ti.gi : TAB_ZERO];
+  } else {
+    // the 64-bit path masks vanished lanes itself (add_term_narrow)
+    if (ti.in_tab) return tb.tab[ti.gi];
+    if (!vanished) err |= ERR_TAB_RANGE;
+    return make_int4(0, 0, 0, 0);
+  }
 }
 
 // e * pi^A pibar^B for a table digit e (|e| < 2^30; pi powers < 2^24): four
-// signed 32 x 32 -> 64-bit products per coefficient, exact in int64.  A term
-// that may carry pi powers (TF_PI, the same for the whole wave) always takes
-// the product, pi^0 pibar^0 = 1 included: most waves hold some lane with a
-// power, and a branch per lane costs more than the products it skips.
+// signed 32 x 32 -> 64-bit products per coefficient, exact in int64.
+// Always (the 128-bit sums, which serve the programs with many terms and
+// phase pairs): a term that may carry pi powers (TF_PI, the same for the
+// whole wave) takes the product, pi^0 pibar^0 = 1 included, since most waves
+// hold some lane with a power and a branch per lane costs more than the
+// products it skips (config-4 batch 4 % faster).  Otherwise the lanes whose
+// exponents are both zero skip it (the 64-bit sums of short Clifford-like
+// graphs, where powers are rare: config 2 7 % faster that way).
 // Exponents of PI_MAX or more break the packer's bound: they set the error
 // bit (the host then fails the call) and read a wrapped table row.
+template <bool Always>
 __device__ __forceinline__ void times_pi(int4 e, uint32_t flags, int A, int Bq,
                                          const LdsTables *lt, int64_t &x0, int64_t &x1,
                                          int64_t &x2, int64_t &x3, uint32_t &err) {
   static_assert((PI_MAX & (PI_MAX - 1)) == 0, "pi table side must be a power of two");
-  if (flags & TF_PI) {
-    if ((uint32_t)(A | Bq) >= (uint32_t)PI_MAX) err |= ERR_PI_RANGE;
-    const int4 p = lt->pi[(A & (PI_MAX - 1)) * PI_MAX + (Bq & (PI_MAX - 1))];
+  if ((flags & TF_PI) && (Always || (A | Bq))) {
+    int pi_row;
+    if constexpr (Always) {
+      if ((uint32_t)(A | Bq) >= (uint32_t)PI_MAX) err |= ERR_PI_RANGE;
+      pi_row = (A & (PI_MAX - 1)) * PI_MAX + (Bq & (PI_MAX - 1));
+    } else {
+      if (A >= PI_MAX || Bq >= PI_MAX) {
+        err |= ERR_PI_RANGE;
+        e = make_int4(0, 0, 0, 0);
+      }
+      pi_row = min(A, PI_MAX - 1) * PI_MAX + min(Bq, PI_MAX - 1);
+    }
+    const int4 p = lt->pi[pi_row];
     x0 = (int64_t)e.x * p.x + (int64_t)e.y * p.w - (int64_t)e.z * p.z + (int64_t)e.w * p.y;
     x1 = (int64_t)e.x * p.y + (int64_t)e.y * p.x + (int64_t)e.z * p.w + (int64_t)e.w * p.z;
     x2 = (int64_t)e.x * p.z + (int64_t)e.y * p.y + (int64_t)e.z * p.x - (int64_t)e.w * p.w;
@@ -435,7 +458,7 @@ __device__ __forceinline__ Acc4 add_term_digit(Acc4 acc, int4 e, bool zero, uint
   const uint64_t live = __builtin_amdgcn_ballot_w64(!zero);
   if (live == 0) return acc;
   int64_t x0, x1, x2, x3;
-  times_pi(e, flags, A, Bq, lt, x0, x1, x2, x3, err);
+  times_pi<true>(e, flags, A, Bq, lt, x0, x1, x2, x3, err);
   // add X * 2^sh into the four 128-bit sums.  The shift is almost always the
   // same on every lane that adds something (it varies only through
   // phase-pair records), so the common path adds 32-bit limbs at a
@@ -481,9 +504,10 @@ __device__ __forceinline__ void add_term_narrow(Acc64 &n, int4 e, bool zero, uin
                                                 int A, int Bq, const LdsTables *lt,
                                                 const cword *__restrict__ th, int sh,
                                                 uint32_t &err) {
-  // e is zero on the lanes whose term vanished (table_digit0)
+  const int32_t keep = (int32_t)zero - 1;
+  e.x &= keep; e.y &= keep; e.z &= keep; e.w &= keep;
   int64_t x0, x1, x2, x3;
-  times_pi(e, flags, A, Bq, lt, x0, x1, x2, x3, err);
+  times_pi<false>(e, flags, A, Bq, lt, x0, x1, x2, x3, err);
   if (flags & TF_RESIDUAL)
     zmul64(x0, x1, x2, x3, (int32_t)th[TH_R0], (int32_t)th[TH_R1], (int32_t)th[TH_R2],
            (int32_t)th[TH_R3]);
@@ -553,7 +577,7 @@ __device__ __forceinline__ void sum_terms(const cword *__restrict__ th, uint32_t
       }
       g.pm = vanished ? 0.0 : __builtin_ldexp(m, sh + e0);
     } else {
-      const int4 e = table_digit0(ti, vanished, lt, tb, err);
+      const int4 e = table_digit0<WIDE>(ti, vanished, lt, tb, err);
       if (!WIDE || (gflags & GF_NARROW)) {
         if (!WIDE && !(gflags & GF_NARROW)) err |= ERR_WIDE;
         add_term_narrow(g.n, e, vanished, flags, A, Bq, lt, th, sh, err);

@@ -448,6 +448,15 @@ __device__ __forceinline__ void times_pi(int4 e, uint32_t flags, int A, int Bq,
 // acc + e * pi^{A,B} * R * 2^sh for one table digit e (|e| < 2^30).
 // zero: the lane adds nothing (a vanished term, or no digit at this place).
 // Uniform: try the wave-uniform shift path first.
+// The low 64 bits of a 128-bit sum as a 64-bit sum: a GF_NARROW graph's exact
+// sum (below 2^62 by the packer's bound) kept in the registers of the
+// 128-bit sums, so the kernels with both paths hold one set of sums.
+__device__ __forceinline__ void add_low64(i128_t &acc, int64_t v) {
+  const u128_t u = (u128_t)acc;
+  acc = (i128_t)((u & ~(u128_t)~0ull) | (u128_t)((uint64_t)u + (uint64_t)v));
+}
+__device__ __forceinline__ int64_t low64(i128_t acc) { return (int64_t)(uint64_t)(u128_t)acc; }
+
 template <bool Uniform>
 __device__ __forceinline__ Acc4 add_term_digit(Acc4 acc, int4 e, bool zero, uint32_t flags, int A,
                                                int Bq, const LdsTables *lt,
@@ -500,7 +509,9 @@ struct Acc64 {
   int64_t a, b, c, d;
 };
 
-__device__ __forceinline__ void add_term_narrow(Acc64 &n, int4 e, bool zero, uint32_t flags,
+// n: the 64-bit sums, or the low halves of the 128-bit sums (Acc4)
+template <class AccT>
+__device__ __forceinline__ void add_term_narrow(AccT &n, int4 e, bool zero, uint32_t flags,
                                                 int A, int Bq, const LdsTables *lt,
                                                 const cword *__restrict__ th, int sh,
                                                 uint32_t &err) {
@@ -516,10 +527,17 @@ __device__ __forceinline__ void add_term_narrow(Acc64 &n, int4 e, bool zero, uin
     x0 = x1 = x2 = x3 = 0;
     sh = 0;
   }
-  n.a += (int64_t)((uint64_t)x0 << sh);
-  n.b += (int64_t)((uint64_t)x1 << sh);
-  n.c += (int64_t)((uint64_t)x2 << sh);
-  n.d += (int64_t)((uint64_t)x3 << sh);
+  if constexpr (sizeof(n.a) == 8) {
+    n.a += (int64_t)((uint64_t)x0 << sh);
+    n.b += (int64_t)((uint64_t)x1 << sh);
+    n.c += (int64_t)((uint64_t)x2 << sh);
+    n.d += (int64_t)((uint64_t)x3 << sh);
+  } else {
+    add_low64(n.a, (int64_t)((uint64_t)x0 << sh));
+    add_low64(n.b, (int64_t)((uint64_t)x1 << sh));
+    add_low64(n.c, (int64_t)((uint64_t)x2 << sh));
+    add_low64(n.d, (int64_t)((uint64_t)x3 << sh));
+  }
 }
 
 
@@ -532,15 +550,17 @@ struct GraphSum {
 };
 
 // clear the sums the graph's flags select (the others stay unset: the
-// evaluation never reads them)
+// evaluation never reads them).  Kernels with the 128-bit path sum a
+// GF_NARROW graph in the low halves of the 128-bit sums (add_low64), so
+// only one set of sums is live across the term loop.
+template <bool WIDE>
 __device__ __forceinline__ void graph_sum_clear(GraphSum &g, uint32_t gflags) {
   if (gflags & GF_MONO) {
     g.pm = 0.0;
-  } else if (gflags & GF_NARROW) {
-    g.n.a = g.n.b = g.n.c = g.n.d = 0;
-  } else {
-    g.n.a = g.n.b = g.n.c = g.n.d = 0;  // kernels without the 128-bit path
+  } else if constexpr (WIDE) {
     g.acc.a = g.acc.b = g.acc.c = g.acc.d = 0;
+  } else {
+    g.n.a = g.n.b = g.n.c = g.n.d = 0;
   }
 }
 
@@ -580,7 +600,10 @@ __device__ __forceinline__ void sum_terms(const cword *__restrict__ th, uint32_t
       const int4 e = table_digit0<WIDE>(ti, vanished, lt, tb, err);
       if (!WIDE || (gflags & GF_NARROW)) {
         if (!WIDE && !(gflags & GF_NARROW)) err |= ERR_WIDE;
-        add_term_narrow(g.n, e, vanished, flags, A, Bq, lt, th, sh, err);
+        if constexpr (WIDE)
+          add_term_narrow(g.acc, e, vanished, flags, A, Bq, lt, th, sh, err);
+        else
+          add_term_narrow(g.n, e, vanished, flags, A, Bq, lt, th, sh, err);
       } else if (kApprox<PW> && WIDE && (gflags & GF_APPROX)) {
         // the term's exact value at its own exponent (digit planes at their
         // own shifts), cast to float64, times its complex64 factor, added
@@ -728,7 +751,7 @@ __device__ __forceinline__ double graph_prob(const cword *__restrict__ P,
     if (p1 < GH_PARTS) t1 = gh[GH_PART_T1 + p1 - 1];
   }
   GraphSum g;
-  graph_sum_clear(g, gflags);
+  graph_sum_clear<WIDE>(g, gflags);
   if constexpr (kApprox<PW> && WIDE) {
     if (gflags & GF_APPROX) {
       g.fr = g.fi = 0.0;
@@ -739,6 +762,12 @@ __device__ __forceinline__ double graph_prob(const cword *__restrict__ P,
   sum_terms<PW, WIDE>(P + off, t1 - t0, gflags, e0, pw, lt, tb, g, err);
   if (gflags & GF_MONO) return g.pm;
   const bool narrow = !WIDE || (gflags & GF_NARROW);
+  if constexpr (WIDE) {
+    if (narrow) {
+      g.n.a = low64(g.acc.a); g.n.b = low64(g.acc.b);
+      g.n.c = low64(g.acc.c); g.n.d = low64(g.acc.d);
+    }
+  }
   if (share) exchange_sums(g, narrow, sp, xs);
   if (narrow) return magnitude(widen(g.n), e0);
   return magnitude(g.acc, e0);
@@ -773,7 +802,7 @@ __device__ __forceinline__ void eval_graph(const cword *__restrict__ P,
                                            const LdsTables *lt, const Tables &tb, Acc4 &acc,
                                            uint32_t &err) {
   GraphSum g;
-  graph_sum_clear(g, 0u);
+  graph_sum_clear<true>(g, 0u);
   sum_terms<PW>(P + gh[GH_TERM_OFF], gh[GH_NTERMS], 0u, (int)gh[GH_E0], pw, lt, tb, g, err);
   acc = g.acc;
 }

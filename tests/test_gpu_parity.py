@@ -303,6 +303,20 @@ def test_version_4_blobs_sample_as_the_oracle(name, make):
     assert np.array_equal(old.sample(f, 5), new.sample(f, 5))
 
 
+@pytest.mark.parametrize("n_sign", [1, 31, 32, 33, 64, 65, 97])
+def test_sign_record_groups(n_sign):
+    # sign-record groups of every size 1-32, several per chunk, mixed with LIN
+    # and PAIR records: exact values and the fed-uniform stream as the oracle
+    from tests.test_packer import sign_record_program
+    prog = sign_record_program(n_sign, seed=n_sign)
+    assert gpu_exact_matches(prog, rows=16, seed=n_sign) > 0
+    s = HipProgramSampler(prog)
+    B = 500
+    f = synth.random_f(B, s.min_num_f, 0.5, n_sign)
+    U = np.random.default_rng(n_sign).random((B, s.tot_out))
+    assert np.array_equal(s.sample_uniforms(f, U), exact.sample_program(prog, f, U))
+
+
 @pytest.mark.parametrize("which", ["single", "multi", "permuted", "wide"])
 def test_packed_rows_equal_byte_rows(which):
     from ksim_amd.sampler import unpack_outputs

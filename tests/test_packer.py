@@ -11,7 +11,7 @@ from ksim_amd import synth
 from ksim_amd.program import FlatComponent, FlatProgram, graph_to_dict
 from ksim_amd.sampler import lookup_tables, pack_words
 from oracle import exact
-from tests.devfmt import Packed
+from tests.devfmt import CH, GH_DWORDS, PH_DWORDS, TH, TH_DWORDS, Packed
 from tests.golden.golden_io import load_program
 
 
@@ -154,6 +154,60 @@ def test_node_phases_and_projectors(tables):
     g.pf_power2[:] = rng.integers(-6, 3, size=G)
     prog = synth.single_program([synth.constant_graph(3), g], F=3)
     check_program(prog, tables, all_rows=True)
+
+
+def sign_record_program(n_sign: int, seed: int, P: int = 40):
+    """One graph whose terms hold n_sign node phases with phi odd or 2 mod 4
+    (sign records), a few half-pi factors with odd coefficients (LIN records
+    with their own delta) and phase pairs, after a constant graph 0."""
+    G, Thp, Tph = 2, 3, 3
+    rng = np.random.default_rng(seed)
+    g = synth.empty_graph(G, P, Tnp=n_sign, Thp=Thp, Tph=Tph)
+    g.np_phases[:] = rng.choice([1, 2, 3, 5, 6, 7], size=(G, n_sign))
+    g.np_params[:] = rng.random((G, n_sign, P)) < 0.2
+    g.np_counts[:] = n_sign
+    g.hp_coeffs[:] = rng.choice([1, 3, 5, 7], size=(G, Thp))
+    g.hp_params[:] = rng.random((G, Thp, P)) < 0.2
+    g.ph_alpha[:] = rng.integers(0, 8, size=(G, Tph))
+    g.ph_beta[:] = rng.integers(0, 8, size=(G, Tph))
+    g.ph_alpha_p[:] = rng.random((G, Tph, P)) < 0.2
+    g.ph_beta_p[:] = rng.random((G, Tph, P)) < 0.2
+    g.ph_counts[:] = Tph
+    g.pf_power2[:] = rng.integers(-4, 1, size=G)
+    return synth.single_program([synth.constant_graph(P - 1), g], F=P - 1)
+
+
+def chunk_shapes(words):
+    """(n_sign, n_lin, n_pair) of every record chunk of the program's last graph."""
+    w = [int(x) for x in words]
+    off = w[PH_DWORDS]
+    ng = w[off + CH["NGRAPHS"]]
+    gh = w[w[off + CH["GRAPH"]] + (ng - 1) * GH_DWORDS:][:GH_DWORDS]
+    W32, th, out = gh[1], gh[3], []
+    for _ in range(gh[0]):
+        h = w[th:th + TH_DWORDS]
+        rp = th + TH_DWORDS + h[TH["NPROJ"]] * (W32 + 1) + h[TH["NPPX"]] * (2 * W32 + 2)
+        for _ in range(h[TH["NCHUNK"]]):
+            nl, npr, npm = w[rp], w[rp + 1], w[rp + 3]
+            out.append((npm, nl, npr))
+            rp += 4 + (npm + 31) // 32 * 4 + npm * W32 + nl * (W32 + 1) + npr * (2 * W32 + 3)
+        th += TH_DWORDS + h[TH["SIZE"]]
+    return out
+
+
+@pytest.mark.parametrize("n_sign", [1, 31, 32, 33, 63, 64, 65, 97])
+def test_sign_record_groups(tables, n_sign):
+    # node phases pack into sign-record groups of up to 32 (one funnel-shift
+    # word each), chunks of at most 63 records mixing them with LIN and PAIR
+    # records; every group size from 1 to 32 and several groups per chunk
+    # evaluate to the oracle's exact values
+    prog = sign_record_program(n_sign, seed=n_sign)
+    shapes = chunk_shapes(pack_words(prog))
+    # (one-sided phase pairs may add sign records of their own)
+    assert sum(m for m, _, _ in shapes) >= 2 * n_sign
+    assert all(m + l + p <= 63 for m, l, p in shapes)
+    assert any(m > 0 and l + p > 0 for m, l, p in shapes) or n_sign >= 63
+    check_program(prog, tables, rows=24, seed=n_sign)
 
 
 def test_empty_graph_and_zero_terms(tables):

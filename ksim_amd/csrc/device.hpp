@@ -407,8 +407,8 @@ __device__ __forceinline__ int4 table_digit0(const TermIdx &ti, bool vanished, c
   }
 }
 
-// e * pi^A pibar^B for a table digit e (|e| < 2^30; pi powers < 2^24): four
-// signed 32 x 32 -> 64-bit products per coefficient, exact in int64.
+// e * pi^A pibar^B for a table digit e (|e| < 2^30; pi powers < 2^24), exact
+// in int64.
 // Always (the 128-bit sums, which serve the programs with many terms and
 // phase pairs): a term that may carry pi powers (TF_PI, the same for the
 // whole wave) takes the product, pi^0 pibar^0 = 1 included, since most waves
@@ -435,11 +435,17 @@ __device__ __forceinline__ void times_pi(int4 e, uint32_t flags, int A, int Bq,
       }
       pi_row = min(A, PI_MAX - 1) * PI_MAX + min(Bq, PI_MAX - 1);
     }
+    // pi^A pibar^B = P0 + P1 i sqrt2 = (P0, P1, 0, -P1) (pi = 1 + w - conj w,
+    // checked when the table is built), so e * pi^A pibar^B = P0 e + P1 (w -
+    // conj w) e: (w - conj w) e is a signed sum of e's coefficients (each below
+    // 2^30, so the sums fit 32 bits) and two 32 x 32 -> 64-bit products per
+    // coefficient remain
     const int4 p = lt->pi[pi_row];
-    x0 = (int64_t)e.x * p.x + (int64_t)e.y * p.w - (int64_t)e.z * p.z + (int64_t)e.w * p.y;
-    x1 = (int64_t)e.x * p.y + (int64_t)e.y * p.x + (int64_t)e.z * p.w + (int64_t)e.w * p.z;
-    x2 = (int64_t)e.x * p.z + (int64_t)e.y * p.y + (int64_t)e.z * p.x - (int64_t)e.w * p.w;
-    x3 = (int64_t)e.x * p.w - (int64_t)e.y * p.z - (int64_t)e.z * p.y + (int64_t)e.w * p.x;
+    const int32_t f0 = e.w - e.y, f1 = e.x - e.z, f2 = e.y + e.w, f3 = -(e.x + e.z);
+    x0 = (int64_t)e.x * p.x + (int64_t)f0 * p.y;
+    x1 = (int64_t)e.y * p.x + (int64_t)f1 * p.y;
+    x2 = (int64_t)e.z * p.x + (int64_t)f2 * p.y;
+    x3 = (int64_t)e.w * p.x + (int64_t)f3 * p.y;
   } else {
     x0 = e.x; x1 = e.y; x2 = e.z; x3 = e.w;
   }

@@ -201,6 +201,10 @@ const int32_t *pi_table() {
       for (int b = 0; b < PI_MAX; ++b) {
         Zw x = zw_mul(zw_pow(zw_pi(), a), zw_pow(zw_pibar(), b));
         if (zw_maxabs(x) >= ((i128)1 << 30)) throw std::logic_error("pi table entry too large");
+        // pi = 1 + i sqrt2 = 1 + w - conj w, so every power lies in Z[i sqrt2]:
+        // (P0, P1, 0, -P1) in the basis (1, w, i, conj w); the kernels use
+        // only P0 and P1
+        if (x.c != 0 || x.d != -x.b) throw std::logic_error("pi power outside Z[i sqrt2]");
         int32_t *e = &tab[(a * PI_MAX + b) * 4];
         e[0] = (int32_t)x.a; e[1] = (int32_t)x.b; e[2] = (int32_t)x.c; e[3] = (int32_t)x.d;
       }

@@ -68,6 +68,10 @@ def test_tables_are_the_closed_forms(tables):
             for _ in range(b):
                 x = exact.zmul(x, pib)
             assert tuple(int(v) for v in pit[a * PM + b]) == x
+    # every entry lies in Z[i sqrt2], (P0, P1, 0, -P1): the kernels multiply
+    # by P0 and P1 only
+    pit = np.asarray(pit).reshape(PM * PM, 4).astype(np.int64)
+    assert not pit[:, 2].any() and np.array_equal(pit[:, 3], -pit[:, 1])
 
 
 def test_golden_programs(tables, golden_files):

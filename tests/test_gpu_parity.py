@@ -125,6 +125,18 @@ def test_fed_uniform_streams_golden(golden_files):
             assert np.array_equal(got, d[f"sample_seed_{seed}"]), (fn, seed)
 
 
+def test_run_sampler_cpu_backend_is_the_reference_numpy_stream(golden_files):
+    # run.py:89-90: the "cpu" backend is sample_flat(flat, f, default_rng(seed));
+    # the golden files hold the reference's own output of exactly that call
+    from ksim_amd.sampler import run_sampler
+    for fn in golden_files:
+        d = np.load(fn)
+        prog = load_program(d)
+        for seed in (5, 6):
+            got = run_sampler(prog, d["f_params"], seed, backend="cpu")
+            assert np.array_equal(got, d[f"sample_seed_{seed}"]), (fn, seed)
+
+
 @pytest.mark.parametrize("seed", [0, 1])
 def test_fed_uniform_streams_shape(seed):
     prog = synth.shape_program(n_q=6, t=6, F=8, n_out=6, seed=20 + seed, n_components=2)

@@ -312,7 +312,8 @@ __device__ __forceinline__ void term_exps(const cword *__restrict__ th, const PW
     // over the group's sign masks give the S and N steps; the chunk base
     // holds the -2^30 |S-| - 2^9 |N-| of the parity-0 records counted there
     for (uint32_t g0 = 0; g0 < npm; g0 += PM_GROUP) {
-      const uint32_t n = min(npm - g0, (uint32_t)PM_GROUP);
+      const uint32_t rem = npm - g0;  // wave-uniform: kept on the scalar unit
+      const uint32_t n = rem < (uint32_t)PM_GROUP ? rem : (uint32_t)PM_GROUP;
       const uint32_t sp = rp[0], sm = rp[1], tp = rp[2], tm = rp[3];
       rp += PM_HDR;
       uint32_t pk = 0;
@@ -327,7 +328,11 @@ __device__ __forceinline__ void term_exps(const cword *__restrict__ th, const PW
       }
       const uint32_t cs = __popc((pk & sp) | (~pk & sm));
       const uint32_t cn = __popc((pk & tp) | (~pk & tm));
-      x += (cs << 30) + (cn << 9);
+      // x += 2^30 cs + 2^9 cn as two shift-adds (the compiler's own form
+      // is two shifts and an add)
+      uint32_t sn;
+      asm("v_lshl_add_u32 %0, %1, 21, %2" : "=v"(sn) : "v"(cs), "v"(cn));
+      asm("v_lshl_add_u32 %0, %1, 9, %0" : "+v"(x) : "v"(sn));
     }
     // LIN records, unrolled so each scalar load fetches several records
     if constexpr (LONG) {

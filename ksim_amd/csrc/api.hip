@@ -110,6 +110,7 @@ std::string err_text(uint32_t e) {
   if (e & ERR_PI_RANGE) s += " pi exponent beyond the table";
   if (e & ERR_SHIFT) s += " alignment shift beyond the 128-bit sum";
   if (e & ERR_WIDE) s += " a graph needing 128-bit sums in a 64-bit kernel";
+  if (e & ERR_FORMAT) s += " parity tables in a kernel that reads only masks";
   return s;
 }
 
@@ -155,6 +156,8 @@ template <int WL, bool NOISE = false, bool WIDE = true>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WIDE ? KSIM_WAVES_PER_EU : KSIM_NARROW_WAVES_PER_EU))) void k_sample(ShotArgs a, Tables tb) {
   __shared__ LdsTables lt;
   __shared__ uint32_t stage[4][NOISE ? 1 : stage_words(WL)];
+  // the waves' parity-table buffers (128-bit kernels: programs with tables)
+  __shared__ __attribute__((aligned(256))) uint4 tbl[WIDE ? 4 : 1][WIDE ? 128 : 1];
   prefetch_program(a.prog, a.prefetch_dwords);
   load_tables(&lt, tb);
   const int lane = threadIdx.x & 63;
@@ -164,8 +167,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WIDE ? K
   const int nrows = (int)min((int64_t)64, a.B - r0);
   const int64_t s = r0 + min(lane, nrows - 1);
   const FStage fs{NOISE ? nullptr : stage[wave], r0, nrows};
-  run_shot<WL, NOISE, false, WIDE>(a, (int)blockIdx.y, s, &lt, tb, Split{1, 0, 0, nullptr},
-                                   lane < nrows, nullptr, fs);
+  const uint32_t tbuf = WIDE ? lds_addr(tbl[wave]) : NO_TBUF;
+  run_shot<WL, NOISE, false, WIDE>(a, (int)blockIdx.y, s, &lt, tb,
+                                   Split{1, 0, 0, nullptr, tbuf}, lane < nrows, nullptr, fs);
 }
 
 // Plain path with a term split: L waves (2 or 4) share each graph's terms
@@ -176,7 +180,7 @@ template <int WL, bool NOISE = false, bool WIDE = true>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WIDE ? KSIM_WAVES_PER_EU : KSIM_NARROW_WAVES_PER_EU))) void k_sample_split(
     ShotArgs a, Tables tb, int L) {
   __shared__ LdsTables lt;
-  __shared__ int4 xch[XCH_INT4];
+  __shared__ __attribute__((aligned(256))) int4 xch[XCH_INT4];
   prefetch_program(a.prog, a.prefetch_dwords);
   load_tables(&lt, tb);
   const int lane = threadIdx.x & 63;
@@ -247,9 +251,17 @@ struct EvalArgs {
 template <int WL>
 __global__ __launch_bounds__(kBlock) void k_eval(EvalArgs a, Tables tb) {
   __shared__ LdsTables lt;
+  __shared__ __attribute__((aligned(256))) uint4 tbl[4][128];
   load_tables(&lt, tb);
-  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= a.B) return;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int64_t r0 = (int64_t)blockIdx.x * blockDim.x + 64 * wave;
+  if (r0 >= a.B) return;  // the whole wave is past the last row
+  // lanes past the last row evaluate a copy of it (the parity tables are
+  // copied into LDS by all 64 lanes of a wave) and write nothing
+  const int64_t s0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = s0 < a.B;
+  const int64_t s = valid ? s0 : a.B - 1;
+  const uint32_t tbuf = lds_addr(tbl[wave]);
   const uint8_t *row = a.params + s * (int64_t)a.P;
   uint32_t pv[WL];
 #pragma unroll
@@ -263,19 +275,22 @@ __global__ __launch_bounds__(kBlock) void k_eval(EvalArgs a, Tables tb) {
   const int e0 = (int)a.gh[GH_E0];
   if (a.mode == 0) {
     Acc4 acc;
-    eval_graph_any<WL>(as_const(a.prog), as_const(a.gh), pv, &lt, tb, acc, err);
-    int64_t *c = a.coeffs + s * 8;
-    const i128_t v[4] = {acc.a, acc.b, acc.c, acc.d};
-    for (int k = 0; k < 4; ++k) {
-      c[2 * k] = (int64_t)(uint64_t)(u128_t)v[k];
-      c[2 * k + 1] = (int64_t)(v[k] >> 64);
+    eval_graph_any<WL>(as_const(a.prog), as_const(a.gh), pv, &lt, tb, acc, err, tbuf);
+    if (valid) {
+      int64_t *c = a.coeffs + s * 8;
+      const i128_t v[4] = {acc.a, acc.b, acc.c, acc.d};
+      for (int k = 0; k < 4; ++k) {
+        c[2 * k] = (int64_t)(uint64_t)(u128_t)v[k];
+        c[2 * k + 1] = (int64_t)(v[k] >> 64);
+      }
+      a.exp2[s] = e0;
     }
-    a.exp2[s] = e0;
   } else {
     // the sampler's own probability path (table magnitudes, 64-bit sums)
     uint32_t xs = 0;
-    a.absout[s] = graph_prob_any<WL>(as_const(a.prog), as_const(a.gh), pv, &lt, tb, err,
-                                     Split{1, 0, 0, nullptr}, xs);
+    const double m = graph_prob_any<WL>(as_const(a.prog), as_const(a.gh), pv, &lt, tb, err,
+                                        Split{1, 0, 0, nullptr, tbuf}, xs);
+    if (valid) a.absout[s] = m;
   }
   if (err) atomicOr(a.err, err);
 }
@@ -324,7 +339,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WIDE ? K
     uint32_t *cursor, uint32_t *err, Tables tb) {
   __shared__ LdsTables lt;
   __shared__ uint32_t s_item;
-  __shared__ int4 xch[XCH_INT4];
+  __shared__ __attribute__((aligned(256))) int4 xch[XCH_INT4];
   __shared__ uint32_t stage[4][kStageBatch<WL, WIDE> ? stage_words(WL) : 1];
   load_tables(&lt, tb);
   uint32_t xs = 0;  // exchange state, the same in every wave of the block
@@ -416,7 +431,7 @@ __global__ __launch_bounds__(kBlock) void k_eval_lds(EvalArgs a, Tables tb, int 
   const LdsWords pw{pvl, (int)gh[GH_W32]};
   if (a.mode == 0) {
     Acc4 acc;
-    eval_graph(as_const(a.prog), gh, pw, &lt, tb, acc, err);
+    eval_graph(as_const(a.prog), gh, pw, &lt, tb, acc, err, NO_TBUF);
     int64_t *c = a.coeffs + s * 8;
     const i128_t v[4] = {acc.a, acc.b, acc.c, acc.d};
     for (int k = 0; k < 4; ++k) {

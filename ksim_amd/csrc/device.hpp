@@ -14,6 +14,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "layout.hpp"
 #include "noise.hpp"
 
@@ -28,6 +30,15 @@ typedef unsigned __int128 u128_t;
 // even after the kernel's own output stores.
 typedef __attribute__((address_space(4))) const uint32_t cword;
 __device__ __forceinline__ const cword *as_const(const uint32_t *p) { return (const cword *)p; }
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(4))) const u32x4 cword4;
+typedef __attribute__((address_space(3))) const uint32_t lword;
+typedef __attribute__((address_space(3))) u32x4 lword4;
+
+// LDS byte address of a pointer into shared memory
+__device__ __forceinline__ uint32_t lds_addr(const void *p) {
+  return (uint32_t)(uintptr_t)(__attribute__((address_space(3))) const void *)p;
+}
 
 struct Acc4 {
   i128_t a, b, c, d;
@@ -197,16 +208,95 @@ __device__ __forceinline__ uint32_t masked_xor(const uint32_t (&pv)[WL],
 // programs of at most 256 parameters).  LdsWords: the words in LDS, word w
 // of the lane at pv[64 w] (a wave's words interleaved by lane, so the 64
 // lanes of one read hit 64 consecutive dwords), any width at run time.
+//
+// Parity tables (GF_TABLE graphs, at most PM_TAB_W32_MAX words): a sign
+// group's 8 W32 nibble tables are copied into the wave's 2 KiB LDS buffer
+// (one 16-byte load and store per lane and KiB) and the group's parity word
+// is the XOR of one LDS read per nibble, T_j[nibble j of p], instead of
+// W32 + 2 VALU ops per record.  Each read address is the buffer base | 4 v
+// for nibble value v (one v_perm), table j at the read's immediate offset
+// 64 j.  (Building the 8 W32 addresses once per graph instead spilled them
+// in the 128-bit kernels.)
+constexpr uint32_t NO_TBUF = 0xffffffffu;  // no table buffer: the groups hold masks
+struct TabBase {
+  uint32_t base;  // the wave's table buffer (LDS byte address), or NO_TBUF
+};
+struct NoTab {
+  static constexpr uint32_t base = NO_TBUF;
+};
+
+// wave-local order between the lanes' LDS stores and the reads of other
+// lanes' data (a wave's LDS operations complete in issue order; the fences
+// keep the compiler from moving memory operations across)
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <int W32, int WL>
 struct RegWords {
   static_assert(W32 <= WL, "graph wider than the lane's parameter words");
+  static constexpr bool kTab = W32 <= PM_TAB_W32_MAX;
+  using Tab = typename std::conditional<kTab, TabBase, NoTab>::type;
   const uint32_t (&pv)[WL];
   __device__ __forceinline__ int width() const { return W32; }
   __device__ __forceinline__ uint32_t mxor(const cword *__restrict__ m) const {
     return masked_xor<W32, WL>(pv, m);
   }
+  __device__ __forceinline__ void tab_init(Tab &t, uint32_t base) const {
+    if constexpr (kTab) t.base = base;
+  }
+  // parity word of the sign group whose tables start at `tabs` (16-byte
+  // aligned, pm_tab_dwords(W32) dwords)
+  __device__ __forceinline__ uint32_t tab_parity(const cword *__restrict__ tabs,
+                                                 const Tab &t) const {
+    if constexpr (kTab) {
+      constexpr int N4 = (int)pm_tab_dwords(W32) / 4;
+      const int lane = threadIdx.x & 63;
+      const cword4 *src = (const cword4 *)tabs;
+      lword4 *dst = (lword4 *)(uintptr_t)t.base;
+#pragma unroll
+      for (int i = 0; i < (N4 + 63) / 64; ++i) {
+        const int e = i * 64 + lane;
+        if (N4 % 64 == 0 || e < N4) dst[e] = src[e];
+      }
+      wave_lds_sync();
+      // read addresses: v_perm takes byte k of (word << 2) & 0x3c3c3c3c (the
+      // even nibbles times 4) or of (word >> 2) & 0x3c3c3c3c (the odd ones)
+      // into byte 0, under bytes 1 and 2 of the base (256-byte aligned,
+      // below 2^24)
+      // (the opaque copy of the base keeps the compiler from building all
+      // 8 W32 addresses once per graph, which spilled them)
+      uint32_t base = t.base;
+      asm volatile("" : "+s"(base));
+      uint32_t y = 0;
+#pragma unroll
+      for (int w = 0; w < W32; ++w) {
+        uint32_t x = pv[w];
+        asm volatile("" : "+v"(x));
+        const uint32_t pe = (x << 2) & 0x3c3c3c3cu, po = (x >> 2) & 0x3c3c3c3cu;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const uint32_t sel = 0x0c060500u | (uint32_t)k;
+          const int j = 8 * w + 2 * k;
+          const uint32_t ae = __builtin_amdgcn_perm(base, pe, sel);
+          const uint32_t ao = __builtin_amdgcn_perm(base, po, sel);
+          const uint32_t u = ((const lword *)(uintptr_t)ae)[16 * j];
+          const uint32_t v = ((const lword *)(uintptr_t)ao)[16 * (j + 1)];
+          y = __builtin_amdgcn_bitop3_b32(u, v, y, 0x96);
+        }
+      }
+      wave_lds_sync();  // the next group's copy after these reads
+      return y;
+    } else {
+      return 0;
+    }
+  }
 };
 struct LdsWords {
+  static constexpr bool kTab = false;
+  using Tab = NoTab;
   const uint32_t *pv;
   int W32;
   __device__ __forceinline__ int width() const { return W32; }
@@ -214,6 +304,10 @@ struct LdsWords {
     uint32_t t = pv[0] & m[0];
     for (int w = 1; w < W32; ++w) t = __builtin_amdgcn_bitop3_b32(m[w], t, pv[64 * w], 0x6c);
     return t;
+  }
+  __device__ __forceinline__ void tab_init(Tab &, uint32_t) const {}
+  __device__ __forceinline__ uint32_t tab_parity(const cword *__restrict__, const Tab &) const {
+    return 0;
   }
 };
 
@@ -265,7 +359,7 @@ __device__ __forceinline__ void stream_done(uint32_t v) { asm volatile("" ::"v"(
 template <class PW, bool LONG = false>
 __device__ __forceinline__ void term_exps(const cword *__restrict__ th, const PW &pw, int &S,
                                           int &V, int &N, int &A, int &Bq, uint32_t &dead,
-                                          Stream &ps) {
+                                          Stream &ps, const typename PW::Tab &tab) {
   const int W32 = pw.width();
   const uint32_t nproj = th[TH_NPROJ], nppx = th[TH_NPPX], nchunk = th[TH_NCHUNK];
   S = (int)th[TH_S]; V = (int)th[TH_V]; N = (int)th[TH_N];
@@ -298,14 +392,18 @@ __device__ __forceinline__ void term_exps(const cword *__restrict__ th, const PW
   S += (int)((ex & 1u) << 2);
   // LIN / PAIR chunks: packed exponent adds
   for (uint32_t ch = 0; ch < nchunk; ++ch) {
-    const uint32_t nl = rp[0], npr = rp[1], npm = rp[3];
+    const uint32_t nl = rp[0], npr = rp[1], nw = rp[3];
+    const uint32_t npm = nw & PM_COUNT_MASK, pad = nw >> PM_PAD_SHIFT;
     uint32_t x = rp[2];
     uint32_t touched = 0;
-    if constexpr (LONG)
-      touched = stream_touch(ps, rp, rp + CHUNK_HDR + (npm + PM_GROUP - 1) / PM_GROUP * PM_HDR +
-                                         npm * W32 + nl * (W32 + 1) + npr * (2 * W32 + 3) +
-                                         STREAM_AHEAD);
-    rp += CHUNK_HDR;
+    if constexpr (LONG) {
+      const uint32_t ngr = (npm + PM_GROUP - 1) / PM_GROUP;
+      const uint32_t pmw = tab.base != NO_TBUF ? ngr * (uint32_t)pm_tab_group_dwords(W32)
+                                    : ngr * PM_HDR + npm * W32;
+      touched = stream_touch(ps, rp, rp + CHUNK_HDR + pad + pmw + nl * (W32 + 1) +
+                                         npr * (2 * W32 + 3) + STREAM_AHEAD);
+    }
+    rp += CHUNK_HDR + pad;
     // sign records (S += 2s, N += t, s, t in {-1, 0, 1}): the parities of a
     // group go into one word, record j of n at bit 32 - n + j (each funnel
     // shift takes bit 0 of the popcount in at the top), then two popcounts
@@ -317,7 +415,10 @@ __device__ __forceinline__ void term_exps(const cword *__restrict__ th, const PW
       const uint32_t sp = rp[0], sm = rp[1], tp = rp[2], tm = rp[3];
       rp += PM_HDR;
       uint32_t pk = 0;
-      if constexpr (LONG) {
+      if (tab.base != NO_TBUF) {
+        pk = pw.tab_parity(rp, tab);
+        rp += pm_tab_dwords(W32);
+      } else if constexpr (LONG) {
 #pragma unroll 8
         for (uint32_t k = 0; k < n; ++k, rp += W32)
           pk = __builtin_amdgcn_alignbit(__popc(pw.mxor(rp)), pk, 1);
@@ -578,18 +679,28 @@ __device__ __forceinline__ void graph_sum_clear(GraphSum &g, uint32_t gflags) {
 // Add terms [0, nt) starting at program dword `th` into g, by the graph's
 // flags: a single table magnitude (GF_MONO), 64-bit sums (GF_NARROW) or
 // 128-bit sums.
+// GF_TABLE graphs read their parity tables through the wave's LDS buffer at
+// `tbuf` (an LDS byte address, 256-byte aligned); the caller has checked
+// that the kernel reads tables (tab_ok).
 template <class PW, bool WIDE = true>
 __device__ __forceinline__ void sum_terms(const cword *__restrict__ th, uint32_t nt,
                                           uint32_t gflags, int e0, const PW &pw,
                                           const LdsTables *lt, const Tables &tb, GraphSum &g,
-                                          uint32_t &err, const cword *__restrict__ apx = nullptr) {
+                                          uint32_t &err, const cword *__restrict__ apx = nullptr,
+                                          uint32_t tbuf = NO_TBUF) {
   Stream ps{(gflags & GF_STREAM) ? th : nullptr};
+  typename PW::Tab tab;
+  if constexpr (PW::kTab) {
+    tab.base = NO_TBUF;
+    if constexpr (WIDE)
+      if (gflags & GF_TABLE) pw.tab_init(tab, tbuf);
+  }
   for (uint32_t t = 0; t < nt; ++t) {
     const uint32_t flags = th[TH_FLAGS];
     const int E = (int)th[TH_E];
     int S, V, N, A, Bq;
     uint32_t dead;
-    term_exps<PW, WIDE>(th, pw, S, V, N, A, Bq, dead, ps);
+    term_exps<PW, WIDE>(th, pw, S, V, N, A, Bq, dead, ps, tab);
     const TermIdx ti = term_index(S, V, N);
     const bool vanished = dead & 1u;
     const int sh = ti.q + E;
@@ -673,8 +784,22 @@ struct Split {
   int part;       // this wave's index in its group
   int first;      // the group's first wave in the block
   int4 *xch;      // the block's exchange area (nullptr when L == 1)
+  uint32_t tbuf = NO_TBUF;  // kernels without an exchange area: the wave's
+                            // parity-table buffer (LDS byte address, 256-byte
+                            // aligned), or NO_TBUF
 };
 constexpr int XCH_INT4 = 4 * 4 * 64;
+
+// The wave's 2 KiB parity-table buffer for the graph about to be summed
+// (NO_TBUF: the kernel has none).  With an exchange area it is the half of the
+// wave's area that the next exchange writes (xs bit 0): the other half may
+// still be read by the group's other waves after the last 64-bit exchange,
+// and this one was last read before that exchange's barrier.
+__device__ __forceinline__ uint32_t table_buffer(const Split &sp, uint32_t xs) {
+  if (sp.xch == nullptr) return sp.tbuf;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  return lds_addr(sp.xch + wave * 256 + (int)(xs & 1u) * 128);
+}
 
 // Sum the partial sums of the group's waves (exact integer sums, so every
 // wave ends with the same value).  Uniform across the block: every wave of
@@ -770,7 +895,18 @@ __device__ __forceinline__ double graph_prob(const cword *__restrict__ P,
       return __dsqrt_rn(__dadd_rn(__dmul_rn(g.fr, g.fr), __dmul_rn(g.fi, g.fi)));
     }
   }
-  sum_terms<PW, WIDE>(P + off, t1 - t0, gflags, e0, pw, lt, tb, g, err);
+  uint32_t tbuf = NO_TBUF;
+  if (gflags & GF_TABLE) {
+    // tables are read by the 128-bit register-word kernels only (the
+    // validator keeps them out of every other program); elsewhere the
+    // graph is flagged and its terms skipped, barriers still reached
+    tbuf = (WIDE && PW::kTab) ? table_buffer(sp, xs) : NO_TBUF;
+    if (tbuf == NO_TBUF) {
+      err |= ERR_FORMAT;
+      t1 = t0;
+    }
+  }
+  sum_terms<PW, WIDE>(P + off, t1 - t0, gflags, e0, pw, lt, tb, g, err, nullptr, tbuf);
   if (gflags & GF_MONO) return g.pm;
   const bool narrow = !WIDE || (gflags & GF_NARROW);
   if constexpr (WIDE) {
@@ -811,10 +947,17 @@ template <class PW>
 __device__ __forceinline__ void eval_graph(const cword *__restrict__ P,
                                            const cword *__restrict__ gh, const PW &pw,
                                            const LdsTables *lt, const Tables &tb, Acc4 &acc,
-                                           uint32_t &err) {
+                                           uint32_t &err, uint32_t tbuf) {
   GraphSum g;
   graph_sum_clear<true>(g, 0u);
-  sum_terms<PW>(P + gh[GH_TERM_OFF], gh[GH_NTERMS], 0u, (int)gh[GH_E0], pw, lt, tb, g, err);
+  const uint32_t table = gh[GH_FLAGS] & GF_TABLE;
+  uint32_t nt = gh[GH_NTERMS];
+  if (table && !(PW::kTab && tbuf != NO_TBUF)) {
+    err |= ERR_FORMAT;
+    nt = 0;
+  }
+  sum_terms<PW>(P + gh[GH_TERM_OFF], nt, table, (int)gh[GH_E0], pw, lt, tb, g, err, nullptr,
+                tbuf);
   acc = g.acc;
 }
 
@@ -822,16 +965,15 @@ template <int WL>
 __device__ __forceinline__ void eval_graph_any(const cword *__restrict__ P,
                                                const cword *__restrict__ gh,
                                                const uint32_t (&pv)[WL], const LdsTables *lt,
-                                               const Tables &tb, Acc4 &acc, uint32_t &err) {
+                                               const Tables &tb, Acc4 &acc, uint32_t &err,
+                                               uint32_t tbuf) {
   switch (gh[GH_W32]) {
-    case 1: eval_graph(P, gh, RegWords<1, WL>{pv}, lt, tb, acc, err); break;
-    case 2: if constexpr (WL >= 2) eval_graph(P, gh, RegWords<2, WL>{pv}, lt, tb, acc, err); break;
-    case 3: if constexpr (WL >= 3) eval_graph(P, gh, RegWords<3, WL>{pv}, lt, tb, acc, err); break;
-    case 4: if constexpr (WL >= 4) eval_graph(P, gh, RegWords<4, WL>{pv}, lt, tb, acc, err); break;
-    case 5: if constexpr (WL >= 5) eval_graph(P, gh, RegWords<5, WL>{pv}, lt, tb, acc, err); break;
-    case 6: if constexpr (WL >= 6) eval_graph(P, gh, RegWords<6, WL>{pv}, lt, tb, acc, err); break;
-    case 7: if constexpr (WL >= 7) eval_graph(P, gh, RegWords<7, WL>{pv}, lt, tb, acc, err); break;
-    case 8: if constexpr (WL >= 8) eval_graph(P, gh, RegWords<8, WL>{pv}, lt, tb, acc, err); break;
+#define KSIM_EG(w)                                                                    \
+  case w:                                                                             \
+    if constexpr (WL >= w) eval_graph(P, gh, RegWords<w, WL>{pv}, lt, tb, acc, err, tbuf); \
+    break;
+    KSIM_EG(1) KSIM_EG(2) KSIM_EG(3) KSIM_EG(4) KSIM_EG(5) KSIM_EG(6) KSIM_EG(7) KSIM_EG(8)
+#undef KSIM_EG
     default: break;
   }
 }

@@ -83,12 +83,24 @@ constexpr uint32_t TF_PI = 2u;        // A or B may be non-zero
 // and adds 2^30 popcount((p & S+) | (~p & S-)) + 2^9 popcount((p & N+) |
 // (~p & N-)); the base holds -2^30 |S-| - 2^9 |N-|.  Blobs of version 3 and 4
 // have n_pm = 0 in every chunk.
-constexpr int CHUNK_HDR = 4;  // n_lin, n_pair, base, n_pm
+constexpr int CHUNK_HDR = 4;  // n_lin, n_pair, base, n_pm | pad << PM_PAD_SHIFT
 constexpr int PM_GROUP = 32;
 constexpr int PM_HDR = 4;     // S+, S-, N+, N- masks over the group's records
 inline uint64_t pm_dwords(uint64_t n_pm, uint64_t W32) {
   return (n_pm + PM_GROUP - 1) / PM_GROUP * PM_HDR + n_pm * W32;
 }
+// Parity tables (graphs flagged GF_TABLE, at most PM_TAB_W32_MAX mask words):
+// a sign group is its PM_HDR header and then, for each of the 8 W32 nibbles
+// j of the parameter words (bits 4j..4j+3), 16 dwords T_j[v], v < 16: bit
+// 32 - n + r of T_j[v] is the parity of v & (nibble j of record r's mask).
+// The XOR over j of T_j[nibble j of p] is the group's parity word.  The n_pm
+// word of such a chunk also holds, at bit PM_PAD_SHIFT, the count (< 4) of
+// zero dwords between the chunk header and its first group, so that every
+// group's tables start on a 16-byte boundary of the program words.
+constexpr int PM_TAB_W32_MAX = 4;
+constexpr uint32_t PM_COUNT_MASK = 0xffu, PM_PAD_SHIFT = 8;
+constexpr uint32_t pm_tab_dwords(uint32_t W32) { return 128 * W32; }
+constexpr uint32_t pm_tab_group_dwords(uint32_t W32) { return PM_HDR + pm_tab_dwords(W32); }
 
 // graph header (dwords).  The terms of a graph are stored as GH_PARTS parts
 // of about equal record work (the packer deals them out heaviest first), so
@@ -117,6 +129,9 @@ constexpr uint32_t GF_APPROX = 4u;  // approximate floatfactors: each term's exa
 constexpr uint32_t GF_STREAM = 8u;  // the 128-bit kernels read the graph's records
                                     // ahead into L2; the program then ends in
                                     // STREAM_PAD spare dwords
+constexpr uint32_t GF_TABLE = 16u;  // sign groups carry parity tables instead of
+                                    // masks (only in programs that run the
+                                    // 128-bit register-word kernels)
 
 // Record read-ahead of GF_STREAM graphs (the packer sets the flag on every
 // graph of several terms): each touch reaches STREAM_AHEAD dwords past the
@@ -146,6 +161,7 @@ constexpr uint32_t ERR_TAB_RANGE = 1u;   // |n'| beyond the table
 constexpr uint32_t ERR_PI_RANGE = 2u;    // pi exponents beyond PI_MAX
 constexpr uint32_t ERR_SHIFT = 4u;       // alignment shift outside [0, max]
 constexpr uint32_t ERR_WIDE = 8u;        // a 128-bit graph in a kernel built without that path
+constexpr uint32_t ERR_FORMAT = 16u;     // parity tables in a kernel that reads only masks
 
 // Largest alignment shift the 128-bit sum accepts; the packer proves
 // bits(term) + shift + log2(G) <= 126 before upload.

@@ -585,6 +585,106 @@ void pack_graph(const ksim_graph_view &g, std::vector<uint32_t> &out, uint32_t *
   st.records += nrec;
 }
 
+// Rewrite a packed graph (offsets relative to its first word, sign groups
+// as masks) so that its sign groups carry parity tables (GF_TABLE; layout.hpp
+// gives the form).  The words must then start at a multiple of 4 dwords in
+// the program, which makes every group's tables 16-byte aligned.  A graph
+// keeps its masks when it is wider than PM_TAB_W32_MAX words, has at most
+// one term or approximate factors, or when its sign groups hold fewer than
+// PM_TAB_MIN_FILL records on average: a table group costs about the same
+// whatever its record count, a mask group W32 + 2 ops per record.
+constexpr int PM_TAB_MIN_FILL = 8;
+
+bool tabulate_graph(std::vector<uint32_t> &words, uint32_t *ghdr) {
+  const uint32_t W32 = ghdr[GH_W32], nt = ghdr[GH_NTERMS];
+  if (W32 > (uint32_t)PM_TAB_W32_MAX || nt < 2 || (ghdr[GH_FLAGS] & (GF_APPROX | GF_MONO)))
+    return false;
+  // walk the terms: (term offset, chunk offsets) and the sign-record fill
+  size_t groups = 0, recs = 0;
+  {
+    size_t to = ghdr[GH_TERM_OFF];
+    for (uint32_t t = 0; t < nt; ++t) {
+      const uint32_t *th = &words[to];
+      size_t rp = to + TH_DWORDS + (size_t)th[TH_NPROJ] * (W32 + 1) +
+                  (size_t)th[TH_NPPX] * (2 * W32 + 2);
+      for (uint32_t c = 0; c < th[TH_NCHUNK]; ++c) {
+        const uint32_t nl = words[rp], np = words[rp + 1], nm = words[rp + 3];
+        groups += (nm + PM_GROUP - 1) / PM_GROUP;
+        recs += nm;
+        rp += CHUNK_HDR + pm_dwords(nm, W32) + (size_t)nl * (W32 + 1) + (size_t)np * (2 * W32 + 3);
+      }
+      to += TH_DWORDS + th[TH_SIZE];
+    }
+  }
+  if (groups == 0 || recs < (size_t)PM_TAB_MIN_FILL * groups) return false;
+  std::vector<uint32_t> out;
+  out.reserve(words.size() + groups * pm_tab_dwords(W32));
+  out.insert(out.end(), words.begin(), words.begin() + ghdr[GH_TERM_OFF]);
+  std::vector<std::pair<uint32_t, uint32_t>> moved;  // old -> new term offsets
+  size_t to = ghdr[GH_TERM_OFF];
+  for (uint32_t t = 0; t < nt; ++t) {
+    const size_t nto = out.size();
+    moved.emplace_back((uint32_t)to, (uint32_t)nto);
+    const uint32_t *th = &words[to];
+    const size_t lead = TH_DWORDS + (size_t)th[TH_NPROJ] * (W32 + 1) +
+                        (size_t)th[TH_NPPX] * (2 * W32 + 2);
+    out.insert(out.end(), words.begin() + to, words.begin() + to + lead);
+    size_t rp = to + lead;
+    for (uint32_t c = 0; c < th[TH_NCHUNK]; ++c) {
+      const uint32_t nl = words[rp], np = words[rp + 1], base = words[rp + 2], nm = words[rp + 3];
+      // zero dwords after the header so that the first group's tables start
+      // at a multiple of 4 dwords
+      const uint32_t pad = nm ? (uint32_t)((4 - (out.size() + CHUNK_HDR + PM_HDR) % 4) % 4) : 0u;
+      out.push_back(nl); out.push_back(np); out.push_back(base);
+      out.push_back(nm | (pad << PM_PAD_SHIFT));
+      out.insert(out.end(), pad, 0u);
+      size_t q = rp + CHUNK_HDR;
+      for (uint32_t g0 = 0; g0 < nm; g0 += PM_GROUP) {
+        const uint32_t n = std::min<uint32_t>(nm - g0, PM_GROUP);
+        out.insert(out.end(), words.begin() + q, words.begin() + q + PM_HDR);
+        const uint32_t *m = &words[q + PM_HDR];
+        for (uint32_t j = 0; j < 8 * W32; ++j) {
+          uint32_t col[4] = {0, 0, 0, 0};  // parity-table entries of v = 1, 2, 4, 8
+          for (uint32_t r = 0; r < n; ++r) {
+            const uint32_t nib = (m[r * W32 + j / 8] >> (4 * (j % 8))) & 15u;
+            const uint32_t bit = 1u << (PM_GROUP - n + r);
+            for (int b = 0; b < 4; ++b)
+              if (nib >> b & 1u) col[b] |= bit;
+          }
+          for (uint32_t v = 0; v < 16; ++v) {
+            uint32_t e = 0;
+            for (int b = 0; b < 4; ++b)
+              if (v >> b & 1u) e ^= col[b];
+            out.push_back(e);
+          }
+        }
+        q += PM_HDR + (size_t)n * W32;
+      }
+      const size_t rest = (size_t)nl * (W32 + 1) + (size_t)np * (2 * W32 + 3);
+      out.insert(out.end(), words.begin() + q, words.begin() + q + rest);
+      rp = q + rest;
+    }
+    to += TH_DWORDS + th[TH_SIZE];
+    out[nto + TH_SIZE] = (uint32_t)(out.size() - nto - TH_DWORDS);
+  }
+  moved.emplace_back((uint32_t)to, (uint32_t)out.size());  // the end (empty trailing parts)
+  for (int p = 0; p < GH_PARTS - 1; ++p) {
+    uint32_t &o = ghdr[GH_PART_O1 + p];
+    bool found = false;
+    for (const auto &mv : moved)
+      if (mv.first == o) {
+        o = mv.second;
+        found = true;
+        break;
+      }
+    if (!found) throw std::logic_error("term part offset is not a term boundary");
+  }
+  out.insert(out.end(), words.begin() + to, words.end());  // nothing follows the terms
+  words.swap(out);
+  ghdr[GH_FLAGS] |= GF_TABLE;
+  return true;
+}
+
 }  // namespace ksim
 
 namespace ksim {
@@ -652,6 +752,35 @@ std::vector<uint32_t> pack_program(const ksim_graph_view *graphs, const int32_t 
     for (auto &j : jobs)
       if (j.err) std::rethrow_exception(j.err);
   }
+  // parity tables replace the sign masks in the programs that run the
+  // 128-bit register-word kernels: some graph needs 128-bit sums, no
+  // approximate factors, at most REG_WORDS_MAX lane words
+  {
+    bool all_narrow = true, approx = false;
+    int wl = 1;
+    for (int c = 0; c < n_comp; ++c)
+      wl = std::max(wl, std::max(1, (F_per_comp[c] + graphs_per_comp[c] - 1 + 31) / 32));
+    for (const Job &j : jobs) {
+      if (!(j.hdr[GH_FLAGS] & (GF_MONO | GF_NARROW))) all_narrow = false;
+      if (j.hdr[GH_FLAGS] & GF_APPROX) approx = true;
+    }
+    if (!all_narrow && !approx && wl <= REG_WORDS_MAX) {
+      std::atomic<size_t> next(0);
+      auto work = [&] {
+        for (;;) {
+          const size_t i = next.fetch_add(1);
+          if (i >= total_graphs) break;
+          tabulate_graph(jobs[i].words, jobs[i].hdr);
+        }
+      };
+      const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+      const unsigned nt = (unsigned)std::min<size_t>(hw, total_graphs);
+      std::vector<std::thread> pool;
+      for (unsigned t = 1; t < nt; ++t) pool.emplace_back(work);
+      work();
+      for (auto &th : pool) th.join();
+    }
+  }
   // lay out: program header, then per component its header, f_selection,
   // output columns, graph headers and the graphs' term records
   std::vector<uint32_t> w(PH_DWORDS + (size_t)n_comp, 0u);
@@ -700,6 +829,7 @@ std::vector<uint32_t> pack_program(const ksim_graph_view *graphs, const int32_t 
       // so that every graph's last parameter, the trying bit of the
       // sampler, lies in its last mask word
       if (j.hdr[GH_NTERMS] == 0) j.hdr[GH_W32] = (uint32_t)std::max(1, (F + i + 31) / 32);
+      if (j.hdr[GH_FLAGS] & GF_TABLE) w.resize((w.size() + 3) / 4 * 4, 0u);  // 16-byte tables
       j.hdr[GH_TERM_OFF] += (uint32_t)w.size();  // relative -> absolute
       for (int p = 0; p < GH_PARTS - 1; ++p) j.hdr[GH_PART_O1 + p] += (uint32_t)w.size();
       if (j.hdr[GH_FLAGS] & GF_APPROX) j.hdr[GH_APX_OFF] += (uint32_t)w.size();

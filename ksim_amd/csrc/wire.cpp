@@ -64,6 +64,7 @@ void validate_words(const uint32_t *w, size_t n) {
   if (wmax < 1 || wmax > (uint32_t)LANE_WORDS_MAX) bad("lane words outside 1..LANE_WORDS_MAX");
   need((uint64_t)PH_DWORDS + nc, "component offsets");
   uint64_t outs = 0;
+  bool all_narrow = true, any_approx = false, any_table = false;
   for (uint32_t c = 0; c < nc; ++c) {
     const uint64_t co = w[PH_DWORDS + c];
     need(co + CH_DWORDS, "component header");
@@ -97,8 +98,17 @@ void validate_words(const uint32_t *w, size_t n) {
       if (W32 < 1 || W32 > cw) bad("graph mask words");
       if (W32 != std::max<uint32_t>(1u, (F + g + 31) / 32)) bad("graph width is not ceil((F + i) / 32)");
       const uint32_t nt = gh[GH_NTERMS];
-      if (gh[GH_FLAGS] & ~(GF_MONO | GF_NARROW | GF_APPROX | GF_STREAM)) bad("graph flags");
+      if (gh[GH_FLAGS] & ~(GF_MONO | GF_NARROW | GF_APPROX | GF_STREAM | GF_TABLE))
+        bad("graph flags");
       if ((gh[GH_FLAGS] & GF_MONO) && nt != 1) bad("single-term flag on a graph of several terms");
+      if (!(gh[GH_FLAGS] & (GF_MONO | GF_NARROW))) all_narrow = false;
+      if (gh[GH_FLAGS] & GF_APPROX) any_approx = true;
+      const bool table = gh[GH_FLAGS] & GF_TABLE;
+      if (table) {
+        any_table = true;
+        if (W32 > (uint32_t)PM_TAB_W32_MAX) bad("parity tables on a graph wider than their limit");
+        if (gh[GH_FLAGS] & (GF_MONO | GF_APPROX)) bad("parity tables on a single-term or approximate graph");
+      }
       if (gh[GH_FLAGS] & GF_APPROX) {
         if (gh[GH_FLAGS] & (GF_MONO | GF_NARROW)) bad("approximate graph with an exact-sum flag");
         if (gh[GH_PART_T1] != nt) bad("approximate graph split into term parts");
@@ -120,9 +130,16 @@ void validate_words(const uint32_t *w, size_t n) {
         uint64_t rp = to + TH_DWORDS + sz;
         for (uint32_t k = 0; k < th[TH_NCHUNK]; ++k) {
           need(rp + CHUNK_HDR, "record chunk");
-          const uint64_t nl = w[rp], np = w[rp + 1], nm = w[rp + 3];
+          const uint32_t nw = w[rp + 3];
+          const uint64_t nl = w[rp], np = w[rp + 1], nm = nw & PM_COUNT_MASK;
+          const uint64_t pad = nw >> PM_PAD_SHIFT;
           if (nl + np + nm > (uint64_t)CHUNK_MAX) bad("record chunk longer than its limit");
-          const uint64_t cs = CHUNK_HDR + pm_dwords(nm, W32) + nl * (W32 + 1) + np * (2 * W32 + 3);
+          if (pad && !(table && nm)) bad("padding in a chunk without parity tables");
+          if (table && nm && (pad > 3 || (rp + CHUNK_HDR + pad + PM_HDR) % 4 != 0))
+            bad("parity tables not on a 16-byte boundary");
+          const uint64_t pm = table ? (nm + PM_GROUP - 1) / PM_GROUP * pm_tab_group_dwords(W32)
+                                    : pm_dwords(nm, W32);
+          const uint64_t cs = CHUNK_HDR + pad + pm + nl * (W32 + 1) + np * (2 * W32 + 3);
           sz += cs;
           rp += cs;
         }
@@ -135,6 +152,9 @@ void validate_words(const uint32_t *w, size_t n) {
     }
   }
   if (outs != tot) bad("total outputs");
+  // parity tables are read by the 128-bit register-word kernels only
+  if (any_table && (all_narrow || any_approx || wmax > (uint32_t)REG_WORDS_MAX))
+    bad("parity tables in a program that runs kernels reading masks only");
 }
 
 std::vector<uint32_t> words_from_blob(const uint8_t *blob, size_t n, double *stats8) {
@@ -142,7 +162,8 @@ std::vector<uint32_t> words_from_blob(const uint8_t *blob, size_t n, double *sta
   if (n < WIRE_HEADER) bad("shorter than its header");
   if (std::memcmp(blob, kMagic, 8) != 0) bad("bad magic");
   const uint32_t version = rd<uint32_t>(blob + 8);
-  if (version != WIRE_VERSION && version != 4 && version != 3) bad("unsupported version");
+  if (version != WIRE_VERSION && version != 5 && version != 4 && version != 3)
+    bad("unsupported version");
   if (rd<uint32_t>(blob + 12) != WIRE_HEADER) bad("header size");
   const uint64_t nw = rd<uint64_t>(blob + 16);
   if (nw < PH_DWORDS || nw > (n - WIRE_HEADER) / 4 || WIRE_HEADER + nw * 4 != n)

@@ -8,15 +8,16 @@
 
 namespace ksim {
 
-// Blob layout (little-endian), version 5 (16-dword graph headers with
+// Blob layout (little-endian), version 6 (16-dword graph headers with
 // flags, term parts and the offset of a graph's approximate factors, graph
 // i of a component ceil((F + i) / 32) mask words wide, up to LANE_WORDS_MAX
-// words, record chunks that open with sign-record groups; version 4 had no
+// words, record chunks that open with sign-record groups, held as parity
+// tables in GF_TABLE graphs; version 5 had no parity tables, version 4 no
 // sign records and version 3 neither approximate factors nor components
-// wider than 8 words, and both are read as well, every chunk of theirs
-// holding zero sign records; version 2 let a graph without terms keep
-// the width of its own parameter count, version 1 had 8-dword graph
-// headers):
+// wider than 8 words, and all three are read as well, every chunk of
+// versions 3 and 4 holding zero sign records; version 2 let a graph without
+// terms keep the width of its own parameter count, version 1 had 8-dword
+// graph headers):
 //   [0, 8)     magic "KSIMPRG\0"
 //   [8, 12)    u32 version
 //   [12, 16)   u32 header bytes (128)
@@ -26,7 +27,7 @@ namespace ksim {
 //   [96, 116)  u32 x 5: n_comp, num_f, num_outputs, total outputs, lane words
 //   [116, 128) zero
 //   [128, ...) u32 program words
-constexpr uint32_t WIRE_VERSION = 5;
+constexpr uint32_t WIRE_VERSION = 6;
 constexpr size_t WIRE_HEADER = 128;
 
 uint64_t fnv1a64(const void *data, size_t n);

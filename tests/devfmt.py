@@ -16,11 +16,32 @@ TH = dict(NPROJ=0, NPPX=1, NCHUNK=2, FLAGS=3, S=4, V=5, N=6, A=7, B=8, E=9, R0=1
 TH_DWORDS = 16
 CHUNK_HDR = 4
 PM_GROUP, PM_HDR = 32, 4
+PM_COUNT_MASK, PM_PAD_SHIFT = 0xFF, 8
 GH_DWORDS = 16
 CH = dict(NGRAPHS=0, F=1, NOUT=2, WMAX=3, COLBASE=4, FSEL=5, DEST=6, GRAPH=7)
 PH_DWORDS = 8
 TF_RESIDUAL, TF_PI = 1, 2
 GF_APPROX = 4
+GF_TABLE = 16
+GH_FLAGS = 5
+
+
+def pm_group_dwords(n, W32, table):
+    """Dwords of a sign group of n records (mask or parity-table form)."""
+    return PM_HDR + (128 * W32 if table else n * W32)
+
+
+def chunk_walk(w, rp, W32, table):
+    """(n_sign, n_lin, n_pair, pad, first group offset, end offset) of the
+    record chunk at rp."""
+    nl, npr, nw = w[rp], w[rp + 1], w[rp + 3]
+    npm, pad = nw & PM_COUNT_MASK, nw >> PM_PAD_SHIFT
+    g = rp + CHUNK_HDR + pad
+    end = g
+    for g0 in range(0, npm, PM_GROUP):
+        end += pm_group_dwords(min(npm - g0, PM_GROUP), W32, table)
+    end += nl * (W32 + 1) + npr * (2 * W32 + 3)
+    return npm, nl, npr, pad, g, end
 
 
 def s32(x: int) -> int:
@@ -103,6 +124,7 @@ class Packed:
         w = self.w
         gh = self.graph_hdr(c, i)
         nterms, W32, off = gh[0], gh[1], gh[3]
+        table = bool(gh[GH_FLAGS] & GF_TABLE)
         pv = param_words(bits, W32)
         th = off
         for _ in range(nterms):
@@ -122,17 +144,24 @@ class Packed:
                 rp += 2 * W32 + 2
             S += (ex & 1) << 2
             for _ in range(h[TH["NCHUNK"]]):
-                nl, npr, x, npm = w[rp], w[rp + 1], w[rp + 2], w[rp + 3]
-                rp += CHUNK_HDR
+                x = w[rp + 2]
+                npm, nl, npr, _, rp, _ = chunk_walk(w, rp, W32, table)
                 for g0 in range(0, npm, PM_GROUP):
                     n = min(npm - g0, PM_GROUP)
                     sp, sm, tp, tm = w[rp:rp + PM_HDR]
                     rp += PM_HDR
                     pk = 0
-                    for j in range(n):
-                        if popc_and(pv, w[rp:rp + W32]) & 1:
-                            pk |= 1 << (PM_GROUP - n + j)
-                        rp += W32
+                    if table:
+                        # XOR of the nibble tables' entries (16-byte aligned)
+                        assert rp % 4 == 0
+                        for j in range(8 * W32):
+                            pk ^= w[rp + 16 * j + ((pv[j // 8] >> (4 * (j % 8))) & 15)]
+                        rp += 128 * W32
+                    else:
+                        for j in range(n):
+                            if popc_and(pv, w[rp:rp + W32]) & 1:
+                                pk |= 1 << (PM_GROUP - n + j)
+                            rp += W32
                     npk = ~pk & 0xFFFFFFFF
                     cs = bin((pk & sp) | (npk & sm)).count("1")
                     cn = bin((pk & tp) | (npk & tm)).count("1")

@@ -11,7 +11,8 @@ from ksim_amd import synth
 from ksim_amd.program import FlatComponent, FlatProgram, graph_to_dict
 from ksim_amd.sampler import lookup_tables, pack_words
 from oracle import exact
-from tests.devfmt import CH, GH_DWORDS, PH_DWORDS, TH, TH_DWORDS, Packed
+from tests.devfmt import (CH, GF_TABLE, GH_DWORDS, GH_FLAGS, PH_DWORDS, TH, TH_DWORDS, Packed,
+                          chunk_walk)
 from tests.golden.golden_io import load_program
 
 
@@ -188,15 +189,57 @@ def chunk_shapes(words):
     ng = w[off + CH["NGRAPHS"]]
     gh = w[w[off + CH["GRAPH"]] + (ng - 1) * GH_DWORDS:][:GH_DWORDS]
     W32, th, out = gh[1], gh[3], []
+    table = bool(gh[GH_FLAGS] & GF_TABLE)
     for _ in range(gh[0]):
         h = w[th:th + TH_DWORDS]
         rp = th + TH_DWORDS + h[TH["NPROJ"]] * (W32 + 1) + h[TH["NPPX"]] * (2 * W32 + 2)
         for _ in range(h[TH["NCHUNK"]]):
-            nl, npr, npm = w[rp], w[rp + 1], w[rp + 3]
+            npm, nl, npr, _, _, rp = chunk_walk(w, rp, W32, table)
             out.append((npm, nl, npr))
-            rp += 4 + (npm + 31) // 32 * 4 + npm * W32 + nl * (W32 + 1) + npr * (2 * W32 + 3)
         th += TH_DWORDS + h[TH["SIZE"]]
     return out
+
+
+def table_graphs(words):
+    """Indices of the graphs of component 0 that carry parity tables."""
+    w = [int(x) for x in words]
+    off = w[PH_DWORDS]
+    gb = w[off + CH["GRAPH"]]
+    return [i for i in range(w[off + CH["NGRAPHS"]])
+            if w[gb + i * GH_DWORDS + GH_FLAGS] & GF_TABLE]
+
+
+def test_parity_tables_evaluate_as_the_oracle(tables):
+    # a program with graphs that need 128-bit sums (it runs the 128-bit
+    # kernels) carries the sign groups of its multi-term graphs as nibble
+    # parity tables: the XOR of one entry per nibble of the parameter words
+    # is the group's parity word, so every graph evaluates to the oracle's
+    # exact value (tables 16-byte aligned, checked by the interpreter)
+    prog = synth.shape_program(n_q=22, t=6, F=12, n_out=22, seed=4062)
+    words = pack_words(prog)
+    assert len(table_graphs(words)) >= 10
+    shapes = chunk_shapes(words)
+    assert sum(m for m, _, _ in shapes) > 0
+    assert check_program(prog, tables, rows=6, seed=7) > 0
+
+
+def test_parity_tables_only_where_the_128_bit_kernels_run():
+    # an all-64-bit program, or one whose component needs more than 8 lane
+    # words, keeps its masks: the kernels that run it read no tables
+    assert table_graphs(pack_words(synth.shape_program(n_q=20, t=12, F=24, n_out=8, seed=12))) == []
+    a = synth.shape_program(n_q=22, t=6, F=12, n_out=22, seed=4062)
+    assert table_graphs(pack_words(a))
+    # the same component beside one of 257 parameters (9 lane words): the
+    # program runs the kernels that keep parameter words in LDS
+    b = synth.shape_program(n_q=6, t=4, F=250, n_out=7, seed=250)
+    ca, cb = a.components[0], b.components[0]
+    n_a = len(ca.graphs) - 1
+    both = FlatProgram(
+        components=[ca, FlatComponent(tuple(range(n_a, n_a + len(cb.graphs) - 1)),
+                                      np.asarray(cb.f_selection), cb.graphs)],
+        output_order=np.arange(n_a + len(cb.graphs) - 1),
+        num_outputs=n_a + len(cb.graphs) - 1, num_detectors=0)
+    assert table_graphs(pack_words(both)) == []
 
 
 @pytest.mark.parametrize("n_sign", [1, 31, 32, 33, 63, 64, 65, 97])

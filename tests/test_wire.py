@@ -60,7 +60,7 @@ def test_blob_holds_the_packed_words(prog, blob):
     st = {}
     w = pack_words(prog, st)
     assert blob[:8] == b"KSIMPRG\0"
-    assert struct.unpack("<II", blob[8:16]) == (5, HDR)
+    assert struct.unpack("<II", blob[8:16]) == (6, HDR)
     assert np.array_equal(words_of(blob), w)
     assert struct.unpack("<Q", blob[24:32])[0] == fnv1a64(w.tobytes())
     got = blob_stats(blob)
@@ -107,7 +107,7 @@ def test_sign_records_written_and_counted():
     # node phases 1 + w^phi (phi odd or 2 mod 4) become sign records: the
     # packed words of a program with such factors are smaller than one
     # (mask, delta) pair per LIN record would make them
-    from tests.devfmt import CH, GH_DWORDS, PH_DWORDS, TH, TH_DWORDS
+    from tests.devfmt import CH, GF_TABLE, GH_DWORDS, GH_FLAGS, PH_DWORDS, TH, TH_DWORDS, chunk_walk
     w = [int(x) for x in pack_words(synth.shape_program(n_q=12, t=8, F=16, n_out=4, seed=2))]
     off = w[PH_DWORDS]
     n_pm = 0
@@ -118,10 +118,9 @@ def test_sign_records_written_and_counted():
             h = w[th:th + TH_DWORDS]
             rp = th + TH_DWORDS + h[TH["NPROJ"]] * (W32 + 1) + h[TH["NPPX"]] * (2 * W32 + 2)
             for _ in range(h[TH["NCHUNK"]]):
-                nl, npr, npm = w[rp], w[rp + 1], w[rp + 3]
+                npm, nl, npr, _, _, rp = chunk_walk(w, rp, W32, bool(gh[GH_FLAGS] & GF_TABLE))
                 assert nl + npr + npm <= 63
                 n_pm += npm
-                rp += 4 + (npm + 31) // 32 * 4 + npm * W32 + nl * (W32 + 1) + npr * (2 * W32 + 3)
             th += TH_DWORDS + h[TH["SIZE"]]
     assert n_pm > 0
 
@@ -139,7 +138,7 @@ def test_rejects_damaged_envelope(blob, case):
     if case == "magic":
         b[0] ^= 1
     elif case == "version":
-        b[8] ^= 3  # another version number
+        b[8] ^= 0x40  # a version number never written
     elif case == "truncated":
         b = b[:-4]
     elif case == "extra":

@@ -256,10 +256,18 @@ struct RegWords {
       const int lane = threadIdx.x & 63;
       const cword4 *src = (const cword4 *)tabs;
       lword4 *dst = (lword4 *)(uintptr_t)t.base;
+      // every load issued before the first store (one memory round trip)
+      constexpr int NL = (N4 + 63) / 64;
+      u32x4 tmp[NL];
 #pragma unroll
-      for (int i = 0; i < (N4 + 63) / 64; ++i) {
+      for (int i = 0; i < NL; ++i) {
         const int e = i * 64 + lane;
-        if (N4 % 64 == 0 || e < N4) dst[e] = src[e];
+        if (N4 % 64 == 0 || e < N4) tmp[i] = src[e];
+      }
+#pragma unroll
+      for (int i = 0; i < NL; ++i) {
+        const int e = i * 64 + lane;
+        if (N4 % 64 == 0 || e < N4) dst[e] = tmp[i];
       }
       wave_lds_sync();
       // read addresses: v_perm takes byte k of (word << 2) & 0x3c3c3c3c (the
@@ -389,7 +397,9 @@ __device__ __forceinline__ void term_exps(const cword *__restrict__ th, const PW
       ex ^= cs & cf;
     }
   }
-  S += (int)((ex & 1u) << 2);
+  // parity-table graphs carry their PROJ and PPX parities in the parity
+  // groups of the first chunk: jd collects the failed projectors there
+  uint32_t jd = 0;
   // LIN / PAIR chunks: packed exponent adds
   for (uint32_t ch = 0; ch < nchunk; ++ch) {
     const uint32_t nl = rp[0], npr = rp[1], nw = rp[3];
@@ -397,43 +407,67 @@ __device__ __forceinline__ void term_exps(const cword *__restrict__ th, const PW
     uint32_t x = rp[2];
     uint32_t touched = 0;
     if constexpr (LONG) {
-      const uint32_t ngr = (npm + PM_GROUP - 1) / PM_GROUP;
-      const uint32_t pmw = tab.base != NO_TBUF ? ngr * (uint32_t)pm_tab_group_dwords(W32)
-                                    : ngr * PM_HDR + npm * W32;
+      const uint32_t pmw = tab.base != NO_TBUF
+                               ? npm * (uint32_t)pm_tab_group_dwords(W32)
+                               : (npm + PM_GROUP - 1) / PM_GROUP * PM_HDR + npm * W32;
       touched = stream_touch(ps, rp, rp + CHUNK_HDR + pad + pmw + nl * (W32 + 1) +
                                          npr * (2 * W32 + 3) + STREAM_AHEAD);
     }
     rp += CHUNK_HDR + pad;
-    // sign records (S += 2s, N += t, s, t in {-1, 0, 1}): the parities of a
-    // group go into one word, record j of n at bit 32 - n + j (each funnel
-    // shift takes bit 0 of the popcount in at the top), then two popcounts
-    // over the group's sign masks give the S and N steps; the chunk base
-    // holds the -2^30 |S-| - 2^9 |N-| of the parity-0 records counted there
-    for (uint32_t g0 = 0; g0 < npm; g0 += PM_GROUP) {
-      const uint32_t rem = npm - g0;  // wave-uniform: kept on the scalar unit
-      const uint32_t n = rem < (uint32_t)PM_GROUP ? rem : (uint32_t)PM_GROUP;
-      const uint32_t sp = rp[0], sm = rp[1], tp = rp[2], tm = rp[3];
-      rp += PM_HDR;
-      uint32_t pk = 0;
-      if (tab.base != NO_TBUF) {
-        pk = pw.tab_parity(rp, tab);
+    if (tab.base != NO_TBUF) {
+      // npm parity groups, each a header [S+, S-, N+, N-, X, Xc, J, Jc] and
+      // its nibble tables: the parity word y holds one bit per form; sign
+      // records step S and N as below, PPX pairs (psi at bit 2m, phi at
+      // 2m + 1, both marked in X at 2m) add the parity of
+      // popcount(z & (z >> 1) & X) with z = y ^ Xc to the PPX sign, and a
+      // PROJ at a bit of J kills the term unless its parity is Jc's bit
+      for (uint32_t g = 0; g < npm; ++g) {
+        const uint32_t sp = rp[0], sm = rp[1], tp = rp[2], tm = rp[3];
+        const uint32_t xm = rp[4], xc = rp[5], jm = rp[6], jc = rp[7];
+        rp += PM_TAB_HDR;
+        const uint32_t y = pw.tab_parity(rp, tab);
         rp += pm_tab_dwords(W32);
-      } else if constexpr (LONG) {
-#pragma unroll 8
-        for (uint32_t k = 0; k < n; ++k, rp += W32)
-          pk = __builtin_amdgcn_alignbit(__popc(pw.mxor(rp)), pk, 1);
-      } else {
-#pragma unroll 4
-        for (uint32_t k = 0; k < n; ++k, rp += W32)
-          pk = __builtin_amdgcn_alignbit(__popc(pw.mxor(rp)), pk, 1);
+        const uint32_t cs = __popc((y & sp) | (~y & sm));
+        const uint32_t cn = __popc((y & tp) | (~y & tm));
+        uint32_t sn;
+        asm("v_lshl_add_u32 %0, %1, 21, %2" : "=v"(sn) : "v"(cs), "v"(cn));
+        asm("v_lshl_add_u32 %0, %1, 9, %0" : "+v"(x) : "v"(sn));
+        if (xm | jm) {  // wave-uniform
+          const uint32_t z = y ^ xc;
+          ex += __popc(z & (z >> 1) & xm);
+          jd |= (y ^ jc) & jm;
+        }
       }
-      const uint32_t cs = __popc((pk & sp) | (~pk & sm));
-      const uint32_t cn = __popc((pk & tp) | (~pk & tm));
-      // x += 2^30 cs + 2^9 cn as two shift-adds (the compiler's own form
-      // is two shifts and an add)
-      uint32_t sn;
-      asm("v_lshl_add_u32 %0, %1, 21, %2" : "=v"(sn) : "v"(cs), "v"(cn));
-      asm("v_lshl_add_u32 %0, %1, 9, %0" : "+v"(x) : "v"(sn));
+    } else {
+      // sign records (S += 2s, N += t, s, t in {-1, 0, 1}): the parities of
+      // a group go into one word, record j of n at bit 32 - n + j (each
+      // funnel shift takes bit 0 of the popcount in at the top), then two
+      // popcounts over the group's sign masks give the S and N steps; the
+      // chunk base holds the -2^30 |S-| - 2^9 |N-| of the parity-0 records
+      // counted there
+      for (uint32_t g0 = 0; g0 < npm; g0 += PM_GROUP) {
+        const uint32_t rem = npm - g0;  // wave-uniform: kept on the scalar unit
+        const uint32_t n = rem < (uint32_t)PM_GROUP ? rem : (uint32_t)PM_GROUP;
+        const uint32_t sp = rp[0], sm = rp[1], tp = rp[2], tm = rp[3];
+        rp += PM_HDR;
+        uint32_t pk = 0;
+        if constexpr (LONG) {
+#pragma unroll 8
+          for (uint32_t k = 0; k < n; ++k, rp += W32)
+            pk = __builtin_amdgcn_alignbit(__popc(pw.mxor(rp)), pk, 1);
+        } else {
+#pragma unroll 4
+          for (uint32_t k = 0; k < n; ++k, rp += W32)
+            pk = __builtin_amdgcn_alignbit(__popc(pw.mxor(rp)), pk, 1);
+        }
+        const uint32_t cs = __popc((pk & sp) | (~pk & sm));
+        const uint32_t cn = __popc((pk & tp) | (~pk & tm));
+        // x += 2^30 cs + 2^9 cn as two shift-adds (the compiler's own form
+        // is two shifts and an add)
+        uint32_t sn;
+        asm("v_lshl_add_u32 %0, %1, 21, %2" : "=v"(sn) : "v"(cs), "v"(cn));
+        asm("v_lshl_add_u32 %0, %1, 9, %0" : "+v"(x) : "v"(sn));
+      }
     }
     // LIN records, unrolled so each scalar load fetches several records
     if constexpr (LONG) {
@@ -472,6 +506,8 @@ __device__ __forceinline__ void term_exps(const cword *__restrict__ th, const PW
     Bq += (int)((x >> PK_OB) & ((1u << PK_WB) - 1));
     S += (int)(x >> PK_OS);
   }
+  S += (int)((ex & 1u) << 2);
+  if constexpr (PW::kTab) dead |= jd ? 1u : 0u;
 }
 
 // materialise w^S (1+w)^V (1+sqrt2)^N pi^A pibar^B R:

@@ -89,18 +89,26 @@ constexpr int PM_HDR = 4;     // S+, S-, N+, N- masks over the group's records
 inline uint64_t pm_dwords(uint64_t n_pm, uint64_t W32) {
   return (n_pm + PM_GROUP - 1) / PM_GROUP * PM_HDR + n_pm * W32;
 }
-// Parity tables (graphs flagged GF_TABLE, at most PM_TAB_W32_MAX mask words):
-// a sign group is its PM_HDR header and then, for each of the 8 W32 nibbles
-// j of the parameter words (bits 4j..4j+3), 16 dwords T_j[v], v < 16: bit
-// 32 - n + r of T_j[v] is the parity of v & (nibble j of record r's mask).
-// The XOR over j of T_j[nibble j of p] is the group's parity word.  The n_pm
-// word of such a chunk also holds, at bit PM_PAD_SHIFT, the count (< 4) of
-// zero dwords between the chunk header and its first group, so that every
-// group's tables start on a 16-byte boundary of the program words.
+// Parity groups (graphs flagged GF_TABLE, at most PM_TAB_W32_MAX mask words):
+// a chunk's count word holds its number of groups (and, at bit
+// PM_PAD_SHIFT, the count (< 4) of zero dwords between the chunk header and
+// its first group, so that every group's tables start on a 16-byte boundary
+// of the program words).  A group is the PM_TAB_HDR-dword header [S+, S-,
+// N+, N-, X, Xc, J, Jc] and, for each of the 8 W32 nibbles j of the
+// parameter words (bits 4j..4j+3), 16 dwords T_j[v], v < 16: bit b of
+// T_j[v] is the parity of v & (nibble j of the mask of the form at bit b),
+// so the XOR over j of T_j[nibble j of p] is the parity word y of the
+// group's (at most 32) forms.  Forms are sign records (marked in S+/S- and
+// N+/N- as above, counted in the chunk base the same way), PPX pairs (psi
+// at bit 2m, phi at bit 2m + 1, X holding bit 2m, Xc their constant
+// parities) and PROJ projectors (J holding the bit, Jc the parity the term
+// needs).  A term of such a graph keeps no PROJ or PPX records of its own:
+// they sit in the groups of its first chunk.
 constexpr int PM_TAB_W32_MAX = 4;
+constexpr int PM_TAB_HDR = 8;
 constexpr uint32_t PM_COUNT_MASK = 0xffu, PM_PAD_SHIFT = 8;
 constexpr uint32_t pm_tab_dwords(uint32_t W32) { return 128 * W32; }
-constexpr uint32_t pm_tab_group_dwords(uint32_t W32) { return PM_HDR + pm_tab_dwords(W32); }
+constexpr uint32_t pm_tab_group_dwords(uint32_t W32) { return PM_TAB_HDR + pm_tab_dwords(W32); }
 
 // graph header (dwords).  The terms of a graph are stored as GH_PARTS parts
 // of about equal record work (the packer deals them out heaviest first), so

@@ -586,100 +586,195 @@ void pack_graph(const ksim_graph_view &g, std::vector<uint32_t> &out, uint32_t *
 }
 
 // Rewrite a packed graph (offsets relative to its first word, sign groups
-// as masks) so that its sign groups carry parity tables (GF_TABLE; layout.hpp
-// gives the form).  The words must then start at a multiple of 4 dwords in
+// as masks) into parity groups (GF_TABLE; layout.hpp gives the form): each
+// chunk's sign records, and in its first chunk also the term's PPX pairs
+// and PROJ projectors, become forms of groups of at most 32 with nibble
+// parity tables.  The words must then start at a multiple of 4 dwords in
 // the program, which makes every group's tables 16-byte aligned.  A graph
 // keeps its masks when it is wider than PM_TAB_W32_MAX words, has at most
-// one term or approximate factors, or when its sign groups hold fewer than
-// PM_TAB_MIN_FILL records on average: a table group costs about the same
-// whatever its record count, a mask group W32 + 2 ops per record.
+// one term or approximate factors, or when its groups would hold fewer than
+// PM_TAB_MIN_FILL forms on average: a group costs about the same whatever
+// its form count, a mask record W32 + 2 ops or more.
 constexpr int PM_TAB_MIN_FILL = 8;
+
+namespace {
+
+struct Form {
+  const uint32_t *m;   // W32 mask words
+  uint32_t kind;       // 0 sign, 1 PPX psi, 2 PPX phi, 3 PROJ
+  int s = 0, t = 0;    // sign record steps
+  uint32_t c = 0;      // PPX constant parity / PROJ expected parity
+};
+
+// one group of at most PM_GROUP forms (form b at bit b; a PPX psi form is
+// followed by its phi form), appended to out
+void emit_group(const std::vector<Form> &f, uint32_t W32, std::vector<uint32_t> &out) {
+  {
+    const size_t g0 = 0, n = f.size();
+    uint32_t h[PM_TAB_HDR] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (size_t b = 0; b < n; ++b) {
+      const Form &x = f[g0 + b];
+      const uint32_t bit = 1u << b;
+      if (x.kind == 0) {
+        if (x.s > 0) h[0] |= bit;
+        if (x.s < 0) h[1] |= bit;
+        if (x.t > 0) h[2] |= bit;
+        if (x.t < 0) h[3] |= bit;
+      } else if (x.kind == 1) {
+        h[4] |= bit;
+        if (x.c & 1u) h[5] |= bit;
+      } else if (x.kind == 2) {
+        if (x.c & 1u) h[5] |= bit;
+      } else {
+        h[6] |= bit;
+        if (x.c & 1u) h[7] |= bit;
+      }
+    }
+    out.insert(out.end(), h, h + PM_TAB_HDR);
+    for (uint32_t j = 0; j < 8 * W32; ++j) {
+      uint32_t col[4] = {0, 0, 0, 0};  // the entries of v = 1, 2, 4, 8
+      for (size_t b = 0; b < n; ++b) {
+        const uint32_t nib = (f[g0 + b].m[j / 8] >> (4 * (j % 8))) & 15u;
+        for (int k = 0; k < 4; ++k)
+          if (nib >> k & 1u) col[k] |= 1u << b;
+      }
+      for (uint32_t v = 0; v < 16; ++v) {
+        uint32_t e = 0;
+        for (int k = 0; k < 4; ++k)
+          if (v >> k & 1u) e ^= col[k];
+        out.push_back(e);
+      }
+    }
+  }
+}
+
+}  // namespace
 
 bool tabulate_graph(std::vector<uint32_t> &words, uint32_t *ghdr) {
   const uint32_t W32 = ghdr[GH_W32], nt = ghdr[GH_NTERMS];
   if (W32 > (uint32_t)PM_TAB_W32_MAX || nt < 2 || (ghdr[GH_FLAGS] & (GF_APPROX | GF_MONO)))
     return false;
-  // walk the terms: (term offset, chunk offsets) and the sign-record fill
-  size_t groups = 0, recs = 0;
-  {
-    size_t to = ghdr[GH_TERM_OFF];
-    for (uint32_t t = 0; t < nt; ++t) {
-      const uint32_t *th = &words[to];
-      size_t rp = to + TH_DWORDS + (size_t)th[TH_NPROJ] * (W32 + 1) +
-                  (size_t)th[TH_NPPX] * (2 * W32 + 2);
-      for (uint32_t c = 0; c < th[TH_NCHUNK]; ++c) {
-        const uint32_t nl = words[rp], np = words[rp + 1], nm = words[rp + 3];
-        groups += (nm + PM_GROUP - 1) / PM_GROUP;
-        recs += nm;
-        rp += CHUNK_HDR + pm_dwords(nm, W32) + (size_t)nl * (W32 + 1) + (size_t)np * (2 * W32 + 3);
-      }
-      to += TH_DWORDS + th[TH_SIZE];
-    }
-  }
-  if (groups == 0 || recs < (size_t)PM_TAB_MIN_FILL * groups) return false;
+  const std::vector<uint32_t> &w = words;
   std::vector<uint32_t> out;
-  out.reserve(words.size() + groups * pm_tab_dwords(W32));
-  out.insert(out.end(), words.begin(), words.begin() + ghdr[GH_TERM_OFF]);
+  out.reserve(words.size() * 3);
+  out.insert(out.end(), w.begin(), w.begin() + ghdr[GH_TERM_OFF]);
   std::vector<std::pair<uint32_t, uint32_t>> moved;  // old -> new term offsets
+  size_t forms = 0, groups = 0;
   size_t to = ghdr[GH_TERM_OFF];
   for (uint32_t t = 0; t < nt; ++t) {
     const size_t nto = out.size();
     moved.emplace_back((uint32_t)to, (uint32_t)nto);
-    const uint32_t *th = &words[to];
-    const size_t lead = TH_DWORDS + (size_t)th[TH_NPROJ] * (W32 + 1) +
-                        (size_t)th[TH_NPPX] * (2 * W32 + 2);
-    out.insert(out.end(), words.begin() + to, words.begin() + to + lead);
-    size_t rp = to + lead;
-    for (uint32_t c = 0; c < th[TH_NCHUNK]; ++c) {
-      const uint32_t nl = words[rp], np = words[rp + 1], base = words[rp + 2], nm = words[rp + 3];
-      // zero dwords after the header so that the first group's tables start
-      // at a multiple of 4 dwords
-      const uint32_t pad = nm ? (uint32_t)((4 - (out.size() + CHUNK_HDR + PM_HDR) % 4) % 4) : 0u;
-      out.push_back(nl); out.push_back(np); out.push_back(base);
-      out.push_back(nm | (pad << PM_PAD_SHIFT));
-      out.insert(out.end(), pad, 0u);
-      size_t q = rp + CHUNK_HDR;
+    const uint32_t *th = &w[to];
+    const uint32_t nproj = th[TH_NPROJ], nppx = th[TH_NPPX], nchunk = th[TH_NCHUNK];
+    // the term's PPX pairs (two adjacent forms) and projectors fill the free
+    // slots of the chunks' last groups; those left over stay mask records
+    // (a group of their own measured slower: more table bytes per form)
+    std::vector<std::vector<Form>> pool;
+    size_t rp = to + TH_DWORDS;
+    std::vector<std::vector<Form>> proj;
+    for (uint32_t k = 0; k < nproj; ++k, rp += W32 + 1) {
+      Form f{&w[rp], 3u};
+      f.c = w[rp + W32];
+      proj.push_back({f});
+    }
+    for (uint32_t k = 0; k < nppx; ++k, rp += 2 * W32 + 2) {
+      Form a{&w[rp], 1u}, b{&w[rp + W32], 2u};
+      a.c = w[rp + 2 * W32];
+      b.c = w[rp + 2 * W32 + 1];
+      pool.push_back({a, b});
+    }
+    pool.insert(pool.end(), proj.begin(), proj.end());
+    std::vector<char> used(pool.size(), 0);
+    // the records of the pool items placed in groups are known only after
+    // the chunks: emit the chunks to a side buffer first
+    std::vector<uint32_t> body;
+    for (uint32_t c = 0; c < nchunk; ++c) {
+      const uint32_t nl = w[rp], np = w[rp + 1], base = w[rp + 2], nm = w[rp + 3];
+      rp += CHUNK_HDR;
+      std::vector<std::vector<Form>> gs;
       for (uint32_t g0 = 0; g0 < nm; g0 += PM_GROUP) {
         const uint32_t n = std::min<uint32_t>(nm - g0, PM_GROUP);
-        out.insert(out.end(), words.begin() + q, words.begin() + q + PM_HDR);
-        const uint32_t *m = &words[q + PM_HDR];
-        for (uint32_t j = 0; j < 8 * W32; ++j) {
-          uint32_t col[4] = {0, 0, 0, 0};  // parity-table entries of v = 1, 2, 4, 8
-          for (uint32_t r = 0; r < n; ++r) {
-            const uint32_t nib = (m[r * W32 + j / 8] >> (4 * (j % 8))) & 15u;
-            const uint32_t bit = 1u << (PM_GROUP - n + r);
-            for (int b = 0; b < 4; ++b)
-              if (nib >> b & 1u) col[b] |= bit;
-          }
-          for (uint32_t v = 0; v < 16; ++v) {
-            uint32_t e = 0;
-            for (int b = 0; b < 4; ++b)
-              if (v >> b & 1u) e ^= col[b];
-            out.push_back(e);
-          }
+        const uint32_t sp = w[rp], sm = w[rp + 1], tp = w[rp + 2], tm = w[rp + 3];
+        rp += PM_HDR;
+        gs.emplace_back();
+        for (uint32_t j = 0; j < n; ++j, rp += W32) {
+          const uint32_t bit = 1u << (PM_GROUP - n + j);
+          Form x{&w[rp], 0u};
+          x.s = (sp & bit) ? 1 : (sm & bit) ? -1 : 0;
+          x.t = (tp & bit) ? 1 : (tm & bit) ? -1 : 0;
+          gs.back().push_back(x);
         }
-        q += PM_HDR + (size_t)n * W32;
       }
+      if (!gs.empty())
+        for (size_t k = 0; k < pool.size(); ++k)
+          if (!used[k] && gs.back().size() + pool[k].size() <= (size_t)PM_GROUP) {
+            gs.back().insert(gs.back().end(), pool[k].begin(), pool[k].end());
+            used[k] = 1;
+          }
+      const uint32_t ng = (uint32_t)gs.size();
+      body.push_back(nl); body.push_back(np); body.push_back(base);
+      body.push_back(ng);  // the padding is set below, once the offset is known
+      for (const auto &g : gs) {
+        emit_group(g, W32, body);
+        forms += g.size();
+      }
+      groups += ng;
       const size_t rest = (size_t)nl * (W32 + 1) + (size_t)np * (2 * W32 + 3);
-      out.insert(out.end(), words.begin() + q, words.begin() + q + rest);
-      rp = q + rest;
+      body.insert(body.end(), w.begin() + rp, w.begin() + rp + rest);
+      rp += rest;
+    }
+    // the term: its header, the PROJ and PPX records left over, the chunks
+    // (each with the zero dwords that put its first group's tables, after
+    // the PM_TAB_HDR-dword header, at a multiple of 4 dwords)
+    out.insert(out.end(), th, th + TH_DWORDS);
+    uint32_t lp = 0, lx = 0;
+    for (size_t k = 0; k < pool.size(); ++k)
+      if (!used[k] && pool[k][0].kind == 3u) {
+        out.insert(out.end(), pool[k][0].m, pool[k][0].m + W32);
+        out.push_back(pool[k][0].c);
+        ++lp;
+      }
+    for (size_t k = 0; k < pool.size(); ++k)
+      if (!used[k] && pool[k][0].kind == 1u) {
+        out.insert(out.end(), pool[k][0].m, pool[k][0].m + W32);
+        out.insert(out.end(), pool[k][1].m, pool[k][1].m + W32);
+        out.push_back(pool[k][0].c);
+        out.push_back(pool[k][1].c);
+        ++lx;
+      }
+    out[nto + TH_NPROJ] = lp;
+    out[nto + TH_NPPX] = lx;
+    for (size_t q = 0; q < body.size();) {
+      const uint32_t nl = body[q], np = body[q + 1], ng = body[q + 3];
+      const uint32_t pad =
+          ng ? (uint32_t)((4 - (out.size() + CHUNK_HDR + PM_TAB_HDR) % 4) % 4) : 0u;
+      out.insert(out.end(), body.begin() + q, body.begin() + q + 3);
+      out.push_back(ng | (pad << PM_PAD_SHIFT));
+      out.insert(out.end(), pad, 0u);
+      const size_t len = (size_t)ng * pm_tab_group_dwords(W32) + (size_t)nl * (W32 + 1) +
+                         (size_t)np * (2 * W32 + 3);
+      out.insert(out.end(), body.begin() + q + CHUNK_HDR, body.begin() + q + CHUNK_HDR + len);
+      q += CHUNK_HDR + len;
     }
     to += TH_DWORDS + th[TH_SIZE];
     out[nto + TH_SIZE] = (uint32_t)(out.size() - nto - TH_DWORDS);
   }
+  if (groups == 0 || forms < (size_t)PM_TAB_MIN_FILL * groups) return false;
   moved.emplace_back((uint32_t)to, (uint32_t)out.size());  // the end (empty trailing parts)
+  uint32_t po[GH_PARTS - 1];
   for (int p = 0; p < GH_PARTS - 1; ++p) {
-    uint32_t &o = ghdr[GH_PART_O1 + p];
+    const uint32_t o = ghdr[GH_PART_O1 + p];
     bool found = false;
     for (const auto &mv : moved)
       if (mv.first == o) {
-        o = mv.second;
+        po[p] = mv.second;
         found = true;
         break;
       }
     if (!found) throw std::logic_error("term part offset is not a term boundary");
   }
-  out.insert(out.end(), words.begin() + to, words.end());  // nothing follows the terms
+  for (int p = 0; p < GH_PARTS - 1; ++p) ghdr[GH_PART_O1 + p] = po[p];
+  out.insert(out.end(), w.begin() + to, w.end());  // nothing follows the terms
   words.swap(out);
   ghdr[GH_FLAGS] |= GF_TABLE;
   return true;

@@ -133,12 +133,13 @@ void validate_words(const uint32_t *w, size_t n) {
           const uint32_t nw = w[rp + 3];
           const uint64_t nl = w[rp], np = w[rp + 1], nm = nw & PM_COUNT_MASK;
           const uint64_t pad = nw >> PM_PAD_SHIFT;
-          if (nl + np + nm > (uint64_t)CHUNK_MAX) bad("record chunk longer than its limit");
+          // (in parity-group graphs nm counts groups, not sign records)
+          if (nl + np + (table ? 0 : nm) > (uint64_t)CHUNK_MAX)
+            bad("record chunk longer than its limit");
           if (pad && !(table && nm)) bad("padding in a chunk without parity tables");
-          if (table && nm && (pad > 3 || (rp + CHUNK_HDR + pad + PM_HDR) % 4 != 0))
+          if (table && nm && (pad > 3 || (rp + CHUNK_HDR + pad + PM_TAB_HDR) % 4 != 0))
             bad("parity tables not on a 16-byte boundary");
-          const uint64_t pm = table ? (nm + PM_GROUP - 1) / PM_GROUP * pm_tab_group_dwords(W32)
-                                    : pm_dwords(nm, W32);
+          const uint64_t pm = table ? nm * pm_tab_group_dwords(W32) : pm_dwords(nm, W32);
           const uint64_t cs = CHUNK_HDR + pad + pm + nl * (W32 + 1) + np * (2 * W32 + 3);
           sz += cs;
           rp += cs;

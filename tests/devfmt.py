@@ -26,20 +26,20 @@ GF_TABLE = 16
 GH_FLAGS = 5
 
 
-def pm_group_dwords(n, W32, table):
-    """Dwords of a sign group of n records (mask or parity-table form)."""
-    return PM_HDR + (128 * W32 if table else n * W32)
+PM_TAB_HDR = 8
 
 
 def chunk_walk(w, rp, W32, table):
-    """(n_sign, n_lin, n_pair, pad, first group offset, end offset) of the
-    record chunk at rp."""
+    """(count, n_lin, n_pair, pad, first group offset, end offset) of the
+    record chunk at rp; count is the number of sign records (mask form) or
+    of parity groups (GF_TABLE graphs)."""
     nl, npr, nw = w[rp], w[rp + 1], w[rp + 3]
     npm, pad = nw & PM_COUNT_MASK, nw >> PM_PAD_SHIFT
     g = rp + CHUNK_HDR + pad
-    end = g
-    for g0 in range(0, npm, PM_GROUP):
-        end += pm_group_dwords(min(npm - g0, PM_GROUP), W32, table)
+    if table:
+        end = g + npm * (PM_TAB_HDR + 128 * W32)
+    else:
+        end = g + (npm + PM_GROUP - 1) // PM_GROUP * PM_HDR + npm * W32
     end += nl * (W32 + 1) + npr * (2 * W32 + 3)
     return npm, nl, npr, pad, g, end
 
@@ -142,30 +142,41 @@ class Packed:
                 cf = w[rp + 2 * W32 + 1] + popc_and(pv, w[rp + W32:rp + 2 * W32])
                 ex ^= cs & cf
                 rp += 2 * W32 + 2
-            S += (ex & 1) << 2
+            jd = 0
             for _ in range(h[TH["NCHUNK"]]):
                 x = w[rp + 2]
                 npm, nl, npr, _, rp, _ = chunk_walk(w, rp, W32, table)
-                for g0 in range(0, npm, PM_GROUP):
-                    n = min(npm - g0, PM_GROUP)
-                    sp, sm, tp, tm = w[rp:rp + PM_HDR]
-                    rp += PM_HDR
-                    pk = 0
-                    if table:
-                        # XOR of the nibble tables' entries (16-byte aligned)
+                groups = []
+                if table:
+                    for _ in range(npm):
+                        # parity word: XOR of the nibble tables' entries (16-byte aligned)
+                        hdr = w[rp:rp + PM_TAB_HDR]
+                        rp += PM_TAB_HDR
                         assert rp % 4 == 0
+                        y = 0
                         for j in range(8 * W32):
-                            pk ^= w[rp + 16 * j + ((pv[j // 8] >> (4 * (j % 8))) & 15)]
+                            y ^= w[rp + 16 * j + ((pv[j // 8] >> (4 * (j % 8))) & 15)]
                         rp += 128 * W32
-                    else:
+                        groups.append((y, hdr))
+                else:
+                    for g0 in range(0, npm, PM_GROUP):
+                        n = min(npm - g0, PM_GROUP)
+                        hdr = w[rp:rp + PM_HDR] + [0, 0, 0, 0]
+                        rp += PM_HDR
+                        y = 0
                         for j in range(n):
                             if popc_and(pv, w[rp:rp + W32]) & 1:
-                                pk |= 1 << (PM_GROUP - n + j)
+                                y |= 1 << (PM_GROUP - n + j)
                             rp += W32
-                    npk = ~pk & 0xFFFFFFFF
-                    cs = bin((pk & sp) | (npk & sm)).count("1")
-                    cn = bin((pk & tp) | (npk & tm)).count("1")
+                        groups.append((y, hdr))
+                for y, (sp, sm, tp, tm, xm, xc, jm, jc) in groups:
+                    ny = ~y & 0xFFFFFFFF
+                    cs = bin((y & sp) | (ny & sm)).count("1")
+                    cn = bin((y & tp) | (ny & tm)).count("1")
                     x = (x + (cs << 30) + (cn << 9)) & 0xFFFFFFFF
+                    z = y ^ xc
+                    ex += bin(z & (z >> 1) & xm).count("1")
+                    jd |= (y ^ jc) & jm
                 for _ in range(nl):
                     if popc_and(pv, w[rp:rp + W32]) & 1:
                         x = (x + w[rp + W32]) & 0xFFFFFFFF
@@ -186,6 +197,8 @@ class Packed:
                 A += (x >> PK_OA) & ((1 << PK_WA) - 1)
                 B += (x >> PK_OB) & ((1 << PK_WB) - 1)
                 S += x >> PK_OS
+            S += (ex & 1) << 2
+            dead |= 1 if jd else 0
             q, r = V >> 2, V & 3
             n2, s2 = N + 2 * q, (S + 2 * q) & 7
             if not -self.W <= n2 <= self.W:

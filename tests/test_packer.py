@@ -194,7 +194,10 @@ def chunk_shapes(words):
         h = w[th:th + TH_DWORDS]
         rp = th + TH_DWORDS + h[TH["NPROJ"]] * (W32 + 1) + h[TH["NPPX"]] * (2 * W32 + 2)
         for _ in range(h[TH["NCHUNK"]]):
-            npm, nl, npr, _, _, rp = chunk_walk(w, rp, W32, table)
+            npm, nl, npr, _, g, rp = chunk_walk(w, rp, W32, table)
+            if table:  # sign records: the forms marked in S+, S-, N+, N- of each group
+                gs = [w[g + k * (8 + 128 * W32):][:4] for k in range(npm)]
+                npm = sum(bin(a | b | c | d).count("1") for a, b, c, d in gs)
             out.append((npm, nl, npr))
         th += TH_DWORDS + h[TH["SIZE"]]
     return out

@@ -197,8 +197,8 @@ void ksim_program_destroy(ksim_program *p);
 
 /* ---- wire format: a packed program as one validated blob ---------------- */
 
-/* Program blob layout, version 5 (little-endian; version 3 and 4 blobs are
- * read too): "KSIMPRG\0", u32 version,
+/* Program blob layout, version 6 (little-endian; version 3, 4 and 5 blobs
+ * are read too): "KSIMPRG\0", u32 version,
  * u32 header bytes (128), u64 word count, u64 FNV-1a 64 of the words,
  * 8 f64 stats (ksim_program_stats order), u32 n_comp / num_f / num_outputs /
  * total outputs / lane words, 12 zero bytes, then the u32 program words.

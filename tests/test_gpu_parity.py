@@ -9,7 +9,7 @@ import pytest
 
 from ksim_amd import synth
 from ksim_amd.program import graph_to_dict
-from ksim_amd.sampler import Component, HipProgramSampler, sample_flat
+from ksim_amd.sampler import Component, HipProgramSampler, pack_words, sample_flat
 from oracle import exact
 from tests.golden.golden_io import load_program
 
@@ -326,6 +326,26 @@ def test_sign_record_groups(n_sign):
     B = 500
     f = synth.random_f(B, s.min_num_f, 0.5, n_sign)
     U = np.random.default_rng(n_sign).random((B, s.tot_out))
+    assert np.array_equal(s.sample_uniforms(f, U), exact.sample_program(prog, f, U))
+
+
+@pytest.mark.parametrize("split", [1, 2, 4])
+def test_parity_groups(split):
+    # a program on the 128-bit kernels carries its sign records, PPX pairs
+    # and projectors as nibble parity tables read through LDS: exact values
+    # (the eval kernel's buffer), and the fed-uniform stream through the
+    # plain kernel (its own buffer) and the term-split kernel (half of the
+    # exchange area, alternating with 64-bit exchanges) as the oracle
+    from tests.test_packer import table_graphs
+    prog = synth.shape_program(n_q=22, t=6, F=12, n_out=22, seed=4062)
+    assert len(table_graphs(pack_words(prog))) >= 10
+    if split == 1:
+        assert gpu_exact_matches(prog, rows=8, seed=11) > 0
+    s = HipProgramSampler(prog)
+    s.set_term_split(split)
+    B = 300  # a ragged last tile
+    f = synth.random_f(B, s.min_num_f, 0.5, 30 + split)
+    U = np.random.default_rng(40 + split).random((B, s.tot_out))
     assert np.array_equal(s.sample_uniforms(f, U), exact.sample_program(prog, f, U))
 
 

@@ -220,6 +220,58 @@ def test_rejects_approximate_graph_corruption():
     blob_stats(with_words(blob, w))
 
 
+def first_table_chunk(w):
+    """(graph header offset, offset of the first record chunk with parity
+    groups) in the words of a program with parity tables."""
+    PH, GH = 8, 16
+    c0 = int(w[PH])
+    g0, ng = int(w[c0 + 7]), int(w[c0])
+    for i in range(ng):
+        gh = g0 + GH * i
+        if not int(w[gh + 5]) & 16:  # GF_TABLE
+            continue
+        W32, th = int(w[gh + 1]), int(w[gh + 3])
+        for _ in range(int(w[gh])):
+            rp = th + 16 + int(w[th]) * (W32 + 1) + int(w[th + 1]) * (2 * W32 + 2)
+            if int(w[th + 2]) and int(w[rp + 3]) & 0xFF:
+                return gh, rp
+            th += 16 + int(w[th + 14])
+    raise AssertionError("no parity groups")
+
+
+def test_parity_group_blobs():
+    # a program that runs the 128-bit kernels carries parity tables: its blob
+    # round-trips; tables off a 16-byte boundary, a padding word in a chunk
+    # without groups, tables in a program that runs the 64-bit kernels and
+    # tables on a graph wider than 4 mask words are each rejected
+    prog = synth.shape_program(n_q=22, t=6, F=12, n_out=22, seed=4062)
+    blob = program_blob(prog)
+    w = words_of(blob)
+    assert struct.unpack("<I", blob[8:12])[0] == 6
+    gh, rp = first_table_chunk(w)
+    assert (rp + 4 + (int(w[rp + 3]) >> 8) + 8) % 4 == 0
+    blob_stats(with_words(blob, w))
+    v = w.copy()
+    v[rp + 3] ^= np.uint32(1 << 8)  # one dword more or less of padding
+    with pytest.raises(ValueError):
+        blob_stats(with_words(blob, v))
+    v = w.copy()
+    v[gh + 1] = 5  # W32 beyond the tables' limit (also the graph's own width)
+    with pytest.raises(ValueError):
+        blob_stats(with_words(blob, v))
+    # an all-64-bit program with a table flag on a multi-term graph
+    narrow = program_blob(synth.shape_program(n_q=8, t=6, F=6, n_out=7, seed=3))
+    u = words_of(narrow)
+    c0 = int(u[8])
+    g0, ng = int(u[c0 + 7]), int(u[c0])
+    multi = [g0 + 16 * i for i in range(ng) if int(u[g0 + 16 * i]) > 1]
+    assert multi
+    bad = u.copy()
+    bad[multi[0] + 5] |= np.uint32(16)
+    with pytest.raises(ValueError):
+        blob_stats(with_words(narrow, bad))
+
+
 def test_stream_flag_needs_its_padding():
     # the packer flags every multi-term graph GF_STREAM (record read-ahead)
     # and ends the words in 256 zero dwords; a flagged graph without that

@@ -156,8 +156,10 @@ template <int WL, bool NOISE = false, bool WIDE = true>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WIDE ? KSIM_WAVES_PER_EU : KSIM_NARROW_WAVES_PER_EU))) void k_sample(ShotArgs a, Tables tb) {
   __shared__ LdsTables lt;
   __shared__ uint32_t stage[4][NOISE ? 1 : stage_words(WL)];
-  // the waves' parity-table buffers (128-bit kernels: programs with tables)
-  __shared__ __attribute__((aligned(256))) uint4 tbl[WIDE ? 4 : 1][WIDE ? 128 : 1];
+  // the waves' parity-table buffers: 2 KiB in the 128-bit kernels, 1 KiB
+  // in the others up to PM_TAB_NARROW_WL lane words
+  constexpr int kTbl = WIDE ? 128 : (WL <= PM_TAB_NARROW_WL ? 64 : 0);
+  __shared__ __attribute__((aligned(256))) uint4 tbl[kTbl ? 4 : 1][kTbl ? kTbl : 1];
   prefetch_program(a.prog, a.prefetch_dwords);
   load_tables(&lt, tb);
   const int lane = threadIdx.x & 63;
@@ -167,7 +169,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WIDE ? K
   const int nrows = (int)min((int64_t)64, a.B - r0);
   const int64_t s = r0 + min(lane, nrows - 1);
   const FStage fs{NOISE ? nullptr : stage[wave], r0, nrows};
-  const uint32_t tbuf = WIDE ? lds_addr(tbl[wave]) : NO_TBUF;
+  const uint32_t tbuf = kTbl ? lds_addr(tbl[wave]) : NO_TBUF;
   run_shot<WL, NOISE, false, WIDE>(a, (int)blockIdx.y, s, &lt, tb,
                                    Split{1, 0, 0, nullptr, tbuf}, lane < nrows, nullptr, fs);
 }

@@ -234,77 +234,89 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Parity word of the group whose tables start at `tabs` (16-byte aligned,
+// pm_tab_dwords(W32) dwords) for the lane's W32 parameter words pw, through
+// the wave's buffer at LDS byte address `tbase`.
+template <int W32, class Word>
+__device__ __forceinline__ uint32_t tab_parity_words(const Word &pw, const cword *__restrict__ tabs,
+                                                     uint32_t tbase) {
+  constexpr int N4 = (int)pm_tab_dwords(W32) / 4;
+  const int lane = threadIdx.x & 63;
+  const cword4 *src = (const cword4 *)tabs;
+  lword4 *dst = (lword4 *)(uintptr_t)tbase;
+  // every load issued before the first store (one memory round trip)
+  constexpr int NL = (N4 + 63) / 64;
+  u32x4 tmp[NL];
+#pragma unroll
+  for (int i = 0; i < NL; ++i) {
+    const int e = i * 64 + lane;
+    if (N4 % 64 == 0 || e < N4) tmp[i] = src[e];
+  }
+#pragma unroll
+  for (int i = 0; i < NL; ++i) {
+    const int e = i * 64 + lane;
+    if (N4 % 64 == 0 || e < N4) dst[e] = tmp[i];
+  }
+  wave_lds_sync();
+  // read addresses: v_perm takes byte k of (word << 2) & 0x3c3c3c3c (the
+  // even nibbles times 4) or of (word >> 2) & 0x3c3c3c3c (the odd ones)
+  // into byte 0, under bytes 1 and 2 of the base (256-byte aligned, below
+  // 2^24).  (The opaque copies keep the compiler from building all 8 W32
+  // addresses once per graph, which spilled them.)
+  uint32_t base = tbase;
+  asm volatile("" : "+s"(base));
+  uint32_t y = 0;
+#pragma unroll
+  for (int w = 0; w < W32; ++w) {
+    uint32_t x = pw(w);
+    asm volatile("" : "+v"(x));
+    const uint32_t pe = (x << 2) & 0x3c3c3c3cu, po = (x >> 2) & 0x3c3c3c3cu;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint32_t sel = 0x0c060500u | (uint32_t)k;
+      const int j = 8 * w + 2 * k;
+      const uint32_t ae = __builtin_amdgcn_perm(base, pe, sel);
+      const uint32_t ao = __builtin_amdgcn_perm(base, po, sel);
+      const uint32_t u = ((const lword *)(uintptr_t)ae)[16 * j];
+      const uint32_t v = ((const lword *)(uintptr_t)ao)[16 * (j + 1)];
+      y = __builtin_amdgcn_bitop3_b32(u, v, y, 0x96);
+    }
+  }
+  wave_lds_sync();  // the next group's copy after these reads
+  return y;
+}
+
 template <int W32, int WL>
 struct RegWords {
   static_assert(W32 <= WL, "graph wider than the lane's parameter words");
   static constexpr bool kTab = W32 <= PM_TAB_W32_MAX;
+  static constexpr bool kTabNarrow = W32 <= PM_TAB_W32_MAX_NARROW;
   using Tab = typename std::conditional<kTab, TabBase, NoTab>::type;
   const uint32_t (&pv)[WL];
   __device__ __forceinline__ int width() const { return W32; }
   __device__ __forceinline__ uint32_t mxor(const cword *__restrict__ m) const {
     return masked_xor<W32, WL>(pv, m);
   }
+  __device__ __forceinline__ uint32_t operator()(int w) const { return pv[w]; }
   __device__ __forceinline__ void tab_init(Tab &t, uint32_t base) const {
     if constexpr (kTab) t.base = base;
   }
-  // parity word of the sign group whose tables start at `tabs` (16-byte
-  // aligned, pm_tab_dwords(W32) dwords)
   __device__ __forceinline__ uint32_t tab_parity(const cword *__restrict__ tabs,
                                                  const Tab &t) const {
-    if constexpr (kTab) {
-      constexpr int N4 = (int)pm_tab_dwords(W32) / 4;
-      const int lane = threadIdx.x & 63;
-      const cword4 *src = (const cword4 *)tabs;
-      lword4 *dst = (lword4 *)(uintptr_t)t.base;
-      // every load issued before the first store (one memory round trip)
-      constexpr int NL = (N4 + 63) / 64;
-      u32x4 tmp[NL];
-#pragma unroll
-      for (int i = 0; i < NL; ++i) {
-        const int e = i * 64 + lane;
-        if (N4 % 64 == 0 || e < N4) tmp[i] = src[e];
-      }
-#pragma unroll
-      for (int i = 0; i < NL; ++i) {
-        const int e = i * 64 + lane;
-        if (N4 % 64 == 0 || e < N4) dst[e] = tmp[i];
-      }
-      wave_lds_sync();
-      // read addresses: v_perm takes byte k of (word << 2) & 0x3c3c3c3c (the
-      // even nibbles times 4) or of (word >> 2) & 0x3c3c3c3c (the odd ones)
-      // into byte 0, under bytes 1 and 2 of the base (256-byte aligned,
-      // below 2^24)
-      // (the opaque copy of the base keeps the compiler from building all
-      // 8 W32 addresses once per graph, which spilled them)
-      uint32_t base = t.base;
-      asm volatile("" : "+s"(base));
-      uint32_t y = 0;
-#pragma unroll
-      for (int w = 0; w < W32; ++w) {
-        uint32_t x = pv[w];
-        asm volatile("" : "+v"(x));
-        const uint32_t pe = (x << 2) & 0x3c3c3c3cu, po = (x >> 2) & 0x3c3c3c3cu;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          const uint32_t sel = 0x0c060500u | (uint32_t)k;
-          const int j = 8 * w + 2 * k;
-          const uint32_t ae = __builtin_amdgcn_perm(base, pe, sel);
-          const uint32_t ao = __builtin_amdgcn_perm(base, po, sel);
-          const uint32_t u = ((const lword *)(uintptr_t)ae)[16 * j];
-          const uint32_t v = ((const lword *)(uintptr_t)ao)[16 * (j + 1)];
-          y = __builtin_amdgcn_bitop3_b32(u, v, y, 0x96);
-        }
-      }
-      wave_lds_sync();  // the next group's copy after these reads
-      return y;
-    } else {
+    if constexpr (kTab)
+      return tab_parity_words<W32>(*this, tabs, t.base);
+    else
       return 0;
-    }
   }
 };
+// The LDS-word kernels read tables too (a batch puts every program in the
+// kernel its widest member needs): their exchange area holds the buffer,
+// and a graph with tables has at most PM_TAB_W32_MAX words (a switch picks
+// the unrolled form).
 struct LdsWords {
-  static constexpr bool kTab = false;
-  using Tab = NoTab;
+  static constexpr bool kTab = true;
+  static constexpr bool kTabNarrow = true;
+  using Tab = TabBase;
   const uint32_t *pv;
   int W32;
   __device__ __forceinline__ int width() const { return W32; }
@@ -313,9 +325,17 @@ struct LdsWords {
     for (int w = 1; w < W32; ++w) t = __builtin_amdgcn_bitop3_b32(m[w], t, pv[64 * w], 0x6c);
     return t;
   }
-  __device__ __forceinline__ void tab_init(Tab &, uint32_t) const {}
-  __device__ __forceinline__ uint32_t tab_parity(const cword *__restrict__, const Tab &) const {
-    return 0;
+  __device__ __forceinline__ uint32_t operator()(int w) const { return pv[64 * w]; }
+  __device__ __forceinline__ void tab_init(Tab &t, uint32_t base) const { t.base = base; }
+  __device__ __forceinline__ uint32_t tab_parity(const cword *__restrict__ tabs,
+                                                 const Tab &t) const {
+    switch (W32) {
+      case 1: return tab_parity_words<1>(*this, tabs, t.base);
+      case 2: return tab_parity_words<2>(*this, tabs, t.base);
+      case 3: return tab_parity_words<3>(*this, tabs, t.base);
+      case 4: return tab_parity_words<4>(*this, tabs, t.base);
+      default: return 0;  // the validator admits no wider graph with tables
+    }
   }
 };
 
@@ -728,7 +748,7 @@ __device__ __forceinline__ void sum_terms(const cword *__restrict__ th, uint32_t
   typename PW::Tab tab;
   if constexpr (PW::kTab) {
     tab.base = NO_TBUF;
-    if constexpr (WIDE)
+    if constexpr (WIDE || PW::kTabNarrow)
       if (gflags & GF_TABLE) pw.tab_init(tab, tbuf);
   }
   for (uint32_t t = 0; t < nt; ++t) {
@@ -933,10 +953,11 @@ __device__ __forceinline__ double graph_prob(const cword *__restrict__ P,
   }
   uint32_t tbuf = NO_TBUF;
   if (gflags & GF_TABLE) {
-    // tables are read by the 128-bit register-word kernels only (the
-    // validator keeps them out of every other program); elsewhere the
-    // graph is flagged and its terms skipped, barriers still reached
-    tbuf = (WIDE && PW::kTab) ? table_buffer(sp, xs) : NO_TBUF;
+    // tables are read by the register-word kernels that hold a buffer for
+    // the graph's width (the validator keeps them out of every other
+    // program); elsewhere the graph is flagged and its terms skipped,
+    // barriers still reached
+    tbuf = (WIDE ? PW::kTab : PW::kTabNarrow) ? table_buffer(sp, xs) : NO_TBUF;
     if (tbuf == NO_TBUF) {
       err |= ERR_FORMAT;
       t1 = t0;

@@ -105,6 +105,11 @@ inline uint64_t pm_dwords(uint64_t n_pm, uint64_t W32) {
 // needs).  A term of such a graph keeps no PROJ or PPX records of its own:
 // they sit in the groups of its first chunk.
 constexpr int PM_TAB_W32_MAX = 4;
+// the kernels without the 128-bit path (8 waves per SIMD) hold a 1 KiB
+// buffer per wave: tables of graphs of at most 2 mask words, in programs of
+// at most PM_TAB_NARROW_WL lane words
+constexpr int PM_TAB_W32_MAX_NARROW = 2;
+constexpr int PM_TAB_NARROW_WL = 4;
 constexpr int PM_TAB_HDR = 8;
 constexpr uint32_t PM_COUNT_MASK = 0xffu, PM_PAD_SHIFT = 8;
 constexpr uint32_t pm_tab_dwords(uint32_t W32) { return 128 * W32; }
@@ -139,7 +144,8 @@ constexpr uint32_t GF_STREAM = 8u;  // the 128-bit kernels read the graph's reco
                                     // STREAM_PAD spare dwords
 constexpr uint32_t GF_TABLE = 16u;  // sign groups carry parity tables instead of
                                     // masks (only in programs that run the
-                                    // 128-bit register-word kernels)
+                                    // register-word kernels; layout.hpp's
+                                    // PM_TAB_* limits)
 
 // Record read-ahead of GF_STREAM graphs (the packer sets the flag on every
 // graph of several terms): each touch reaches STREAM_AHEAD dwords past the

@@ -650,9 +650,9 @@ void emit_group(const std::vector<Form> &f, uint32_t W32, std::vector<uint32_t> 
 
 }  // namespace
 
-bool tabulate_graph(std::vector<uint32_t> &words, uint32_t *ghdr) {
+bool tabulate_graph(std::vector<uint32_t> &words, uint32_t *ghdr, uint32_t w32max) {
   const uint32_t W32 = ghdr[GH_W32], nt = ghdr[GH_NTERMS];
-  if (W32 > (uint32_t)PM_TAB_W32_MAX || nt < 2 || (ghdr[GH_FLAGS] & (GF_APPROX | GF_MONO)))
+  if (W32 > w32max || nt < 2 || (ghdr[GH_FLAGS] & (GF_APPROX | GF_MONO)))
     return false;
   const std::vector<uint32_t> &w = words;
   std::vector<uint32_t> out;
@@ -848,8 +848,10 @@ std::vector<uint32_t> pack_program(const ksim_graph_view *graphs, const int32_t 
       if (j.err) std::rethrow_exception(j.err);
   }
   // parity tables replace the sign masks in the programs that run the
-  // 128-bit register-word kernels: some graph needs 128-bit sums, no
-  // approximate factors, at most REG_WORDS_MAX lane words
+  // register-word kernels (no approximate factors): graphs of up to
+  // PM_TAB_W32_MAX mask words where some graph needs 128-bit sums (at most
+  // REG_WORDS_MAX lane words), of up to PM_TAB_W32_MAX_NARROW where none
+  // does (at most PM_TAB_NARROW_WL lane words)
   {
     bool all_narrow = true, approx = false;
     int wl = 1;
@@ -859,13 +861,16 @@ std::vector<uint32_t> pack_program(const ksim_graph_view *graphs, const int32_t 
       if (!(j.hdr[GH_FLAGS] & (GF_MONO | GF_NARROW))) all_narrow = false;
       if (j.hdr[GH_FLAGS] & GF_APPROX) approx = true;
     }
-    if (!all_narrow && !approx && wl <= REG_WORDS_MAX) {
+    const uint32_t w32max = approx ? 0u
+                            : !all_narrow ? (wl <= REG_WORDS_MAX ? (uint32_t)PM_TAB_W32_MAX : 0u)
+                            : (wl <= PM_TAB_NARROW_WL ? (uint32_t)PM_TAB_W32_MAX_NARROW : 0u);
+    if (w32max) {
       std::atomic<size_t> next(0);
       auto work = [&] {
         for (;;) {
           const size_t i = next.fetch_add(1);
           if (i >= total_graphs) break;
-          tabulate_graph(jobs[i].words, jobs[i].hdr);
+          tabulate_graph(jobs[i].words, jobs[i].hdr, w32max);
         }
       };
       const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));

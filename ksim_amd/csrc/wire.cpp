@@ -65,6 +65,7 @@ void validate_words(const uint32_t *w, size_t n) {
   need((uint64_t)PH_DWORDS + nc, "component offsets");
   uint64_t outs = 0;
   bool all_narrow = true, any_approx = false, any_table = false;
+  uint32_t table_w32 = 0;  // the widest graph with parity tables
   for (uint32_t c = 0; c < nc; ++c) {
     const uint64_t co = w[PH_DWORDS + c];
     need(co + CH_DWORDS, "component header");
@@ -106,6 +107,7 @@ void validate_words(const uint32_t *w, size_t n) {
       const bool table = gh[GH_FLAGS] & GF_TABLE;
       if (table) {
         any_table = true;
+        table_w32 = std::max(table_w32, W32);
         if (W32 > (uint32_t)PM_TAB_W32_MAX) bad("parity tables on a graph wider than their limit");
         if (gh[GH_FLAGS] & (GF_MONO | GF_APPROX)) bad("parity tables on a single-term or approximate graph");
       }
@@ -153,9 +155,13 @@ void validate_words(const uint32_t *w, size_t n) {
     }
   }
   if (outs != tot) bad("total outputs");
-  // parity tables are read by the 128-bit register-word kernels only
-  if (any_table && (all_narrow || any_approx || wmax > (uint32_t)REG_WORDS_MAX))
-    bad("parity tables in a program that runs kernels reading masks only");
+  // parity tables are read by the register-word kernels that hold a buffer
+  // for them: graphs of up to PM_TAB_W32_MAX words in programs with 128-bit
+  // sums, of up to PM_TAB_W32_MAX_NARROW words in programs without
+  if (any_table && (any_approx || wmax > (uint32_t)REG_WORDS_MAX ||
+                    (all_narrow && (wmax > (uint32_t)PM_TAB_NARROW_WL ||
+                                    table_w32 > (uint32_t)PM_TAB_W32_MAX_NARROW))))
+    bad("parity tables in a program whose kernels cannot read them");
 }
 
 std::vector<uint32_t> words_from_blob(const uint8_t *blob, size_t n, double *stats8) {

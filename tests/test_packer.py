@@ -226,10 +226,19 @@ def test_parity_tables_evaluate_as_the_oracle(tables):
     assert check_program(prog, tables, rows=6, seed=7) > 0
 
 
-def test_parity_tables_only_where_the_128_bit_kernels_run():
-    # an all-64-bit program, or one whose component needs more than 8 lane
-    # words, keeps its masks: the kernels that run it read no tables
-    assert table_graphs(pack_words(synth.shape_program(n_q=20, t=12, F=24, n_out=8, seed=12))) == []
+def test_parity_tables_only_where_the_kernels_read_them(tables):
+    # the 64-bit kernels hold a 1 KiB buffer per wave: an all-64-bit program
+    # carries tables in its graphs of at most 2 mask words (and they
+    # evaluate as the oracle), none in wider graphs; a program whose
+    # component needs more than 8 lane words keeps every mask
+    narrow = synth.shape_program(n_q=20, t=12, F=24, n_out=8, seed=12)
+    words = pack_words(narrow)
+    w = [int(x) for x in words]
+    gb = w[w[PH_DWORDS] + CH["GRAPH"]]
+    tg = table_graphs(words)
+    assert tg and all(w[gb + i * GH_DWORDS + 1] <= 2 for i in tg)
+    assert check_program(narrow, tables, rows=4, seed=3) > 0
+    assert table_graphs(pack_words(synth.shape_program(n_q=20, t=12, F=70, n_out=8, seed=12))) == []
     a = synth.shape_program(n_q=22, t=6, F=12, n_out=22, seed=4062)
     assert table_graphs(pack_words(a))
     # the same component beside one of 257 parameters (9 lane words): the

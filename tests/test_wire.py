@@ -241,9 +241,9 @@ def first_table_chunk(w):
 
 def test_parity_group_blobs():
     # a program that runs the 128-bit kernels carries parity tables: its blob
-    # round-trips; tables off a 16-byte boundary, a padding word in a chunk
-    # without groups, tables in a program that runs the 64-bit kernels and
-    # tables on a graph wider than 4 mask words are each rejected
+    # round-trips; tables off a 16-byte boundary, tables on a graph wider
+    # than 4 mask words and tables in a program whose kernels hold no buffer
+    # for them are each rejected
     prog = synth.shape_program(n_q=22, t=6, F=12, n_out=22, seed=4062)
     blob = program_blob(prog)
     w = words_of(blob)
@@ -259,17 +259,21 @@ def test_parity_group_blobs():
     v[gh + 1] = 5  # W32 beyond the tables' limit (also the graph's own width)
     with pytest.raises(ValueError):
         blob_stats(with_words(blob, v))
-    # an all-64-bit program with a table flag on a multi-term graph
+    # an all-64-bit program carries tables only while its kernels hold a
+    # buffer: at most 4 lane words (the same words claiming 5 are rejected)
     narrow = program_blob(synth.shape_program(n_q=8, t=6, F=6, n_out=7, seed=3))
     u = words_of(narrow)
-    c0 = int(u[8])
-    g0, ng = int(u[c0 + 7]), int(u[c0])
-    multi = [g0 + 16 * i for i in range(ng) if int(u[g0 + 16 * i]) > 1]
-    assert multi
-    bad = u.copy()
-    bad[multi[0] + 5] |= np.uint32(16)
-    with pytest.raises(ValueError):
-        blob_stats(with_words(narrow, bad))
+    first_table_chunk(u)
+
+    def lane_words(b, k):
+        v = u.copy()
+        v[4] = k  # PH_WMAX, and the header's copy of it
+        out = bytearray(with_words(b, v))
+        out[112:116] = struct.pack("<I", k)
+        return bytes(out)
+    blob_stats(lane_words(narrow, 4))
+    with pytest.raises(ValueError, match="cannot read"):
+        blob_stats(lane_words(narrow, 5))
 
 
 def test_stream_flag_needs_its_padding():

@@ -242,21 +242,19 @@ __device__ __forceinline__ uint32_t tab_parity_words(const Word &pw, const cword
                                                      uint32_t tbase) {
   constexpr int N4 = (int)pm_tab_dwords(W32) / 4;
   const int lane = threadIdx.x & 63;
-  const cword4 *src = (const cword4 *)tabs;
-  lword4 *dst = (lword4 *)(uintptr_t)tbase;
-  // every load issued before the first store (one memory round trip)
+  // straight from global memory into the buffer (global_load_lds_dwordx4:
+  // lane l of a load fills 16 bytes at the base + 16 l), then one wait
   constexpr int NL = (N4 + 63) / 64;
-  u32x4 tmp[NL];
 #pragma unroll
   for (int i = 0; i < NL; ++i) {
     const int e = i * 64 + lane;
-    if (N4 % 64 == 0 || e < N4) tmp[i] = src[e];
+    if (N4 % 64 == 0 || e < N4)
+      __builtin_amdgcn_global_load_lds(
+          (const __attribute__((address_space(1))) void *)((uintptr_t)tabs + 16 * (uintptr_t)e),
+          (__attribute__((address_space(3))) void *)(uintptr_t)(tbase + 1024u * (uint32_t)i), 16,
+          0, 0);
   }
-#pragma unroll
-  for (int i = 0; i < NL; ++i) {
-    const int e = i * 64 + lane;
-    if (N4 % 64 == 0 || e < N4) dst[e] = tmp[i];
-  }
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   wave_lds_sync();
   // read addresses: v_perm takes byte k of (word << 2) & 0x3c3c3c3c (the
   // even nibbles times 4) or of (word >> 2) & 0x3c3c3c3c (the odd ones)

@@ -209,14 +209,13 @@ __device__ __forceinline__ uint32_t masked_xor(const uint32_t (&pv)[WL],
 // of the lane at pv[64 w] (a wave's words interleaved by lane, so the 64
 // lanes of one read hit 64 consecutive dwords), any width at run time.
 //
-// Parity tables (GF_TABLE graphs, at most PM_TAB_W32_MAX words): a sign
-// group's 8 W32 nibble tables are copied into the wave's 2 KiB LDS buffer
-// (one 16-byte load and store per lane and KiB) and the group's parity word
-// is the XOR of one LDS read per nibble, T_j[nibble j of p], instead of
-// W32 + 2 VALU ops per record.  Each read address is the buffer base | 4 v
-// for nibble value v (one v_perm), table j at the read's immediate offset
-// 64 j.  (Building the 8 W32 addresses once per graph instead spilled them
-// in the 128-bit kernels.)
+// Parity tables (GF_TABLE graphs, at most PM_TAB_W32_MAX words): a parity
+// group's 8 W32 nibble tables are copied into the wave's LDS buffer (2 KiB,
+// 1 KiB in the plain kernel without the 128-bit path; one 16-byte
+// global-to-LDS load per lane and KiB) and the group's parity word is the
+// XOR of one LDS read per nibble, T_j[nibble j of p], instead of W32 + 2
+// VALU ops per record.  Each read address is the buffer base | 4 v for
+// nibble value v (one v_perm), table j at the read's immediate offset 64 j.
 constexpr uint32_t NO_TBUF = 0xffffffffu;  // no table buffer: the groups hold masks
 struct TabBase {
   uint32_t base;  // the wave's table buffer (LDS byte address), or NO_TBUF
@@ -225,9 +224,10 @@ struct NoTab {
   static constexpr uint32_t base = NO_TBUF;
 };
 
-// wave-local order between the lanes' LDS stores and the reads of other
-// lanes' data (a wave's LDS operations complete in issue order; the fences
-// keep the compiler from moving memory operations across)
+// wave-local order between LDS writes and the reads of other lanes' data
+// (a wave's LDS operations complete in issue order, the global-to-LDS
+// loads once vmcnt says so; the fences keep the compiler from moving
+// memory operations across)
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();

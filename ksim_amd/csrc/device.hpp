@@ -284,11 +284,56 @@ __device__ __forceinline__ uint32_t tab_parity_words(const Word &pw, const cword
   return y;
 }
 
+// Parity words of two groups of at most 2 mask words (tables of 1 KiB or
+// less each, the second at +1 KiB of a 2 KiB buffer): one set of read
+// addresses serves both (the second group's reads take 1 KiB more in their
+// immediate offsets).
+template <int W32, class Word>
+__device__ __forceinline__ void tab_parity2_words(const Word &pw, const cword *__restrict__ t1,
+                                                  const cword *__restrict__ t2, uint32_t tbase,
+                                                  uint32_t &y1, uint32_t &y2) {
+  static_assert(W32 <= 2, "two groups' tables fill the buffer at 2 mask words");
+  constexpr int N4 = (int)pm_tab_dwords(W32) / 4;
+  const int lane = threadIdx.x & 63;
+  if (N4 % 64 == 0 || lane < N4) {
+    __builtin_amdgcn_global_load_lds(
+        (const __attribute__((address_space(1))) void *)((uintptr_t)t1 + 16 * (uintptr_t)lane),
+        (__attribute__((address_space(3))) void *)(uintptr_t)tbase, 16, 0, 0);
+    __builtin_amdgcn_global_load_lds(
+        (const __attribute__((address_space(1))) void *)((uintptr_t)t2 + 16 * (uintptr_t)lane),
+        (__attribute__((address_space(3))) void *)(uintptr_t)(tbase + 1024u), 16, 0, 0);
+  }
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  wave_lds_sync();
+  uint32_t base = tbase;
+  asm volatile("" : "+s"(base));
+  uint32_t a = 0, b = 0;
+#pragma unroll
+  for (int w = 0; w < W32; ++w) {
+    uint32_t x = pw(w);
+    asm volatile("" : "+v"(x));
+    const uint32_t pe = (x << 2) & 0x3c3c3c3cu, po = (x >> 2) & 0x3c3c3c3cu;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint32_t sel = 0x0c060500u | (uint32_t)k;
+      const int j = 8 * w + 2 * k;
+      const lword *ae = (const lword *)(uintptr_t)__builtin_amdgcn_perm(base, pe, sel);
+      const lword *ao = (const lword *)(uintptr_t)__builtin_amdgcn_perm(base, po, sel);
+      a = __builtin_amdgcn_bitop3_b32(ae[16 * j], ao[16 * (j + 1)], a, 0x96);
+      b = __builtin_amdgcn_bitop3_b32(ae[16 * j + 256], ao[16 * (j + 1) + 256], b, 0x96);
+    }
+  }
+  wave_lds_sync();  // the next groups' copies after these reads
+  y1 = a;
+  y2 = b;
+}
+
 template <int W32, int WL>
 struct RegWords {
   static_assert(W32 <= WL, "graph wider than the lane's parameter words");
   static constexpr bool kTab = W32 <= PM_TAB_W32_MAX;
   static constexpr bool kTabNarrow = W32 <= PM_TAB_W32_MAX_NARROW;
+  static constexpr bool kTab2 = W32 <= 2;  // two groups per 2 KiB buffer
   using Tab = typename std::conditional<kTab, TabBase, NoTab>::type;
   const uint32_t (&pv)[WL];
   __device__ __forceinline__ int width() const { return W32; }
@@ -306,6 +351,11 @@ struct RegWords {
     else
       return 0;
   }
+  __device__ __forceinline__ void tab_parity2(const cword *__restrict__ t1,
+                                              const cword *__restrict__ t2, const Tab &t,
+                                              uint32_t &y1, uint32_t &y2) const {
+    if constexpr (kTab2) tab_parity2_words<W32>(*this, t1, t2, t.base, y1, y2);
+  }
 };
 // The LDS-word kernels read tables too (a batch puts every program in the
 // kernel its widest member needs): their exchange area holds the buffer,
@@ -314,6 +364,7 @@ struct RegWords {
 struct LdsWords {
   static constexpr bool kTab = true;
   static constexpr bool kTabNarrow = true;
+  static constexpr bool kTab2 = false;
   using Tab = TabBase;
   const uint32_t *pv;
   int W32;
@@ -335,6 +386,8 @@ struct LdsWords {
       default: return 0;  // the validator admits no wider graph with tables
     }
   }
+  __device__ __forceinline__ void tab_parity2(const cword *__restrict__, const cword *__restrict__,
+                                              const Tab &, uint32_t &, uint32_t &) const {}
 };
 
 // Graphs with approximate floatfactors (GF_APPROX) are evaluated only by
@@ -435,16 +488,13 @@ __device__ __forceinline__ void term_exps(const cword *__restrict__ th, const PW
     if (tab.base != NO_TBUF) {
       // npm parity groups, each a header [S+, S-, N+, N-, X, Xc, J, Jc] and
       // its nibble tables: the parity word y holds one bit per form; sign
-      // records step S and N as below, PPX pairs (psi at bit 2m, phi at
-      // 2m + 1, both marked in X at 2m) add the parity of
-      // popcount(z & (z >> 1) & X) with z = y ^ Xc to the PPX sign, and a
-      // PROJ at a bit of J kills the term unless its parity is Jc's bit
-      for (uint32_t g = 0; g < npm; ++g) {
-        const uint32_t sp = rp[0], sm = rp[1], tp = rp[2], tm = rp[3];
-        const uint32_t xm = rp[4], xc = rp[5], jm = rp[6], jc = rp[7];
-        rp += PM_TAB_HDR;
-        const uint32_t y = pw.tab_parity(rp, tab);
-        rp += pm_tab_dwords(W32);
+      // records step S and N as below, PPX pairs (psi at a bit b marked in
+      // X, phi at b + 1) add the parity of popcount(z & (z >> 1) & X) with
+      // z = y ^ Xc to the PPX sign, and a PROJ at a bit of J kills the term
+      // unless its parity is Jc's bit
+      auto apply = [&](const cword *__restrict__ h, uint32_t y) {
+        const uint32_t sp = h[0], sm = h[1], tp = h[2], tm = h[3];
+        const uint32_t xm = h[4], xc = h[5], jm = h[6], jc = h[7];
         const uint32_t cs = __popc((y & sp) | (~y & sm));
         const uint32_t cn = __popc((y & tp) | (~y & tm));
         uint32_t sn;
@@ -455,7 +505,20 @@ __device__ __forceinline__ void term_exps(const cword *__restrict__ th, const PW
           ex += __popc(z & (z >> 1) & xm);
           jd |= (y ^ jc) & jm;
         }
+      };
+      const uint32_t gd = pm_tab_group_dwords(W32);
+      uint32_t g = 0;
+      if constexpr (LONG && PW::kTab2) {
+        // the 128-bit kernels' 2 KiB buffers take two groups of up to 2
+        // mask words at a time
+        for (; g + 1 < npm; g += 2, rp += 2 * gd) {
+          uint32_t y1, y2;
+          pw.tab_parity2(rp + PM_TAB_HDR, rp + gd + PM_TAB_HDR, tab, y1, y2);
+          apply(rp, y1);
+          apply(rp + gd, y2);
+        }
       }
+      for (; g < npm; ++g, rp += gd) apply(rp, pw.tab_parity(rp + PM_TAB_HDR, tab));
     } else {
       // sign records (S += 2s, N += t, s, t in {-1, 0, 1}): the parities of
       // a group go into one word, record j of n at bit 32 - n + j (each

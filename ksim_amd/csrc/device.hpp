@@ -492,18 +492,24 @@ __device__ __forceinline__ void term_exps(const cword *__restrict__ th, const PW
       // X, phi at b + 1) add the parity of popcount(z & (z >> 1) & X) with
       // z = y ^ Xc to the PPX sign, and a PROJ at a bit of J kills the term
       // unless its parity is Jc's bit
-      auto apply = [&](const cword *__restrict__ h, uint32_t y) {
-        const uint32_t sp = h[0], sm = h[1], tp = h[2], tm = h[3];
-        const uint32_t xm = h[4], xc = h[5], jm = h[6], jc = h[7];
-        const uint32_t cs = __popc((y & sp) | (~y & sm));
-        const uint32_t cn = __popc((y & tp) | (~y & tm));
+      // (each header is read before the group's tables, so its scalar
+      // loads overlap the table copy)
+      struct Hdr {
+        uint32_t sp, sm, tp, tm, xm, xc, jm, jc;
+      };
+      auto header = [](const cword *__restrict__ h) {
+        return Hdr{h[0], h[1], h[2], h[3], h[4], h[5], h[6], h[7]};
+      };
+      auto apply = [&](const Hdr &h, uint32_t y) {
+        const uint32_t cs = __popc((y & h.sp) | (~y & h.sm));
+        const uint32_t cn = __popc((y & h.tp) | (~y & h.tm));
         uint32_t sn;
         asm("v_lshl_add_u32 %0, %1, 21, %2" : "=v"(sn) : "v"(cs), "v"(cn));
         asm("v_lshl_add_u32 %0, %1, 9, %0" : "+v"(x) : "v"(sn));
-        if (xm | jm) {  // wave-uniform
-          const uint32_t z = y ^ xc;
-          ex += __popc(z & (z >> 1) & xm);
-          jd |= (y ^ jc) & jm;
+        if (h.xm | h.jm) {  // wave-uniform
+          const uint32_t z = y ^ h.xc;
+          ex += __popc(z & (z >> 1) & h.xm);
+          jd |= (y ^ h.jc) & h.jm;
         }
       };
       const uint32_t gd = pm_tab_group_dwords(W32);
@@ -512,13 +518,17 @@ __device__ __forceinline__ void term_exps(const cword *__restrict__ th, const PW
         // the 128-bit kernels' 2 KiB buffers take two groups of up to 2
         // mask words at a time
         for (; g + 1 < npm; g += 2, rp += 2 * gd) {
+          const Hdr h1 = header(rp), h2 = header(rp + gd);
           uint32_t y1, y2;
           pw.tab_parity2(rp + PM_TAB_HDR, rp + gd + PM_TAB_HDR, tab, y1, y2);
-          apply(rp, y1);
-          apply(rp + gd, y2);
+          apply(h1, y1);
+          apply(h2, y2);
         }
       }
-      for (; g < npm; ++g, rp += gd) apply(rp, pw.tab_parity(rp + PM_TAB_HDR, tab));
+      for (; g < npm; ++g, rp += gd) {
+        const Hdr h = header(rp);
+        apply(h, pw.tab_parity(rp + PM_TAB_HDR, tab));
+      }
     } else {
       // sign records (S += 2s, N += t, s, t in {-1, 0, 1}): the parities of
       // a group go into one word, record j of n at bit 32 - n + j (each

@@ -76,6 +76,19 @@ def test_version_3_blobs_still_load(prog, blob):
     assert blob_stats(bytes(b))["words"] == blob_stats(LEGACY["two_comp"][1])["words"]
 
 
+def test_version_5_blobs_still_load():
+    # version 5 is version 6 without parity tables: a mask-form blob under
+    # its number loads and keeps its words
+    b = bytearray(program_blob(synth.shape_program(n_q=20, t=12, F=70, n_out=8, seed=12)))
+    from tests.devfmt import CH, GF_TABLE, GH_DWORDS, GH_FLAGS, PH_DWORDS
+    w = words_of(bytes(b))
+    c0 = int(w[PH_DWORDS])
+    g0, ng = int(w[c0 + CH["GRAPH"]]), int(w[c0 + CH["NGRAPHS"]])
+    assert not any(int(w[g0 + GH_DWORDS * i + GH_FLAGS]) & GF_TABLE for i in range(ng))
+    b[8:12] = struct.pack("<I", 5)
+    assert blob_stats(bytes(b))["words"] == len(w)
+
+
 @pytest.mark.parametrize("name", list(LEGACY))
 def test_version_4_blobs_evaluate_as_the_oracle(name):
     # blobs written by the version-4 packer: every LIN record with its own
